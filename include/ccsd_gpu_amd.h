@@ -1,0 +1,73 @@
+/*
+ * ccsd_gpu_amd -- C ABI of the MI355X (gfx950) CCSD-gradient hot path.
+ *
+ * All device pointers are HIP device memory; `stream` is a hipStream_t (NULL =
+ * the null stream). Every entry returns 0 on success and a negative code on
+ * failure (-1 null pointer, -2 bad shape, -3 unsupported input, -4 HIP error,
+ * -5 allocation failure); ccsd_last_error() then describes the failure
+ * (thread-local). Calls are stream-ordered; a plan must not be used from two
+ * host threads at once.
+ *
+ * Integral layout (what the reference's wrappers return):
+ *   out[comp][i - ao_loc[ish0]][j][k*(k+1)/2 + l],  k >= l,
+ *   shape (ncomp, nf, nao, nao*(nao+1)/2), ncomp = 1 (int2e) or 3 (int2e_ip1,
+ *   x, y, z of d/dr1), AO order of the native basis (Cartesian or spherical).
+ */
+#ifndef CCSD_GPU_AMD_H
+#define CCSD_GPU_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ccsd_plan ccsd_plan;
+
+/* Text of the last error on this host thread. */
+const char* ccsd_last_error(void);
+
+/* Basis plan from libcint-layout host arrays (the `mol._atm`, `mol._bas`,
+ * `mol._env` the reference hands to its kernels, ccsd_gpu/cuda/int2e_ip1_module.py:273-284,
+ * uploaded per call at ccsd_gpu/cuda/int2e_ip1_gpu.py:181-182) and, optionally,
+ * `mol.cart2sph_coeff()` as a row-major (nao_cart, nao_sph) host matrix
+ * (ccsd_gpu/cuda/int2e_ip1_gpu.py:243). Replaces the reference's per-call
+ * topology cache `_get_basis_cached` (ccsd_gpu/cuda/int2e_ip1_gpu.py:85-141). */
+int ccsd_plan_create(ccsd_plan** plan, const int* atm, int natm, const int* bas, int nbas, const double* env,
+                     int nenv, const double* c2s);
+int ccsd_plan_destroy(ccsd_plan* plan);
+
+/* New geometry (same basis): replaces env; coordinates are read from it. */
+int ccsd_plan_set_env(ccsd_plan* plan, const double* env, int nenv);
+
+/* Number of AOs (cart != 0: Cartesian). Returns the count or a negative code. */
+int ccsd_plan_nao(const ccsd_plan* plan, int cart);
+
+/* int2e, s2kl, i-shells [ish0, ish1): the drop-in for
+ * compute_int2e_gpu(mol, b0, b1) (ccsd_gpu/cuda/int2e_gpu.py:186-215), i.e.
+ * mol.intor('int2e', aosym='s2kl', shls_slice=(b0,b1,0,nbas,0,nbas,0,nbas)).
+ * cart = 1: Cartesian AOs; cart = 0: spherical (needs c2s at plan creation).
+ * out: device array (nf, nao, nao_pair), fully overwritten. */
+int ccsd_int2e_s2kl(ccsd_plan* plan, double* out, int ish0, int ish1, int cart, void* stream);
+
+/* int2e_ip1, s2kl: the drop-in for compute_int2e_ip1_gpu(mol, b0, b1)
+ * (ccsd_gpu/cuda/int2e_ip1_gpu.py:296-326), kernel fill_int2e_ip1_kernel
+ * (ccsd_gpu/cuda/int2e_ip1_module.py:273-667). out: (3, nf, nao, nao_pair). */
+int ccsd_int2e_ip1_s2kl(ccsd_plan* plan, double* out, int ish0, int ish1, int cart, void* stream);
+
+/* Fused ERI-derivative contraction for i-shells [ish0, ish1):
+ *   de[atom(i)][x] += scale * sum_{i, j, k>=l} int2e_ip1[x](i j | k l) * gamma[i][j][k(k+1)/2 + l]
+ * with gamma in the Cartesian s2kl layout (nao_cart, nao_cart, nao_cart*(nao_cart+1)/2).
+ * With scale = -2 and gamma the packed 2-RDM rows whose kl diagonal is halved this
+ * is the reference's per-block `de -= 2 * eri1 @ dm2buf` (ccsd_gpu/gpu_ccsd_grad.py:430-459)
+ * without storing eri1. de: device (natm, 3), accumulated into. */
+int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double scale, int ish0, int ish1,
+                        void* stream);
+
+/* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
+ * arrays; r, w are (count, n)). Counterpart of rys_roots_simple
+ * (ccsd_gpu/cuda/int2e_ip1_module.py:192-244), n = 1..9. */
+int ccsd_rys_roots(int n, const double* T, int count, double* r, double* w, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
