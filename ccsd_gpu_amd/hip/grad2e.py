@@ -1,0 +1,63 @@
+"""Fused ERI-derivative contraction: sum_ijkl d(ij|kl)/dR * Gamma without storing d(ij|kl).
+
+The reference forms the derivative integrals of one shell block in HBM and
+contracts them with a GEMV (`de_k -= eri1.reshape(3, -1) @ dm2buf.reshape(-1) * 2`,
+ccsd_gpu/gpu_ccsd_grad.py:449-459). Here the contraction happens inside the
+int2e_ip1 kernel: every wave dots its quartet's derivative integrals with the
+matching Gamma entries and adds three numbers into dE/dR.
+
+Gamma is given in the Cartesian AO basis in the s2kl layout
+(nao_c, nao_c, nao_c (nao_c+1)/2): gamma[i, j, k(k+1)/2 + l], k >= l. A
+spherical-basis density is brought there with `sph_to_cart_s2kl` (the full
+contraction is basis independent: sum_sph X_sph G_sph = sum_cart X_cart (C G)).
+"""
+import torch
+
+from .. import _lib
+
+
+def contract_int2e_ip1(mol, gamma_cart, scale=-2.0, b0=0, b1=None, de=None):
+    """de[atom(i), x] += scale * sum_{i in [b0,b1), j, k>=l} ip1_x(ij|kl) gamma[i, j, kl].
+
+    Returns de as a (natm, 3) float64 tensor on the current device."""
+    lib = _lib.load()
+    b1 = mol.nbas if b1 is None else b1
+    if not (0 <= b0 < b1 <= mol.nbas):
+        raise ValueError("bad shell range")
+    nao_c = mol.nao_cart()
+    shape = (nao_c, nao_c, nao_c * (nao_c + 1) // 2)
+    if tuple(gamma_cart.shape) != shape or gamma_cart.dtype != torch.float64 or not gamma_cart.is_contiguous() \
+            or not gamma_cart.is_cuda:
+        raise ValueError(f"gamma_cart must be a contiguous float64 device tensor of shape {shape}")
+    if de is None:
+        de = torch.zeros((mol.natm, 3), dtype=torch.float64, device=gamma_cart.device)
+    elif tuple(de.shape) != (mol.natm, 3) or de.dtype != torch.float64 or not de.is_contiguous():
+        raise ValueError("de must be a contiguous float64 (natm, 3) tensor")
+    plan = _lib.get_plan(mol)
+    _lib.check(lib.ccsd_grad_ip1_gamma(plan, _lib.ptr(de), _lib.ptr(gamma_cart), float(scale), b0, b1,
+                                       _lib.current_stream_ptr()), "ccsd_grad_ip1_gamma")
+    return de
+
+
+def tril_pairs(n, device):
+    k, l = torch.tril_indices(n, n, device=device)
+    return k, l
+
+
+def pack_s2kl(full, halve_diag=False):
+    """(n, n, n, n) -> (n, n, n(n+1)/2) lower-triangle pairs of the last two indices."""
+    n = full.shape[-1]
+    k, l = tril_pairs(n, full.device)
+    out = full[..., k, l].contiguous()
+    if halve_diag:
+        out[..., k == l] *= 0.5
+    return out
+
+
+def sph_to_cart_4index(g_sph, c2s):
+    """Gamma_cart = (C x C x C x C) Gamma_sph with C = cart2sph_coeff (nao_c, nao_s)."""
+    t = torch.tensordot(c2s, g_sph, dims=([1], [0]))                     # (c, s, s, s)
+    t = torch.tensordot(c2s, t, dims=([1], [1])).transpose(0, 1)         # (c, c, s, s)
+    t = torch.tensordot(t, c2s, dims=([2], [1])).permute(0, 1, 3, 2)     # (c, c, c, s)
+    t = torch.tensordot(t, c2s, dims=([3], [1]))                         # (c, c, c, c)
+    return t.contiguous()
