@@ -34,6 +34,9 @@ def lib():
         L.oracle_rys_roots.restype = None
         L.oracle_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_int1e_cart.restype = ctypes.c_int
+        L.oracle_int1e_cart.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -121,3 +124,35 @@ def ip1_full(mol):
     """(3, nao, nao, nao, nao) d/dr1 ERIs in the native basis."""
     n = mol.nao_nr()
     return _unpack_kl(intor_int2e_ip1(mol), n)
+
+
+INT1E_KINDS = {"ovlp": 0, "kin": 1, "nuc": 2, "ipovlp": 3, "ipkin": 4, "ipnuc_atom": 5}
+
+
+def int1e(mol, kind):
+    """One-electron integrals in the native AO basis (PySCF `int1e_*` conventions).
+
+    ovlp / kin / nuc: (nao, nao); ipovlp / ipkin: (3, nao, nao) = <d/dr i| O |j>;
+    ipnuc_atom: (natm, 3, nao, nao) = <d/dr i| -Z_C/|r-C| |j> for every nucleus C
+    (its sum over C is PySCF's int1e_ipnuc; row C times -1/Z_C is int1e_iprinv at C).
+    """
+    k = INT1E_KINDS[kind]
+    atm = np.ascontiguousarray(mol._atm, dtype=np.int32)
+    bas = np.ascontiguousarray(mol._bas, dtype=np.int32)
+    env = np.ascontiguousarray(mol._env, dtype=np.float64)
+    nc = mol.nao_cart()
+    nmat = {0: 1, 1: 1, 2: 1, 3: 3, 4: 3, 5: 3 * mol.natm}[k]
+    out = np.zeros((nmat, nc, nc))
+    rc = lib().oracle_int1e_cart(out.ctypes.data, k, atm.ctypes.data, mol.natm, bas.ctypes.data, mol.nbas,
+                                 env.ctypes.data)
+    if rc != 0:
+        raise ValueError(f"oracle_int1e_cart failed with code {rc}")
+    if not mol.cart:
+        C = mol.cart2sph_coeff()
+        out = np.einsum("mij,ia,jb->mab", out, C, C, optimize=True)
+    if k <= 2:
+        return out[0]
+    if k <= 4:
+        return out
+    n = out.shape[-1]
+    return out.reshape(mol.natm, 3, n, n)
