@@ -1,0 +1,253 @@
+"""Closed-shell CCSD on the GPU: MO integrals and the T-amplitude solve.
+
+Producer of (t1, t2, eris) for the lambda / gradient path. The reference takes
+these from gpu4pyscf or PySCF (examples/h2o_ccsd_grad.py:128-151,
+ccsd_gpu/gpu_ccsd_lambda.py:352-353). MO integrals follow PySCF's
+_ChemistsERIs layout (chemist's notation blocks oooo, ovoo, ovvo, oovv, ovov,
+ovvv, vvvv, fock, mo_energy); the amplitude equations are the standard
+closed-shell spin-adapted CCSD equations with Fov/Foo/Fvv, Loo/Lvv and
+W-intermediates. Tensors are float64 torch tensors on the HIP device.
+"""
+import numpy as np
+import torch
+
+from .scf import DEV
+
+E = torch.einsum
+
+
+class ERIs:
+    """MO integrals on the device (from the Cartesian AO ERIs of the RHF object)."""
+
+    def __init__(self, mf, mo_coeff=None):
+        ints = mf._ints()
+        c = torch.as_tensor(np.asarray(mf.mo_coeff if mo_coeff is None else mo_coeff), dtype=torch.float64,
+                            device=DEV)
+        cc = ints.c2s @ c                                   # Cartesian AO -> MO
+        nocc = int((mf.mo_occ > 0).sum())
+        self.nocc = nocc
+        eri = torch.tensordot(ints.eri_c, cc, dims=([3], [0]))              # (p q r l)
+        eri = torch.tensordot(eri, cc, dims=([2], [0])).permute(0, 1, 3, 2)  # (p q k l)
+        eri = torch.tensordot(cc, eri, dims=([0], [1]))                      # (j p k l)
+        eri = torch.tensordot(cc, eri, dims=([0], [1]))                      # (i j k l)
+        eri = eri.contiguous()
+        dm = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+        j, k = ints.get_jk(dm)
+        fock_ao = ints.T + ints.V + j - 0.5 * k
+        self.fock = c.T @ fock_ao @ c
+        self.mo_energy = torch.diagonal(self.fock).clone()
+        o, v = slice(0, nocc), slice(nocc, None)
+        self.oooo = eri[o, o, o, o].contiguous()
+        self.ovoo = eri[o, v, o, o].contiguous()
+        self.ovvo = eri[o, v, v, o].contiguous()
+        self.oovv = eri[o, o, v, v].contiguous()
+        self.ovov = eri[o, v, o, v].contiguous()
+        self.ovvv = eri[o, v, v, v].contiguous()
+        self.vvvv = eri[v, v, v, v].contiguous()
+        del eri
+
+    def get_ovvv(self, *slices):
+        return self.ovvv[slices] if slices else self.ovvv
+
+
+def ccsd_energy(t1, t2, eris):
+    nocc = t1.shape[0]
+    e = 2 * torch.sum(eris.fock[:nocc, nocc:] * t1)
+    tau = t2 + E("ia,jb->ijab", t1, t1)
+    e = e + 2 * E("ijab,iajb->", tau, eris.ovov) - E("ijab,ibja->", tau, eris.ovov)
+    return float(e)
+
+
+def update_amps(t1, t2, eris):
+    nocc, nvir = t1.shape
+    fock = eris.fock
+    e_o, e_v = eris.mo_energy[:nocc], eris.mo_energy[nocc:]
+    fov, foo, fvv = fock[:nocc, nocc:], fock[:nocc, :nocc], fock[nocc:, nocc:]
+    ovov, ovoo, ovvo, oovv, ovvv = eris.ovov, eris.ovoo, eris.ovvo, eris.oovv, eris.ovvv
+    t1t1 = E("ic,ld->icld", t1, t1)
+    Foo = foo + 2 * E("kcld,ilcd->ki", ovov, t2) - E("kdlc,ilcd->ki", ovov, t2) \
+        + 2 * E("kcld,icld->ki", ovov, t1t1) - E("kdlc,icld->ki", ovov, t1t1)
+    Fvv = fvv - 2 * E("kcld,klad->ac", ovov, t2) + E("kdlc,klad->ac", ovov, t2) \
+        - 2 * E("kcld,kald->ac", ovov, t1t1) + E("kdlc,kald->ac", ovov, t1t1)
+    Fov = fov + 2 * E("kcld,ld->kc", ovov, t1) - E("kdlc,ld->kc", ovov, t1)
+    Loo = Foo + E("kc,ic->ki", fov, t1) + 2 * E("lcki,lc->ki", ovoo, t1) - E("kcli,lc->ki", ovoo, t1)
+    Lvv = Fvv - E("kc,ka->ac", fov, t1) + 2 * E("kdac,kd->ac", ovvv, t1) - E("kcad,kd->ac", ovvv, t1)
+    Foo = Foo - torch.diag(e_o)
+    Fvv = Fvv - torch.diag(e_v)
+    Loo = Loo - torch.diag(e_o)
+    Lvv = Lvv - torch.diag(e_v)
+
+    t1n = -2 * E("kc,ka,ic->ia", fov, t1, t1)
+    t1n = t1n + E("ac,ic->ia", Fvv, t1) - E("ki,ka->ia", Foo, t1)
+    t1n = t1n + 2 * E("kc,kica->ia", Fov, t2) - E("kc,ikca->ia", Fov, t2)
+    t1n = t1n + E("kc,ic,ka->ia", Fov, t1, t1) + fov
+    t1n = t1n + 2 * E("kcai,kc->ia", ovvo, t1) - E("kiac,kc->ia", oovv, t1)
+    t1n = t1n + 2 * E("kdac,ikcd->ia", ovvv, t2) - E("kcad,ikcd->ia", ovvv, t2)
+    tmp = E("kd,ic->kdic", t1, t1)
+    t1n = t1n + 2 * E("kdac,kdic->ia", ovvv, tmp) - E("kcad,kdic->ia", ovvv, tmp)
+    t1n = t1n - 2 * E("lcki,klac->ia", ovoo, t2) + E("kcli,klac->ia", ovoo, t2)
+    tmp = E("lc,ka->lcka", t1, t1)
+    t1n = t1n - 2 * E("lcki,lcka->ia", ovoo, tmp) + E("kcli,lcka->ia", ovoo, tmp)
+
+    tmp2 = E("kibc,ka->abic", oovv, -t1) + ovvv.permute(1, 3, 0, 2)
+    tmp = E("abic,jc->ijab", tmp2, t1)
+    t2n = tmp + tmp.permute(1, 0, 3, 2)
+    tmp2 = E("kcai,jc->akij", ovvo, t1) + ovoo.permute(1, 3, 0, 2)
+    tmp = E("akij,kb->ijab", tmp2, t1)
+    t2n = t2n - (tmp + tmp.permute(1, 0, 3, 2))
+    t2n = t2n + ovov.permute(0, 2, 1, 3)
+
+    Woooo = E("lcki,jc->klij", ovoo, t1) + E("kclj,ic->klij", ovoo, t1) + E("kcld,ijcd->klij", ovov, t2) \
+        + E("kcld,icjd->klij", ovov, E("ic,jd->icjd", t1, t1)) + eris.oooo.permute(0, 2, 1, 3)
+    Wvvvv = -E("kdac,kb->abcd", ovvv, t1) - E("kcbd,ka->abcd", ovvv, t1) + eris.vvvv.permute(0, 2, 1, 3)
+    Wvoov = E("kcad,id->akic", ovvv, t1) - E("kcli,la->akic", ovoo, t1) + ovvo.permute(2, 0, 3, 1) \
+        - 0.5 * E("ldkc,ilda->akic", ovov, t2) - 0.5 * E("lckd,ilad->akic", ovov, t2) \
+        - E("ldkc,idla->akic", ovov, E("id,la->idla", t1, t1)) + E("ldkc,ilad->akic", ovov, t2)
+    Wvovo = E("kdac,id->akci", ovvv, t1) - E("lcki,la->akci", ovoo, t1) + oovv.permute(2, 0, 3, 1) \
+        - 0.5 * E("lckd,ilda->akci", ovov, t2) - E("lckd,idla->akci", ovov, E("id,la->idla", t1, t1))
+
+    tau = t2 + E("ia,jb->ijab", t1, t1)
+    t2n = t2n + E("klij,klab->ijab", Woooo, tau)
+    t2n = t2n + E("abcd,ijcd->ijab", Wvvvv, tau)
+    del Wvvvv
+    tmp = E("ac,ijcb->ijab", Lvv, t2)
+    t2n = t2n + tmp + tmp.permute(1, 0, 3, 2)
+    tmp = E("ki,kjab->ijab", Loo, t2)
+    t2n = t2n - (tmp + tmp.permute(1, 0, 3, 2))
+    tmp = 2 * E("akic,kjcb->ijab", Wvoov, t2) - E("akci,kjcb->ijab", Wvovo, t2)
+    t2n = t2n + tmp + tmp.permute(1, 0, 3, 2)
+    tmp = E("akic,kjbc->ijab", Wvoov, t2)
+    t2n = t2n - (tmp + tmp.permute(1, 0, 3, 2))
+    tmp = E("bkci,kjac->ijab", Wvovo, t2)
+    t2n = t2n - (tmp + tmp.permute(1, 0, 3, 2))
+
+    eia = e_o[:, None] - e_v
+    t1n = t1n / eia
+    t2n = t2n / (eia[:, None, :, None] + eia[None, :, None, :])
+    return t1n, t2n
+
+
+class DIIS:
+    """Pulay DIIS on device vectors; error = difference of successive vectors."""
+
+    def __init__(self, space=6):
+        self.space = space
+        self.xs, self.errs = [], []
+        self.prev = None
+
+    def update(self, x):
+        if self.prev is None:
+            self.prev = x
+            return x
+        self.xs.append(x)
+        self.errs.append(x - self.prev)
+        self.xs, self.errs = self.xs[-self.space:], self.errs[-self.space:]
+        n = len(self.xs)
+        B = -torch.ones((n + 1, n + 1), dtype=torch.float64)
+        B[n, n] = 0
+        for i in range(n):
+            for j in range(i, n):
+                B[i, j] = B[j, i] = float(torch.dot(self.errs[i], self.errs[j]))
+        rhs = torch.zeros(n + 1, dtype=torch.float64)
+        rhs[n] = -1
+        c = torch.linalg.lstsq(B, rhs.unsqueeze(1)).solution[:n, 0]
+        out = sum(float(ci) * xi for ci, xi in zip(c, self.xs))
+        self.prev = out
+        return out
+
+
+def amplitudes_to_vector(t1, t2):
+    return torch.cat([t1.reshape(-1), t2.reshape(-1)])
+
+
+def vector_to_amplitudes(v, nocc, nvir):
+    n1 = nocc * nvir
+    return v[:n1].reshape(nocc, nvir), v[n1:].reshape(nocc, nocc, nvir, nvir)
+
+
+class CCSD:
+    """Closed-shell CCSD with the attributes the reference's lambda/gradient code reads."""
+
+    def __init__(self, mf, frozen=None):
+        if frozen:
+            raise NotImplementedError("frozen orbitals are not supported")
+        self._scf = mf
+        self.mol = mf.mol
+        self.mo_coeff = mf.mo_coeff
+        self.mo_occ = mf.mo_occ
+        self.max_cycle = 100
+        self.conv_tol = 1e-10
+        self.conv_tol_normt = 1e-8
+        self.level_shift = 0.0
+        self.diis = True
+        self.diis_space = 6
+        self.verbose = 0
+        self.t1 = self.t2 = self.l1 = self.l2 = None
+        self.converged = False
+        self.converged_lambda = False
+        self._eris = None
+
+    @property
+    def nocc(self):
+        return int((self.mo_occ > 0).sum())
+
+    @property
+    def nmo(self):
+        return self.mo_coeff.shape[1]
+
+    def get_frozen_mask(self):
+        return np.ones(self.nmo, dtype=bool)
+
+    def ao2mo(self, mo_coeff=None):
+        if mo_coeff is None and self._eris is not None:
+            return self._eris
+        eris = ERIs(self._scf, mo_coeff)
+        if mo_coeff is None:
+            self._eris = eris
+        return eris
+
+    def amplitudes_to_vector(self, t1, t2):
+        return amplitudes_to_vector(t1, t2)
+
+    def kernel(self, eris=None):
+        eris = self.ao2mo() if eris is None else eris
+        nocc = self.nocc
+        e_o, e_v = eris.mo_energy[:nocc], eris.mo_energy[nocc:]
+        nvir = e_v.numel()
+        eia = e_o[:, None] - e_v
+        t1 = eris.fock[:nocc, nocc:] / eia
+        t2 = eris.ovov.permute(0, 2, 1, 3) / (eia[:, None, :, None] + eia[None, :, None, :])
+        e = ccsd_energy(t1, t2, eris)
+        diis = DIIS(self.diis_space) if self.diis else None
+        self.converged = False
+        for it in range(self.max_cycle):
+            t1n, t2n = update_amps(t1, t2, eris)
+            vec = amplitudes_to_vector(t1n, t2n)
+            norm = float(torch.linalg.norm(vec - amplitudes_to_vector(t1, t2)))
+            if diis is not None:
+                vec = diis.update(vec)
+            t1, t2 = vector_to_amplitudes(vec, nocc, nvir)
+            e_new = ccsd_energy(t1, t2, eris)
+            if abs(e_new - e) < self.conv_tol and norm < self.conv_tol_normt:
+                e = e_new
+                self.converged = True
+                break
+            e = e_new
+        self.e_corr = e
+        self.e_tot = self._scf.e_tot + e
+        self.t1, self.t2 = t1, t2
+        return e, t1, t2
+
+    def solve_lambda(self, t1=None, t2=None, l1=None, l2=None, eris=None):
+        from .gpu_ccsd_lambda import solve_lambda_gpu
+        return solve_lambda_gpu(self, t1, t2, l1, l2, eris)
+
+
+def restore_ccsd_solver(mf, t1, t2, e_corr=None):
+    """A CCSD object holding given amplitudes (examples/h2o_ccsd_grad.py:58-70)."""
+    mycc = CCSD(mf)
+    mycc.t1 = torch.as_tensor(np.asarray(t1) if not torch.is_tensor(t1) else t1, dtype=torch.float64, device=DEV)
+    mycc.t2 = torch.as_tensor(np.asarray(t2) if not torch.is_tensor(t2) else t2, dtype=torch.float64, device=DEV)
+    mycc.e_corr = e_corr
+    mycc.converged = True
+    return mycc

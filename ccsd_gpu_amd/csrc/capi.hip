@@ -20,6 +20,12 @@ hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const 
                            long long npair_s, int nao_c, long long npair_c, int nfi_c, int ncomp, hipStream_t stream);
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
 size_t ccsd_c2s_term_size();
+hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
+hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
+                                  long long lda, long long ldb, long long ldc, long long sA, long long sB,
+                                  long long sC, int batch, hipStream_t stream);
+hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
+                             const double* charges, int natm, int nbas, int nao, hipStream_t stream);
 
 namespace {
 
@@ -71,6 +77,7 @@ struct ccsd_plan {
     ShellInfo* d_shells = nullptr;
     double* d_env = nullptr;
     double* d_coords = nullptr;
+    double* d_charges = nullptr;
     std::vector<PairClass> ket;                      // ksh >= lsh
     std::map<std::pair<int, int>, std::vector<PairClass>> bra_cache;
     bool has_c2s = false;
@@ -149,6 +156,9 @@ int upload_geometry(ccsd_plan* p)
     }
     HIP_TRY(hipMemcpy(p->d_env, p->env.data(), sizeof(double) * p->nenv, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_coords, coords.data(), sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice));
+    std::vector<double> charges(p->natm);
+    for (int a = 0; a < p->natm; a++) charges[a] = (double)p->atm[a * CCSD_ATM_SLOTS + 0];
+    HIP_TRY(hipMemcpy(p->d_charges, charges.data(), sizeof(double) * p->natm, hipMemcpyHostToDevice));
     return CCSD_OK;
 }
 
@@ -324,7 +334,8 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
     int rc = CCSD_OK;
     if (hipMalloc(&p->d_shells, sizeof(ShellInfo) * nbas) != hipSuccess ||
         hipMalloc(&p->d_env, sizeof(double) * nenv) != hipSuccess ||
-        hipMalloc(&p->d_coords, sizeof(double) * 3 * natm) != hipSuccess) {
+        hipMalloc(&p->d_coords, sizeof(double) * 3 * natm) != hipSuccess ||
+        hipMalloc(&p->d_charges, sizeof(double) * natm) != hipSuccess) {
         ccsd_plan_destroy(p);
         return fail(CCSD_ERR_ALLOC, "device allocation for the basis plan failed");
     }
@@ -387,6 +398,7 @@ int ccsd_plan_destroy(ccsd_plan* p)
     if (p->d_shells) (void)hipFree(p->d_shells);
     if (p->d_env) (void)hipFree(p->d_env);
     if (p->d_coords) (void)hipFree(p->d_coords);
+    if (p->d_charges) (void)hipFree(p->d_charges);
     if (p->d_c2s) (void)hipFree(p->d_c2s);
     delete p;
     return CCSD_OK;
@@ -429,6 +441,37 @@ int ccsd_grad_ip1_gamma(ccsd_plan* p, double* de, const double* gamma, double sc
     rc = get_bra(p, ish0, ish1, &bra);
     if (rc) return rc;
     return run_classes(p, *bra, 1, nullptr, 0, 0, gamma, de, scale, (hipStream_t)stream);
+}
+
+int ccsd_int1e(ccsd_plan* p, int kind, double* out, void* stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!out) return fail(CCSD_ERR_NULL, "null output pointer");
+    if (kind < 0 || kind > 5) return fail(CCSD_ERR_SHAPE, "int1e kind must be 0..5");
+    const long long nn = (long long)p->nao_cart * p->nao_cart;
+    const long long nmat = kind <= 2 ? 1 : (kind <= 4 ? 3 : 3LL * p->natm);
+    hipError_t e = ccsd_launch_fill(out, nmat * nn, 0.0, (hipStream_t)stream);
+    if (e == hipSuccess)
+        e = ccsd_launch_int1e(kind, out, p->d_shells, p->d_env, p->d_coords, p->d_charges, p->natm, p->nbas,
+                              p->nao_cart, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "int1e_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K, long long lda,
+                    long long ldb, long long ldc, long long strideA, long long strideB, long long strideC, int batch,
+                    void* stream)
+{
+    if (!A || !B || !C) return fail(CCSD_ERR_NULL, "null matrix pointer");
+    if (mode < 0 || mode > 1) return fail(CCSD_ERR_SHAPE, "mode must be 0 (fp32) or 1 (bf16x3)");
+    if (M < 0 || N < 0 || K < 0 || batch < 0 || lda < K || ldb < N || ldc < N)
+        return fail(CCSD_ERR_SHAPE, "bad GEMM dimensions");
+    if ((M + 63) / 64 > 65535 || batch > 65535) return fail(CCSD_ERR_SHAPE, "M or batch too large for one launch");
+    hipError_t e = ccsd_launch_gemm_mixed(mode, A, B, C, M, N, K, lda, ldb, ldc, strideA, strideB, strideC, batch,
+                                          (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gemm_mixed_kernel");
+    return CCSD_OK;
 }
 
 int ccsd_rys_roots(int n, const double* T, int count, double* r, double* w, void* stream)

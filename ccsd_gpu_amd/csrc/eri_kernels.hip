@@ -20,8 +20,7 @@
 // the two-particle density and one atomic add per wave into dE/dR.
 #include "eri_common.h"
 
-#define RYS_STORAGE __device__
-#include "rys_table.inc"
+#include "rys_device.h"
 
 namespace ccsd {
 
@@ -39,37 +38,6 @@ __device__ const int kCart[56] = {
 
 __device__ __forceinline__ int cart_comp(int l, int f) { return kCart[l * (l + 1) * (l + 2) / 6 + f]; }
 __device__ __forceinline__ int ncart_dev(int l) { return (l + 1) * (l + 2) / 2; }
-
-// Rys roots (u = t^2, ascending) and weights for n roots at argument T.
-__device__ void rys_roots_dev(int n, double T, double* __restrict__ r, double* __restrict__ w)
-{
-    if (T >= RYS_TASYM(n)) {
-        const int o = n * (n - 1) / 2;
-        const double s = sqrt(0.78539816339744830962 / T);
-        const double inv = 1.0 / T;
-        for (int i = 0; i < n; i++) {
-            r[i] = RYS_LARGE_R[o + i] * inv;
-            w[i] = RYS_LARGE_W[o + i] * s;
-        }
-        return;
-    }
-    const int it = (int)(T * RYS_INV_WIDTH);
-    const double u = (T - it * RYS_WIDTH) * (2.0 / RYS_WIDTH) - 1.0;
-    const double u2 = 2.0 * u;
-    const double* c = RYS_TABLE + RYS_OFFSET[n - 1] + (long)it * 2 * n * (RYS_DEGREE + 1);
-    for (int s = 0; s < 2 * n; s++) {
-        const double* cs = c + s * (RYS_DEGREE + 1);
-        double b1 = 0.0, b2 = 0.0;
-#pragma unroll
-        for (int k = RYS_DEGREE; k >= 1; k--) {
-            const double b0 = cs[k] + u2 * b1 - b2;
-            b2 = b1;
-            b1 = b0;
-        }
-        const double v = cs[0] + u * b1 - b2;
-        if (s < n) r[s] = v; else w[s - n] = v;
-    }
-}
 
 // LDS layout (doubles) of the generic quartet kernel; sizes from the launch.
 struct LdsLayout {
@@ -402,6 +370,12 @@ __global__ void c2s_s2kl_kernel(double* __restrict__ out, const double* __restri
     }
 }
 
+__global__ void fill_kernel(double* __restrict__ p, long long n, double v)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x)
+        p[t] = v;
+}
+
 __global__ void rys_roots_kernel(int n, const double* __restrict__ T, int count, double* __restrict__ r,
                                  double* __restrict__ w)
 {
@@ -444,3 +418,12 @@ hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double*
 }
 
 size_t ccsd_c2s_term_size() { return sizeof(ccsd::C2sTerm); }
+
+hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536) nb = 65536;
+    hipLaunchKernelGGL(ccsd::fill_kernel, dim3((unsigned)nb), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}

@@ -1,0 +1,374 @@
+"""CCSD nuclear gradient on the GPU.
+
+Keeps the reference's API (ccsd_gpu/gpu_ccsd_grad.py):
+    _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2)          (:101-126)
+    _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2)          (:129-246)
+    _rdm2_mo2ao_gpu(mycc, d2, mo_coeff)                       (:254-336)
+    grad_elec_gpu(mycc, t1, t2, l1, l2, atmlst=None, verbose=...)  (:344-586)
+    compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2)            (:711-739)
+and the same gradient formula (Phases 1-5 of grad_elec_gpu). What is
+organised differently, MI355X-first:
+  * All AO-side two-electron work is done in the Cartesian AO basis: the
+    2-RDM goes MO -> Cartesian AO in one transform with C_cart = cart2sph @ C;
+    full contractions are basis independent, and Imat is brought back with
+    Imat = C^T Imat_cc C (C^T C)^-1.
+  * The AO ERIs are computed once (HIP Rys kernel) and stay in HBM; Imat is
+    one DGEMM over them and every CPHF JK build two DGEMVs.
+  * The reference's per-atom loop over shell blocks (int2e_ip1 -> HBM ->
+    GEMV, plus the HF vhf1 J/K pieces, :423-496) becomes one fused launch set
+    of the int2e_ip1 kernel contracting with Gamma + Gamma_sep, where
+    Gamma_sep = D P + P D - (D x P + P x D)/4 (exchange-like terms, kl
+    symmetrized) reproduces -sum vhf1 . dm1p. The reordering (Z-vector before
+    the ip1 pass, so dm1p is known) is exact.
+  * The Z-vector is a preconditioned conjugate-gradient solve on the device
+    (the reference calls pyscf.scf.cphf.solve, :662).
+`fused=False` keeps the reference's structure on the GPU (ip1 tensors in HBM
+per atom block, GEMV, explicit vhf1): the plain FP64 comparison path.
+"""
+import time
+
+import numpy as np
+import torch
+
+from .gpu_ccsd_lambda import _dev
+from .hip.grad2e import contract_int2e_ip1
+from .hip.int1e_gpu import int1e
+from .hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
+from .precision import contract
+from .scf import DEV
+
+E = torch.einsum
+
+
+# ---------------------------------------------------------------------------
+# Phase 1: RDM intermediates
+# ---------------------------------------------------------------------------
+def _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
+    C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+    t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
+    doo = -C("ja,ia->ij", t1, l1)
+    dvv = C("ia,ib->ab", t1, l1)
+    xtv = C("ie,me->im", t1, l1)
+    dvo = t1.T - C("im,ma->ai", xtv, t1)
+    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
+    doo = doo - C("jkab,ikab->ij", theta, l2)
+    dvv = dvv + C("jica,jicb->ab", theta, l2)
+    xt1 = C("mnef,inef->mi", l2, theta)
+    xt2 = C("mnaf,mnef->ea", l2, theta)
+    dvo = dvo + C("imae,me->ai", theta, l1) - C("mi,ma->ai", xt1, t1) - C("ie,ae->ai", t1, xt2)
+    return doo, l1.clone(), dvo, dvv
+
+
+def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
+    C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+    t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
+    pvOOv = C("ikca,jkcb->aijb", l2, t2)
+    moo = E("dljd->jl", pvOOv) * 2
+    mvv = E("blld->db", pvOOv) * 2
+    gooov = C("kc,cija->jkia", t1, pvOOv)
+    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
+    pvoOV = -C("ikca,jkbc->aijb", l2, t2) + C("ikac,jkbc->aijb", l2, theta)
+    moo = moo + E("dljd->jl", pvoOV)
+    mvv = mvv + E("blld->db", pvoOV)
+    gooov = gooov - C("jc,cika->jkia", t1, pvoOV)
+    mia = C("kc,ikac->ia", l1, t2) * 2 - C("kc,ikca->ia", l1, t2)
+    mab = C("kc,kb->cb", l1, t1)
+    mij = C("kc,jc->jk", l1, t1) + moo * 0.5
+    tau = E("ia,jb->ijab", t1, t1) + t2
+    goooo = C("ijab,klab->ijkl", tau, l2) * 0.5
+    doooo = goooo.permute(0, 2, 1, 3) * 2 - goooo.permute(0, 3, 1, 2)
+    gooov = gooov + C("ji,ka->jkia", -0.5 * moo, t1) + C("la,jkil->jkia", 2 * t1, goooo)
+    gooov = gooov - C("ib,jkba->jkia", l1, tau) - C("jkba,ib->jkia", l2, t1)
+    dooov = gooov.permute(0, 2, 1, 3) * 2 - gooov.permute(1, 2, 0, 3)
+
+    goovv = E("ia,jb->ijab", mia, t1)
+    tmp = C("ijkl,klab->ijab", goooo, tau) - C("jk,ikab->ijab", mij, tau)
+    tmp = tmp - C("cb,ijac->ijab", mab, t2) - C("bd,ijad->ijab", mvv * 0.5, tau)
+    goovv = goovv + tmp + 0.5 * tau + 0.5 * l2
+    t1t1 = E("jc,kb->jckb", t1, t1)
+    gOvvO = C("kiac,jckb->iabj", l2, t1t1) + pvOOv.permute(1, 0, 3, 2)
+    govVO = E("ia,jb->iabj", l1, t1) - C("ikac,jckb->iabj", l2, t1t1) + pvoOV.permute(1, 0, 3, 2)
+    dovvo = 2 * govVO + gOvvO
+    doovv = (-2 * gOvvO - govVO).permute(3, 0, 1, 2)
+    tau2 = tau - t2 * 0.5
+    goovv = goovv + C("dlib,jlda->ijab", pvOOv, tau2) - C("dlia,jldb->ijab", pvoOV, tau2)
+    goovv = goovv + C("dlia,jlbd->ijab", pvoOV + pvOOv * 0.5, t2)
+    dovov = goovv.permute(0, 2, 1, 3) * 2 - goovv.permute(1, 2, 0, 3)
+    del goovv
+
+    gvvvv = C("ijab,ijcd->abcd", l2, t2)
+    jabc = C("ijab,ic->jabc", l2, t1)
+    gvvvv = gvvvv + C("jabc,jd->abcd", jabc, t1)
+    vvv = gvvvv.permute(0, 2, 1, 3)
+    dvvvv = vvv - vvv.permute(0, 3, 2, 1) * 0.5
+    gvovv = C("adbc,id->aibc", gvvvv, -t1)
+    del gvvvv, vvv
+    gvovv = gvovv + C("akic,kb->aibc", pvoOV, t1) - C("akib,kc->aibc", pvOOv, t1)
+    gvovv = gvovv + C("ja,jibc->aibc", l1, t2) + C("ja,jbic->aibc", l1, E("jb,ic->jbic", t1, t1))
+    gvovv = gvovv + C("ba,ic->aibc", mvv * 0.5, t1) + C("ja,jibc->aibc", t1, l2)
+    dovvv = gvovv.permute(1, 3, 0, 2) * 2 - gvovv.permute(1, 2, 0, 3)
+    return (dovov, dvvvv, doooo, doovv, dovvo, None, dovvv, dooov)
+
+
+# ---------------------------------------------------------------------------
+# Phase 2: full 2-RDM and MO -> AO
+# ---------------------------------------------------------------------------
+def _assemble_dm2_mo(d2):
+    dovov, dvvvv, doooo, doovv, dovvo, _, dovvv, dooov = d2
+    nocc, nvir = dovov.shape[:2]
+    nmo = nocc + nvir
+    o, v = slice(0, nocc), slice(nocc, None)
+    dm2 = torch.zeros((nmo,) * 4, dtype=torch.float64, device=DEV)
+    dm2[o, v, o, v] = dovov + dovov.permute(2, 3, 0, 1)
+    dm2[v, o, v, o] = dm2[o, v, o, v].permute(1, 0, 3, 2)
+    dm2[o, o, v, v] = doovv + doovv.permute(1, 0, 3, 2)
+    dm2[v, v, o, o] = dm2[o, o, v, v].permute(2, 3, 0, 1)
+    dm2[o, v, v, o] = dovvo + dovvo.permute(3, 2, 1, 0)
+    dm2[v, o, o, v] = dm2[o, v, v, o].permute(1, 0, 3, 2)
+    dm2[v, v, v, v] = (dvvvv + dvvvv.permute(1, 0, 3, 2)) * 2
+    dm2[o, o, o, o] = (doooo + doooo.permute(1, 0, 3, 2)) * 2
+    dm2[o, v, v, v] = dovvv
+    dm2[v, v, o, v] = dovvv.permute(2, 3, 0, 1)
+    dm2[v, v, v, o] = dovvv.permute(3, 2, 1, 0)
+    dm2[v, o, v, v] = dovvv.permute(1, 0, 3, 2)
+    dm2[o, o, o, v] = dooov
+    dm2[o, v, o, o] = dooov.permute(2, 3, 0, 1)
+    dm2[o, o, v, o] = dooov.permute(1, 0, 3, 2)
+    dm2[v, o, o, o] = dooov.permute(3, 2, 1, 0)
+    return dm2
+
+
+def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff):
+    """Symmetrized AO 2-RDM, full (nao^4), in the AO basis of the columns of
+    `mo_coeff`'s rows (Cartesian when mo_coeff = cart2sph @ C)."""
+    dm2 = _assemble_dm2_mo(d2).permute(1, 0, 3, 2).contiguous()      # chemist's order
+    c = _dev(mo_coeff)
+    t = torch.tensordot(c, dm2, dims=([1], [0]))                       # (i q r s)
+    del dm2
+    t = torch.tensordot(c, t, dims=([1], [1])).permute(1, 0, 2, 3)     # (i j r s)
+    t = torch.tensordot(t, c, dims=([2], [1])).permute(0, 1, 3, 2)     # (i j s k) -> (i j k s)
+    t = torch.tensordot(t, c, dims=([3], [1])).contiguous()            # (i j k l)
+    t = t + t.permute(1, 0, 2, 3)
+    t = t + t.permute(0, 1, 3, 2)
+    return t.mul_(0.5)
+
+
+def _pack_halved(full):
+    n = full.shape[-1]
+    k, l = torch.tril_indices(n, n, device=full.device)
+    out = full[..., k, l].contiguous()
+    out[..., k == l] *= 0.5
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Phase 4: Z-vector
+# ---------------------------------------------------------------------------
+def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
+    """Solve (e_a - e_i) x_ai + fvind(x)_ai = -Xvo_ai by preconditioned CG on the device."""
+    mf = mycc._scf
+    nocc = mycc.nocc
+    c = _dev(mycc.mo_coeff)
+    co, cv = c[:, :nocc], c[:, nocc:]
+    e = _dev(mf.mo_energy)
+    eai = e[nocc:, None] - e[None, :nocc]
+
+    def fvind(x):
+        dm = cv @ x @ co.T
+        j, k = ints.get_jk(dm + dm.T)
+        return cv.T @ (j - 0.5 * k) @ co * 2
+
+    def A(x):
+        return eai * x + fvind(x)
+
+    b = -Xvo
+    x = b / eai
+    r = b - A(x)
+    z = r / eai
+    p = z.clone()
+    rz = torch.sum(r * z)
+    bn = float(torch.linalg.norm(b))
+    for it in range(max_cycle):
+        if float(torch.linalg.norm(r)) <= tol * max(bn, 1e-300):
+            break
+        Ap = A(p)
+        alpha = rz / torch.sum(p * Ap)
+        x = x + alpha * p
+        r = r - alpha * Ap
+        z = r / eai
+        rz_new = torch.sum(r * z)
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    mycc._zvector_iterations = it
+    nmo = c.shape[1]
+    dm1 = torch.zeros((nmo, nmo), dtype=torch.float64, device=DEV)
+    dm1[nocc:, :nocc] = x
+    dm1[:nocc, nocc:] = x.T
+    return dm1
+
+
+# ---------------------------------------------------------------------------
+# Phases 3-5
+# ---------------------------------------------------------------------------
+def _separable_gamma(D, P):
+    """Gamma_sep(ij,kl) = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl)/4."""
+    g = E("ij,kl->ijkl", D, P)
+    g += E("ij,kl->ijkl", P, D)
+    g -= 0.25 * E("il,kj->ijkl", D, P)
+    g -= 0.25 * E("ik,lj->ijkl", D, P)
+    g -= 0.25 * E("il,jk->ijkl", P, D)
+    g -= 0.25 * E("ik,jl->ijkl", P, D)
+    return g
+
+
+
+
+def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose=0, fused=True,
+                  precision="fp64", timings=None):
+    """Electronic CCSD gradient dE/dR (natm, 3) as numpy, as ccsd_gpu/gpu_ccsd_grad.py:344."""
+    tm = {} if timings is None else timings
+
+    def mark(name, t0):
+        torch.cuda.synchronize()
+        tm[name] = tm.get(name, 0.0) + time.perf_counter() - t0
+        return time.perf_counter()
+
+    t1 = _dev(mycc.t1 if t1 is None else t1)
+    t2 = _dev(mycc.t2 if t2 is None else t2)
+    l1 = _dev(mycc.l1 if l1 is None else l1)
+    l2 = _dev(mycc.l2 if l2 is None else l2)
+    mol = mycc.mol
+    mf = mycc._scf
+    ints = mf._ints()
+    c2s = ints.c2s
+    mo = _dev(mycc.mo_coeff)
+    mo_energy = _dev(mf.mo_energy)
+    nao, nmo = mo.shape
+    nocc = mycc.nocc
+    nc = c2s.shape[0]
+    t0 = time.perf_counter()
+
+    # Phase 1
+    d1 = _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
+    doo, dov, dvo, dvv = d1
+    d2 = _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
+    t0 = mark("rdm", t0)
+
+    # Phase 2: Cartesian AO 2-RDM (symmetrized)
+    gam = _rdm2_mo2ao_gpu(mycc, d2, c2s @ mo)
+    del d2
+    t0 = mark("mo2ao", t0)
+
+    # Phase 3a: Imat (one DGEMM over the HBM-resident Cartesian ERIs)
+    imat_cc = 0.5 * (ints.eri_c.reshape(nc, -1) @ gam.reshape(nc, -1).T)
+    ctc = c2s.T @ c2s
+    imat = c2s.T @ imat_cc @ c2s @ torch.linalg.inv(ctc)
+    t0 = mark("imat", t0)
+
+    # Phase 4: orbital response
+    S = ints.S
+    Imo = -(mo.T @ imat @ S @ mo)
+    dm1mo = torch.zeros((nmo, nmo), dtype=torch.float64, device=DEV)
+    dm1mo[:nocc, :nocc] = doo + doo.T
+    dm1mo[nocc:, nocc:] = dvv + dvv.T
+    dm1 = mo @ dm1mo @ mo.T
+    j, k = ints.get_jk(dm1)
+    vhf = (j - 0.5 * k) * 2
+    Xvo = mo[:, nocc:].T @ vhf @ mo[:, :nocc]
+    Xvo = Xvo + Imo[:nocc, nocc:].T - Imo[nocc:, :nocc]
+    dm1mo = dm1mo + _response_dm1_gpu(mycc, Xvo, ints)
+    Imo[nocc:, :nocc] = Imo[:nocc, nocc:].T
+    im1 = mo @ Imo @ mo.T
+    t0 = mark("zvector", t0)
+
+    # HF and relaxed densities
+    hf_dm1 = 2.0 * mo[:, :nocc] @ mo[:, :nocc].T
+    dm1 = mo @ dm1mo @ mo.T
+    dm1p = hf_dm1 + dm1 * 2
+
+    # Phase 3b: two-electron derivative contribution
+    de2 = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
+    Dc = c2s @ hf_dm1 @ c2s.T
+    Pc = c2s @ dm1p @ c2s.T
+    if fused:
+        gam += _separable_gamma(Dc, Pc)
+        gp = _pack_halved(gam)
+        del gam
+        contract_int2e_ip1(mol, gp, scale=-2.0, de=de2)
+        del gp
+    else:
+        gp = _pack_halved(gam)
+        del gam
+        _grad2e_unfused(mol, gp, Dc, Pc, de2)
+        del gp
+    t0 = mark("grad2e", t0)
+
+    # Phase 5: one-electron and overlap terms
+    s1 = -int1e(mol, "ipovlp", c2s)
+    vrinv_all = int1e(mol, "ipnuc_atom", c2s)
+    h1 = -(int1e(mol, "ipkin", c2s) + vrinv_all.sum(dim=0))
+    zeta = (mo_energy[:, None] + mo_energy[None, :]) * 0.5
+    zeta[nocc:, :nocc] = mo_energy[:nocc]
+    zeta[:nocc, nocc:] = mo_energy[:nocc].reshape(-1, 1)
+    zeta = mo @ (zeta * dm1mo) @ mo.T
+    p1 = mo[:, :nocc] @ mo[:, :nocc].T
+    j, k = ints.get_jk(dm1 + dm1.T)
+    vhf_s1occ = p1 @ (j - 0.5 * k) @ p1
+    dm1_tot = dm1 + hf_dm1
+    mocc = mo[:, :nocc]
+    zeta = zeta + (mocc * (mo_energy[:nocc] * 2.0)) @ mocc.T
+    offs = mol.offset_nr_by_atom()
+    atoms = range(mol.natm) if atmlst is None else list(atmlst)
+    de = torch.zeros((len(atoms), 3), dtype=torch.float64, device=DEV)
+    for kk, ia in enumerate(atoms):
+        p0, p1_ = int(offs[ia, 2]), int(offs[ia, 3])
+        vrinv = vrinv_all[ia].clone()
+        vrinv[:, p0:p1_] += h1[:, p0:p1_]
+        h1ao = vrinv + vrinv.transpose(1, 2)
+        de[kk] += E("xij,ij->x", s1[:, p0:p1_], im1[p0:p1_])
+        de[kk] += E("xji,ij->x", s1[:, p0:p1_], im1[:, p0:p1_])
+        de[kk] += E("xij,ji->x", h1ao, dm1_tot)
+        de[kk] -= E("xij,ij->x", s1[:, p0:p1_], zeta[p0:p1_])
+        de[kk] -= E("xji,ij->x", s1[:, p0:p1_], zeta[:, p0:p1_])
+        de[kk] -= E("xij,ij->x", s1[:, p0:p1_], vhf_s1occ[p0:p1_]) * 2
+        de[kk] += de2[ia]
+    t0 = mark("onee", t0)
+    return de.cpu().numpy()
+
+
+def _grad2e_unfused(mol, gp, Dc, Pc, de2):
+    """Reference structure on the GPU: per shell block, int2e_ip1 -> HBM, GEMV with
+    the packed 2-RDM rows, and the HF vhf1 J/K pieces contracted with dm1p
+    (ccsd_gpu/gpu_ccsd_grad.py:423-496), in the Cartesian AO basis."""
+    mol_c = mol.copy()
+    mol_c.cart = True
+    loc = mol_c.ao_loc_nr()
+    nc = mol_c.nao_nr()
+    k, l = torch.tril_indices(nc, nc, device=DEV)
+    for ib in range(mol.nbas):
+        i0, i1 = int(loc[ib]), int(loc[ib + 1])
+        ip1 = compute_int2e_ip1_gpu(mol_c, ib, ib + 1)                  # (3, nf, nc, npair)
+        a = mol.bas_atom(ib)
+        de2[a] -= 2.0 * (ip1.reshape(3, -1) @ gp[i0:i1].reshape(-1))
+        # HF part: -sum_x vhf1[x] . dm1p with i restricted to this block
+        full = torch.empty((3, i1 - i0, nc, nc, nc), dtype=torch.float64, device=DEV)
+        full[..., k, l] = ip1
+        full[..., l, k] = ip1
+        D, P = Dc, Pc
+        v = E("xijkl,ij,kl->x", full, D[i0:i1], P) + E("xijkl,ij,kl->x", full, P[i0:i1], D)
+        v -= 0.5 * E("xijkl,il,kj->x", full, D[i0:i1], P) + 0.5 * E("xijkl,il,jk->x", full, P[i0:i1], D)
+        de2[a] -= v
+        del full, ip1
+
+
+def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None):
+    """Total dE/dR (natm, 3) in Eh/Bohr (despite the name, not -dE/dR), as
+    ccsd_gpu/gpu_ccsd_grad.py:711-739: electronic gradient + nuclear repulsion."""
+    from .cc import CCSD
+    mycc = CCSD(mf)
+    mycc.t1, mycc.t2, mycc.l1, mycc.l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
+    mycc.converged = True
+    mycc.converged_lambda = True
+    de = grad_elec_gpu(mycc, t1, t2, l1, l2, verbose=0, fused=fused, precision=precision, timings=timings)
+    de += mf.nuc_grad_method().grad_nuc()
+    return np.array(de)

@@ -1,0 +1,249 @@
+"""CCSD lambda solver on the GPU.
+
+Keeps the reference's API (ccsd_gpu/gpu_ccsd_lambda.py):
+    make_intermediates_gpu(mycc, t1, t2, eris)               (:68-209)
+    update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds)       (:212-337)
+    kernel_gpu(mycc, eris, t1, t2, l1, l2, max_cycle, tol)    (:340-408)
+    solve_lambda_gpu(mycc, t1=None, t2=None, l1=None, l2=None, eris=None,
+                     use_cpu_vvvv=True, fallback_to_cpu=True)  (:411-449)
+The lambda equations are the reference's (a port of PySCF ccsd_lambda). What
+differs is where things run: everything -- the l2.vvvv term the reference
+sends to the CPU (:238-243), the DIIS extrapolation and the convergence norm
+(:390-401) -- stays on the device, so an iteration moves no amplitudes across
+PCIe. Contractions go through `contract` (precision.py): FP64 for the plain
+path, FP32 / split-BF16 MFMA GEMMs with FP64 residual accumulation for the
+mixed path.
+
+Differences of behaviour, on purpose:
+  * there is no CPU fallback: without a HIP device every call raises, whatever
+    `fallback_to_cpu` says (the reference falls back to PySCF);
+  * use_cpu_vvvv=True includes the l2.vvvv term (computed on the GPU);
+    use_cpu_vvvv=False omits it, as the reference does (:244-245).
+"""
+import numpy as np
+import torch
+
+from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
+from .precision import contract
+from .scf import DEV
+
+
+class _IMDS:
+    pass
+
+
+def _dev(a):
+    if torch.is_tensor(a):
+        return a.to(device=DEV, dtype=torch.float64)
+    return torch.as_tensor(np.asarray(a), dtype=torch.float64, device=DEV)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("No usable HIP device for the lambda solve (this build has no CPU fallback).")
+
+
+def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
+    C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+    t1 = _dev(t1)
+    t2 = _dev(t2)
+    nocc, nvir = t1.shape
+    fock = eris.fock
+    foo, fov, fvv = fock[:nocc, :nocc], fock[:nocc, nocc:], fock[nocc:, nocc:]
+    w1 = fvv - C("ja,jb->ba", fov, t1)
+    w2 = foo + C("ib,jb->ij", fov, t1)
+    w3 = C("kc,jkbc->bj", fov, t2) * 2 + fov.T
+    w3 = w3 - C("kc,kjbc->bj", fov, t2)
+    w3 = w3 + C("kc,kb,jc->bj", fov, t1, t1)
+    w4 = fov.clone()
+
+    ovvv = eris.ovvv
+    vvov = ovvv.permute(2, 3, 0, 1)
+    tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
+
+    w1 = w1 + C("jcba,jc->ba", ovvv, t1 * 2) - C("jabc,jc->ba", ovvv, t1)
+    theta = t2 * 2 - t2.permute(1, 0, 2, 3)
+    w3 = w3 + C("jkcd,kdcb->bj", theta, ovvv)
+    wVOov = C("jbcd,kd->bjkc", ovvv, t1)
+    wvOOv = C("cbjd,kd->cjkb", vvov, -t1)
+    g2vovv = vvov.permute(0, 2, 1, 3) * 2 - vvov.permute(0, 2, 3, 1)
+    wvooo = C("cibd,jkbd->ckij", g2vovv, tau)
+    del g2vovv
+    wvvov = C("jabd,jkcd->abkc", ovvv, t2) * -1.5
+    wvvov = wvvov + vvov.permute(0, 3, 2, 1) * 2 - vvov
+    g2vvov = vvov * 2 - ovvv.permute(1, 2, 0, 3)
+    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
+    vackb = C("acjd,kjbd->ackb", g2vvov, theta)
+    del g2vvov
+    wvvov = wvvov + vackb.permute(0, 3, 2, 1) - vackb * 0.5
+    del vackb
+
+    ovoo = eris.ovoo
+    w2 = w2 + C("kbij,kb->ij", ovoo, t1) * 2 - C("ibkj,kb->ij", ovoo, t1)
+    theta = t2.permute(1, 0, 2, 3) * 2 - t2
+    w3 = w3 - C("lckj,klcb->bj", ovoo, theta)
+    tmp = C("lc,jcik->ijkl", t1, ovoo)
+    woooo = tmp + tmp.permute(1, 0, 3, 2)
+    wvOOv = wvOOv + C("lbjk,lc->bjkc", ovoo, t1)
+    wVOov = wVOov - C("jbkl,lc->bjkc", ovoo, t1)
+    wvooo = wvooo + ovoo.permute(1, 3, 2, 0) * 2 - ovoo.permute(1, 0, 2, 3)
+    wvooo = wvooo - C("klbc,iblj->ckij", t2, ovoo * 1.5)
+    g2ovoo = ovoo * 2 - ovoo.permute(2, 1, 0, 3)
+    theta = t2 * 2 - t2.permute(1, 0, 2, 3)
+    vcjik = C("jlcb,lbki->cjki", theta, g2ovoo)
+    wvooo = wvooo + vcjik.permute(0, 3, 2, 1) - vcjik * 0.5
+
+    voov = eris.ovvo.permute(1, 0, 3, 2)
+    woooo = woooo + C("cijd,klcd->ijkl", voov, tau)
+    g2voov = voov * 2 - voov.permute(0, 2, 1, 3)
+    tmpw4 = C("ckld,ld->kc", g2voov, t1)
+    w1 = w1 - C("ckja,kjcb->ba", g2voov, t2) - C("ja,jb->ba", tmpw4, t1)
+    w2 = w2 + C("jkbc,bikc->ij", t2, g2voov) + C("ib,jb->ij", tmpw4, t1)
+    w3 = w3 + t1.T @ tmpw4 @ t1.T
+    w4 = w4 + tmpw4
+    t1t1 = torch.einsum("kd,lc->kdlc", t1, t1)
+    wvOOv = wvOOv + C("bljd,kdlc->bjkc", voov, t1t1)
+    wVOov = wVOov - C("bjld,kdlc->bjkc", voov, t1t1)
+    VOov = C("bjld,klcd->bjkc", g2voov, t2) - C("bjld,kldc->bjkc", voov, t2) + voov
+    vOOv = C("bljd,kldc->bjkc", voov, t2) - eris.oovv.permute(2, 1, 0, 3)
+    wVOov = wVOov + VOov
+    wvOOv = wvOOv + vOOv
+    ov1 = vOOv * 2 + VOov
+    ov2 = VOov * 2 + vOOv
+    wvooo = wvooo - C("jb,bikc->ckij", t1, ov1) + C("kb,bijc->ckij", t1, ov2)
+    w3 = w3 + C("ckjb,kc->bj", ov2, t1)
+    wvvov = wvvov + C("ajkc,jb->abkc", ov1, t1) - C("ajkb,jc->abkc", ov2, t1)
+    wvvov = wvvov + C("laki,klbc->abic", g2ovoo, tau)
+
+    woooo = woooo + eris.oooo.permute(0, 2, 1, 3)
+    w3 = w3 + C("bc,jc->bj", w1, t1) - C("kj,kb->bj", w2, t1)
+    imds = _IMDS()
+    imds.w1, imds.w2, imds.w3, imds.w4 = w1, w2, w3, w4
+    imds.woooo, imds.wvooo = woooo, wvooo
+    imds.wVOov, imds.wvOOv, imds.wvvov = wVOov, wvOOv, wvvov
+    return imds
+
+
+def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=True, precision="fp64"):
+    C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+    if imds is None:
+        imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
+    t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
+    nocc, nvir = t1.shape
+    fov = eris.fock[:nocc, nocc:]
+    e_o = eris.mo_energy[:nocc]
+    e_v = eris.mo_energy[nocc:] + getattr(mycc, "level_shift", 0.0)
+
+    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
+    mba = C("klca,klcb->ba", l2, theta)
+    mij = C("ikcd,jkcd->ij", l2, theta)
+    mba1 = C("jc,jb->bc", l1, t1) + mba
+    mij1 = C("kb,jb->kj", l1, t1) + mij
+    mia1 = t1 + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
+    mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
+
+    if use_cpu_vvvv:
+        l2new = C("ijcd,acbd->ijab", l2, eris.vvvv)
+    else:
+        l2new = torch.zeros_like(l2)
+    l1new = C("ijab,jb->ia", l2new, t1) * 2 - C("jiab,jb->ia", l2new, t1)
+    l2new = l2new * 0.5
+
+    w1 = imds.w1 - torch.diag(e_v)
+    w2 = imds.w2 - torch.diag(e_o)
+    l1new = l1new + fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
+    l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
+    l1new = l1new + C("ijab,bj->ia", l2, imds.w3) * 2 - C("ijba,bj->ia", l2, imds.w3)
+    l2new = l2new + torch.einsum("ia,jb->ijab", l1, imds.w4)
+    l2new = l2new + C("jibc,ca->jiba", l2, w1) - C("jk,kiba->jiba", w2, l2)
+
+    ovoo = eris.ovoo
+    l1new = l1new - C("iajk,kj->ia", ovoo, mij1) * 2 + C("jaik,kj->ia", ovoo, mij1)
+    l2new = l2new - C("jbki,ka->jiba", ovoo, l1)
+
+    tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
+    l2tau = C("ijcd,klcd->ijkl", l2, tau)
+    l2t1 = C("jidc,kc->ijkd", l2, t1)
+
+    l1new = l1new - C("jb,jiab->ia", l1, eris.oovv)
+    ovvv = eris.ovvv
+    l1new = l1new + C("iabc,bc->ia", ovvv, mba1) * 2 - C("ibca,bc->ia", ovvv, mba1)
+    l2new = l2new + C("jbac,ic->jiba", ovvv, l1)
+    m4 = C("ijkd,kadb->ijab", l2t1, ovvv)
+    l2new = l2new - m4
+    l1new = l1new - C("ijab,jb->ia", m4, t1) * 2 - C("ijab,ia->jb", m4, t1) * 2
+    l1new = l1new + C("jiab,jb->ia", m4, t1) + C("jiab,ia->jb", m4, t1)
+
+    voov = eris.ovvo.permute(1, 0, 3, 2)
+    l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
+    l2new = l2new + voov.permute(1, 2, 0, 3) * 0.5
+    l2new = l2new - C("bjic,ca->jiba", voov, mba1) - C("bjka,ik->jiba", voov, mij1)
+    l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
+    m4 = C("ijkl,aklb->ijab", l2tau, voov)
+    l2new = l2new + m4 * 0.5
+    l1new = l1new + C("ijab,jb->ia", m4, t1) * 2 - C("ijba,jb->ia", m4, t1)
+
+    l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
+    tmp_voov = imds.wVOov * 2 + imds.wvOOv
+    tmp = l2.permute(0, 2, 1, 3) - l2.permute(0, 3, 1, 2) * 0.5
+    l2new = l2new + C("iakc,bjkc->jiba", tmp, tmp_voov)
+    tmp = C("jkca,bikc->jiba", l2, imds.wvOOv)
+    l2new = l2new + tmp + tmp.permute(1, 0, 2, 3) * 0.5
+
+    m3 = C("ijkl,klab->ijab", imds.woooo, l2)
+    l2new = l2new + m3 * 0.5
+    l1new = l1new + C("ijab,jb->ia", m3, t1) * 2 - C("ijba,jb->ia", m3, t1)
+
+    eia = e_o[:, None] - e_v
+    l1new = l1new / eia
+    # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
+    l2new = (l2new + l2new.permute(1, 0, 3, 2)) / (eia[:, None, :, None] + eia[None, :, None, :])
+    return l1new, l2new
+
+
+def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=None, tol=None, verbose=0,
+               use_cpu_vvvv=True, precision="fp64"):
+    _require_gpu()
+    if eris is None:
+        eris = mycc.ao2mo()
+    t1 = _dev(mycc.t1 if t1 is None else t1)
+    t2 = _dev(mycc.t2 if t2 is None else t2)
+    l1 = t1.clone() if l1 is None else _dev(l1)
+    l2 = t2.clone() if l2 is None else _dev(l2)
+    max_cycle = mycc.max_cycle if max_cycle is None else max_cycle
+    tol = mycc.conv_tol_normt if tol is None else tol
+    nocc, nvir = t1.shape
+    imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
+    adiis = DIIS(getattr(mycc, "diis_space", 6)) if getattr(mycc, "diis", True) else None
+    conv = False
+    normt = None
+    for istep in range(max_cycle):
+        l1n, l2n = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision)
+        vec = amplitudes_to_vector(l1n, l2n)
+        normt = float(torch.linalg.norm(vec - amplitudes_to_vector(l1, l2)))
+        if adiis is not None:
+            vec = adiis.update(vec)
+        l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
+        if normt < tol:
+            conv = True
+            break
+    mycc._lambda_iterations = istep + 1
+    mycc._lambda_normt = normt
+    return conv, l1, l2
+
+
+def solve_lambda_gpu(mycc, t1=None, t2=None, l1=None, l2=None, eris=None, use_cpu_vvvv=True,
+                     fallback_to_cpu=True, precision="fp64"):
+    """Solve the CCSD lambda equations on the GPU.
+
+    Side effects as the reference: mycc.converged_lambda, mycc.l1, mycc.l2 and
+    mycc._lambda_solver_mode ("gpu" here). Returns (l1, l2) as device tensors.
+    """
+    _require_gpu()
+    conv, l1o, l2o = kernel_gpu(mycc, eris=eris, t1=t1, t2=t2, l1=l1, l2=l2, max_cycle=mycc.max_cycle,
+                                tol=mycc.conv_tol_normt, use_cpu_vvvv=use_cpu_vvvv, precision=precision)
+    mycc.converged_lambda = conv
+    mycc.l1 = l1o
+    mycc.l2 = l2o
+    mycc._lambda_solver_mode = "gpu" if precision == "fp64" else f"gpu-{precision}"
+    return mycc.l1, mycc.l2

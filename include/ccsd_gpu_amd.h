@@ -62,6 +62,25 @@ int ccsd_int2e_ip1_s2kl(ccsd_plan* plan, double* out, int ish0, int ish1, int ca
 int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double scale, int ish0, int ish1,
                         void* stream);
 
+/* One-electron integrals in the Cartesian AO basis (row-major nao_c x nao_c
+ * matrices, fully overwritten): kind 0 overlap, 1 kinetic, 2 nuclear attraction
+ * (all nuclei), 3 <d/dr i|j> (3 matrices), 4 <d/dr i|T|j> (3), 5 <d/dr i|-Z_C/|r-C||j>
+ * for every nucleus C (natm x 3). These replace the PySCF one-electron pieces of
+ * the gradient -- mf_grad.get_ovlp, hcore_generator (int1e_ipkin, int1e_ipnuc,
+ * int1e_iprinv) -- called at ccsd_gpu/gpu_ccsd_grad.py:554-583. */
+int ccsd_int1e(ccsd_plan* plan, int kind, double* out, void* stream);
+
+/* Batched C = A B on the matrix cores, float64 operands in device memory:
+ * A [batch][M][K] (row stride lda, batch stride strideA), B [batch][K][N],
+ * C [batch][M][N] (overwritten). mode 0: FP32 MFMA (operands rounded to FP32);
+ * mode 1: split-BF16 (x = hi + lo, three bf16 MFMA products); both accumulate
+ * in FP32. These replace the cuBLAS / cuTENSOR GEMMs behind the CuPy einsum
+ * calls of the lambda and RDM code (ccsd_gpu/gpu_ccsd_lambda.py:68-337,
+ * ccsd_gpu/gpu_ccsd_grad.py:101-246) on the mixed-precision path. */
+int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K, long long lda,
+                    long long ldb, long long ldc, long long strideA, long long strideB, long long strideC, int batch,
+                    void* stream);
+
 /* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
  * arrays; r, w are (count, n)). Counterpart of rys_roots_simple
  * (ccsd_gpu/cuda/int2e_ip1_module.py:192-244), n = 1..9. */

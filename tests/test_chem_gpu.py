@@ -1,0 +1,156 @@
+"""GPU CCSD-gradient path against the FP64 oracle (H2O and friends).
+
+Mirrors the reference's tests/test_gpu_ccsd_grad.py: lambda against the CPU
+solution (there 1e-7 on l1/l2), the gradient against the CPU gradient on the
+same amplitudes (there 1e-6 Eh/Bohr). Orbitals, amplitudes and lambdas are
+shared between the two sides so that elementwise comparisons are meaningful.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import chem
+from oracle import integrals as oi
+
+from . import molecules
+
+GEOM = "O 0.0 -0.0757 0.0521; H 0.7572 0.5 -0.02; H -0.76 0.47 0.03"
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("make,cart", [(molecules.h2o_ccpvdz, False), (molecules.h2o_ccpvdz, True),
+                                       (molecules.g_shell_mol, False)])
+def test_int1e_gpu_matches_oracle(gpu, make, cart):
+    from ccsd_gpu_amd.hip.int1e_gpu import int1e
+    mol = make(cart=cart)
+    for kind in ("ovlp", "kin", "nuc", "ipovlp", "ipkin", "ipnuc_atom"):
+        g = int1e(mol, kind).cpu().numpy()
+        r = oi.int1e(mol, kind)
+        err = np.abs(g - r).max()
+        print(kind, err, flush=True)
+        assert err < 1e-10, (kind, err)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode,tol", [("fp32", 2e-6), ("bf16x3", 5e-5)])
+def test_mixed_gemm_matches_fp64(gpu, mode, tol):
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    for (nb, m, k, n) in [(1, 64, 32, 64), (3, 100, 77, 130), (2, 1, 300, 7), (1, 257, 1000, 65)]:
+        A = torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen)
+        B = torch.randn((nb, k, n), dtype=torch.float64, device="cuda", generator=gen)
+        C = mfma.bmm(A, B, mode)
+        ref = torch.bmm(A, B)
+        scale = torch.sqrt(torch.tensor(float(k)))
+        err = float((C - ref).abs().max() / scale)
+        print(mode, (nb, m, k, n), err, flush=True)
+        assert err < tol
+
+
+@pytest.mark.gpu
+def test_contract_mixed_matches_einsum(gpu):
+    from ccsd_gpu_amd.precision import contract
+    gen = torch.Generator(device="cuda").manual_seed(9)
+    o, v = 5, 9
+    t2 = torch.randn((o, o, v, v), dtype=torch.float64, device="cuda", generator=gen)
+    l2 = torch.randn((o, o, v, v), dtype=torch.float64, device="cuda", generator=gen)
+    vvvv = torch.randn((v, v, v, v), dtype=torch.float64, device="cuda", generator=gen)
+    t1 = torch.randn((o, v), dtype=torch.float64, device="cuda", generator=gen)
+    for eq, ops in [("ijcd,acbd->ijab", (l2, vvvv)), ("jkca,bikc->jiba", (l2, t2.permute(2, 0, 1, 3))),
+                    ("kc,kb,jc->bj", (t1, t1, t1)), ("dljd->jl", (t2.permute(2, 0, 1, 3),)),
+                    ("ijab,ijcd->abcd", (l2, t2))]:
+        ref = torch.einsum(eq, *ops)
+        for prec, tol in (("fp32", 1e-5), ("bf16x3", 2e-4)):
+            got = contract(eq, *ops, precision=prec)
+            err = float((got - ref).abs().max() / (ref.abs().max() + 1e-30))
+            assert err < tol, (eq, prec, err)
+
+
+@pytest.fixture(scope="module")
+def h2o_pair():
+    """Oracle RHF/CCSD/lambda and a GPU RHF sharing the oracle's orbitals."""
+    from ccsd_gpu_amd.gto import M
+    from ccsd_gpu_amd import scf
+    mol = M(GEOM, basis="cc-pvdz")
+    omf = chem.RHF(mol)
+    omf.kernel()
+    occ = chem.CCSD(omf)
+    occ.kernel()
+    conv, l1, l2 = chem.solve_lambda(occ.t1, occ.t2, occ.ao2mo())
+    gmf = scf.RHF(mol)
+    gmf.kernel()
+    return mol, omf, occ, (l1, l2), gmf
+
+
+@pytest.mark.gpu
+def test_rhf_and_ccsd_energies_match_oracle(gpu, h2o_pair):
+    from ccsd_gpu_amd import cc
+    mol, omf, occ, _, gmf = h2o_pair
+    print("E_rhf", gmf.e_tot, omf.e_tot, flush=True)
+    assert abs(gmf.e_tot - omf.e_tot) < 1e-9
+    mycc = cc.CCSD(gmf)
+    mycc.kernel()
+    print("E_ccsd", mycc.e_corr, occ.e_corr, flush=True)
+    assert abs(mycc.e_corr - occ.e_corr) < 1e-8
+
+
+def _gpu_cc_with_oracle_orbitals(gmf, omf, occ):
+    from ccsd_gpu_amd import cc
+    gmf.mo_coeff = omf.mo_coeff.copy()
+    gmf.mo_energy = omf.mo_energy.copy()
+    gmf.mo_occ = omf.mo_occ.copy()
+    mycc = cc.CCSD(gmf)
+    mycc.t1 = torch.as_tensor(occ.t1, device="cuda")
+    mycc.t2 = torch.as_tensor(occ.t2, device="cuda")
+    return mycc
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision, tol):
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    mycc.conv_tol_normt = 1e-10
+    g1, g2 = solve_lambda_gpu(mycc, precision=precision)
+    assert mycc.converged_lambda or precision != "fp64"
+    e1 = float(np.abs(g1.cpu().numpy() - l1).max())
+    e2 = float(np.abs(g2.cpu().numpy() - l2).max())
+    print(precision, "l1", e1, "l2", e2, flush=True)
+    assert e1 < tol and e2 < tol
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("fused", [True, False])
+def test_gradient_gpu_matches_oracle(gpu, h2o_pair, fused):
+    from ccsd_gpu_amd.gpu_ccsd_grad import grad_elec_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    ref = chem.grad_elec(occ, occ.t1, occ.t2, l1, l2)
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    de = grad_elec_gpu(mycc, occ.t1, occ.t2, l1, l2, fused=fused)
+    err = np.abs(de - ref).max()
+    print("fused" if fused else "unfused", "grad err", err, flush=True)
+    assert err < 1e-8
+
+
+@pytest.mark.gpu
+def test_forces_entry_point_end_to_end(gpu):
+    """RHF -> CCSD -> lambda -> compute_ml_ccsd_forces_gpu, all on the GPU, vs oracle."""
+    from ccsd_gpu_amd import cc, scf
+    from ccsd_gpu_amd.gpu_ccsd_grad import compute_ml_ccsd_forces_gpu
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    from ccsd_gpu_amd.gto import M
+    mol = M(GEOM, basis="sto-3g")
+    mf = scf.RHF(mol)
+    mf.kernel()
+    mycc = cc.CCSD(mf)
+    mycc.kernel()
+    l1, l2 = solve_lambda_gpu(mycc)
+    de = compute_ml_ccsd_forces_gpu(mf, mycc.t1, mycc.t2, l1, l2)
+    e0, omf, occ = chem.ccsd_total_energy(mol)
+    conv, ol1, ol2 = chem.solve_lambda(occ.t1, occ.t2, occ.ao2mo())
+    ref = chem.grad_elec(occ, occ.t1, occ.t2, ol1, ol2) + chem.grad_nuc(mol)
+    err = np.abs(de - ref).max()
+    print("forces err", err, flush=True)
+    assert err < 1e-6
+    assert np.abs(de.sum(axis=0)).max() < 1e-8
