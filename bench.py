@@ -1,18 +1,23 @@
 """Benchmark of the CCSD-gradient hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config fused_ip1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad|eri] [--waters N]
 
-Workload (default): the ERI-derivative contraction of the autograd backward on
-a seeded water tetramer (H2O)_4 / cc-pVDZ (spherical, 96 AOs; rigid monomers,
-random orientations, O-O 2.8 +- 0.2 A), upstream gradient a seeded
-8-fold-symmetric N(0,1) AO tensor. One step = one fused int2e_ip1 x G pass over
-all shell quartets (dL/dR, natm x 3), inputs resident in HBM. The
-reference-equivalent quartet count (decontracted shells, s2kl enumeration,
-no screening) is the unit of `value`.
+--config grad (default): the full CCSD gradient of a seeded water octamer
+(H2O)8 / cc-pVDZ (192 AOs, nocc 40, nvir 152; rigid monomers, random
+orientations, O-O 2.8 +- 0.2 A). RHF and the CCSD T-solve run on the GPU first
+and are not timed. One step = lambda solve + dE/dR from the converged
+amplitudes with everything resident in HBM (the reference's
+solve_lambda_gpu + compute_ml_ccsd_forces_gpu). `value` is the mixed path's
+wall time per gradient (split-BF16 MFMA lambda/RDM contractions with FP64
+residual refinement, fused int2e_ip1 x Gamma); the plain FP64 path (FP64
+lambda, reference-structured unfused gradient) is timed in the same run and
+reported in config, with the max |difference| of the two gradients.
 
-Prints one JSON line with `roofline` (FP64 vector peak; kernel time from HIP
-events on the launch stream) and `cpu_baseline` (the FP64 C oracle, OpenMP,
-on the host cores, same workload) objects.
+--config eri: the fused int2e_ip1 x G backward pass of the ERI autograd
+function on (H2O)4 (one step = one pass), value = ERI' quartets/s.
+
+Every line carries `roofline` (the dominant kernel: FP64 vector for the ERI
+kernels, time from HIP events on the launch stream) and `cpu_baseline`.
 """
 import argparse
 import json
@@ -26,153 +31,187 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6      # MI355X FP64 vector (and matrix) peak
-HBM_PEAK_GBS = 8000.0
+METRIC = "CCSD-grad wall-s & ERI' quartets/s, mixed vs FP64 plain, @1e-6 Eh/Bohr"
 
 
-def make_workload(seed=1):
+def log(*a):
+    print("#", *a, file=sys.stderr, flush=True)
+
+
+def timed_fused_pass(mol, gp, scale, reps=3):
+    """HIP-event time (ms) of one fused int2e_ip1 x Gamma pass on torch's current stream."""
+    import torch
+    from ccsd_gpu_amd.hip.grad2e import contract_int2e_ip1
+    stream = torch.cuda.current_stream()
+    contract_int2e_ip1(mol, gp, scale=scale)
+    ev = []
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        contract_int2e_ip1(mol, gp, scale=scale)
+        b.record(stream)
+        ev.append((a, b))
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(b) for a, b in ev]))
+
+
+def roofline(mol, kern_ms):
+    from ccsd_gpu_amd import flopcount
+    flops, quartets, ref_quartets = flopcount.eri_work(mol, 1)
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}, flops, quartets, ref_quartets
+
+
+def cpu_eri_rate(mol, nshell=3):
+    """Oracle (C, OpenMP) int2e_ip1 on the first `nshell` i-shells: flops/s of the CPU path."""
+    from ccsd_gpu_amd import flopcount
+    from oracle import integrals as oi
+    mc = mol.copy()
+    mc.cart = True
+    t0 = time.perf_counter()
+    oi.int2e_s2kl_cart(mc, 0, nshell, 1)
+    dt = time.perf_counter() - t0
+    f, _, _ = flopcount.eri_work(mol, 1, 0, nshell)
+    return f / dt, dt, oi.num_threads()
+
+
+def bench_eri(args):
     import torch
     from ccsd_gpu_amd import systems
     from ccsd_gpu_amd.autograd import symmetrize8
-    from ccsd_gpu_amd.hip.grad2e import pack_s2kl, sph_to_cart_4index
-    mol = systems.water_cluster(4, seed=seed, cart=False)
+    from ccsd_gpu_amd.hip.grad2e import contract_int2e_ip1, pack_s2kl, sph_to_cart_4index
+    mol = systems.water_cluster(args.waters or 4, seed=1, cart=False)
     n = mol.nao
-    gen = torch.Generator(device="cuda").manual_seed(seed)
+    gen = torch.Generator(device="cuda").manual_seed(1)
     g = torch.randn((n, n, n, n), dtype=torch.float64, device="cuda", generator=gen)
-    g = symmetrize8(g) * np.sqrt(8.0)          # unit-variance entries, 8-fold symmetric
+    g = symmetrize8(g) * np.sqrt(8.0)
     c2s = torch.as_tensor(mol.cart2sph_coeff(), dtype=torch.float64, device="cuda")
     gp = pack_s2kl(sph_to_cart_4index(g, c2s), halve_diag=True)
     del g
-    return mol, gp
-
-
-def cpu_baseline(mol, gp_cpu, de_gpu):
-    """FP64 C oracle (OpenMP) on the same workload: full int2e_ip1 (Cartesian s2kl) + contraction."""
-    from oracle import integrals as oi
-    mol_c = mol.copy()
-    mol_c.cart = True
-    nthreads = oi.num_threads()
+    for _ in range(args.warmup):
+        contract_int2e_ip1(mol, gp, scale=-8.0)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ip1 = oi.int2e_s2kl_cart(mol_c, 0, mol_c.nbas, 1)            # (3, nao_c, nao_c, npair_c)
-    aoa = np.repeat(np.arange(mol.natm), np.diff(
-        [mol.ao_loc_nr(cart=True)[mol.offset_nr_by_atom()[a, 0]] for a in range(mol.natm)] + [mol.nao_cart()]))
-    per_ao = np.einsum("xijp,ijp->xi", ip1, gp_cpu)
-    de = np.zeros((mol.natm, 3))
-    np.add.at(de, aoa, -8.0 * per_ao.T)
-    dt = time.perf_counter() - t0
-    err = float(np.abs(de - de_gpu).max())
-    return dt, nthreads, err, de
+    for _ in range(args.steps):
+        contract_int2e_ip1(mol, gp, scale=-8.0)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    kern_ms = timed_fused_pass(mol, gp, -8.0)
+    roof, flops, quartets, refq = roofline(mol, kern_ms)
+    cpu = None
+    if not args.no_cpu:
+        rate, dt, nth = cpu_eri_rate(mol)
+        cpu_s = flops / rate
+        cpu = {"value": refq / cpu_s, "unit": "ERI' quartets/s", "cores": nth, "kind": "port",
+               "sample": f"FP64 C oracle int2e_ip1 over i-shells [0,3) of the same molecule ({dt:.1f} s), "
+                         "scaled to the full pass by the flop model"}
+    return {
+        "metric": METRIC, "value": refq / (ms * 1e-3), "unit": "ERI' quartets/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"fused int2e_ip1 x G backward, (H2O){args.waters or 4}/cc-pVDZ, "
+                               "seeded 8-fold-symmetric N(0,1) G",
+                   "quartets_evaluated": quartets, "quartets_reference_equivalent": refq,
+                   "screening": "none"},
+        "roofline": roof, "cpu_baseline": cpu}
+
+
+def bench_grad(args):
+    import torch
+    from ccsd_gpu_amd import pipeline, systems
+    from ccsd_gpu_amd.gpu_ccsd_grad import _pack_halved  # noqa: F401
+    nw = args.waters or 8
+    mol = systems.water_cluster(nw, seed=1, cart=False)
+    log(f"(H2O){nw}/cc-pVDZ nao={mol.nao} nbas={mol.nbas}")
+    t0 = time.perf_counter()
+    mf, mycc = pipeline.setup(mol)
+    torch.cuda.synchronize()
+    log(f"setup (RHF {mf.e_tot:.10f}, CCSD corr {mycc.e_corr:.10f}) {time.perf_counter() - t0:.1f} s")
+    res = []
+    for s in range(args.warmup + args.steps):
+        de, info = pipeline.ccsd_gradient(mf, mycc, "mixed", args.precision)
+        log(f"mixed step {s}: {json.dumps(info)}")
+        if s >= args.warmup:
+            res.append((de, info))
+    wall = float(np.mean([i["total_s"] for _, i in res]))
+    de_mixed = res[-1][0]
+    de_plain, info_plain = pipeline.ccsd_gradient(mf, mycc, "plain")
+    log(f"plain: {json.dumps(info_plain)}")
+    diff = float(np.abs(de_mixed - de_plain).max())
+    net_force = float(np.abs(de_mixed.sum(axis=0)).max())
+    coords = mol.atom_coords()
+    torque = float(np.abs(np.cross(coords, de_mixed).sum(axis=0)).max())
+
+    # dominant ERI kernel: the fused ip1 x Gamma pass, timed live with HIP events
+    nc = mol.nao_cart()
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    gp = torch.randn((nc, nc, nc * (nc + 1) // 2), dtype=torch.float64, device="cuda", generator=gen)
+    kern_ms = timed_fused_pass(mol, gp, -2.0, reps=2)
+    del gp
+    roof, flops, quartets, refq = roofline(mol, kern_ms)
+
+    cpu = None
+    if not args.no_cpu:
+        rate, dt, nth = cpu_eri_rate(mol)
+        # CPU estimate of the same gradient: the ERI passes (int2e for Imat + int2e_ip1) at the
+        # oracle's measured rate, plus the lambda iterations' o^2 v^4 contractions at the
+        # measured numpy rate of the dominant one.
+        from ccsd_gpu_amd import flopcount
+        f0, _, _ = flopcount.eri_work(mol, 0)
+        nocc, nvir = mycc.nocc, mol.nao - mycc.nocc
+        a = np.random.default_rng(0).standard_normal((nocc * nocc, nvir * nvir))
+        b = np.random.default_rng(1).standard_normal((nvir * nvir, nvir * nvir // 8))
+        t1 = time.perf_counter()
+        a @ b
+        tl = time.perf_counter() - t1
+        gemm_rate = 2 * a.shape[0] * a.shape[1] * b.shape[1] / tl
+        lam_flops = (info_plain["lambda_iterations"] or 20) * 4 * 2 * nocc ** 2 * nvir ** 4
+        cpu_s = (flops + f0) / rate + lam_flops / gemm_rate
+        cpu = {"value": cpu_s, "unit": "s per CCSD gradient", "cores": nth, "kind": "port",
+               "sample": f"FP64 C oracle int2e_ip1 on i-shells [0,3) ({dt:.1f} s, {rate / 1e9:.1f} GFLOP/s) "
+                         "scaled by the flop model to the int2e + int2e_ip1 passes, plus "
+                         f"{info_plain['lambda_iterations']} lambda iterations x 4 o^2v^4 GEMMs at the "
+                         f"measured numpy DGEMM rate ({gemm_rate / 1e9:.0f} GFLOP/s); RDM, Z-vector, 1e terms omitted"}
+
+    return {
+        "metric": METRIC, "value": wall, "unit": "s per CCSD gradient (mixed path)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16x3", "data": "synthetic",
+        "config": {"workload": f"CCSD gradient (lambda + dE/dR), (H2O){nw}/cc-pVDZ seeded, nocc {mycc.nocc}, "
+                               f"nvir {mol.nao - mycc.nocc}",
+                   "mixed_precision": args.precision,
+                   "mixed": res[-1][1], "plain_fp64": info_plain,
+                   "plain_fp64_s": info_plain["total_s"],
+                   "speedup_mixed_vs_plain": info_plain["total_s"] / wall,
+                   "max_abs_diff_mixed_vs_plain_Eh_per_Bohr": diff,
+                   "net_force": net_force, "net_torque": torque,
+                   "eri_prime_quartets_per_s": refq / (kern_ms * 1e-3),
+                   "eri_prime_pass_ms": kern_ms,
+                   "quartets_reference_equivalent": refq},
+        "roofline": roof, "cpu_baseline": cpu}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="grad", choices=["grad", "eri"])
+    ap.add_argument("--waters", type=int, default=0)
+    ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
         sys.exit(2)
-
     import torch
-    from ccsd_gpu_amd import _lib, flopcount
-    from ccsd_gpu_amd.hip.grad2e import contract_int2e_ip1
-    from ccsd_gpu_amd.hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
-
+    from ccsd_gpu_amd import _lib
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     _lib.load()
-    mol, gp = make_workload()
-    flops, quartets, ref_quartets = flopcount.eri_work(mol, 1)
-    stream = torch.cuda.current_stream()
-
-    def step():
-        return contract_int2e_ip1(mol, gp, scale=-8.0)
-
-    print(f"# workload (H2O)4/cc-pVDZ nao={mol.nao} nbas={mol.nbas} quartets={quartets} "
-          f"ref_quartets={ref_quartets} flops/pass={flops:.3e}", file=sys.stderr, flush=True)
-    for _ in range(args.warmup):
-        de = step()
-    torch.cuda.synchronize()
-    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        starts[s].record(stream)
-        de = step()
-        ends[s].record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in zip(starts, ends)]))
-    ms_per_step = wall * 1e3 / args.steps
-    de_gpu = de.cpu().numpy()
-    print(f"# fused: {ms_per_step:.3f} ms/step (events {kern_ms:.3f} ms)", file=sys.stderr, flush=True)
-
-    # plain (unfused) FP64 path: ip1 tensor in HBM, then GEMV per atom (reference's structure)
-    mol_c = mol.copy()
-    mol_c.cart = True
-    aoc = torch.as_tensor(np.repeat(np.arange(mol.natm), np.diff(
-        [mol.ao_loc_nr(cart=True)[mol.offset_nr_by_atom()[a, 0]] for a in range(mol.natm)] + [mol.nao_cart()])),
-        device="cuda")
-
-    def plain():
-        ip1 = compute_int2e_ip1_gpu(mol_c, 0, mol_c.nbas)
-        per_ao = torch.einsum("xijp,ijp->xi", ip1, gp)
-        return torch.zeros((mol.natm, 3), dtype=torch.float64, device="cuda").index_add_(0, aoc, -8.0 * per_ao.T)
-
-    de_plain = plain()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    nplain = max(1, args.steps // 4)
-    for _ in range(nplain):
-        de_plain = plain()
-    torch.cuda.synchronize()
-    plain_ms = (time.perf_counter() - t1) * 1e3 / nplain
-    fused_vs_plain = float((de_plain - de).abs().max())
-
-    cpu = None
-    oracle_err = None
-    if not args.no_cpu:
-        dt, nth, oracle_err, _ = cpu_baseline(mol, gp.cpu().numpy(), de_gpu)
-        cpu = {"value": ref_quartets / dt, "unit": "quartets/s", "cores": nth, "kind": "port",
-               "sample": "full (H2O)4/cc-pVDZ int2e_ip1 (FP64 C oracle, OpenMP, s2kl Cartesian) "
-                         f"+ contraction with G, one pass, {dt:.2f} s"}
-
-    achieved = flops / (kern_ms * 1e-3) / 1e12
-    line = {
-        "metric": "CCSD-grad wall-s & ERI' quartets/s, mixed vs FP64 plain, @1e-6 Eh/Bohr",
-        "value": ref_quartets / (ms_per_step * 1e-3),
-        "unit": "ERI' quartets/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "fused int2e_ip1 x G backward, (H2O)4/cc-pVDZ (96 AO sph), seeded 8-fold-symmetric N(0,1) G",
-            "quartets_evaluated": quartets,
-            "quartets_reference_equivalent": ref_quartets,
-            "screening": "none",
-            "plain_unfused_ms": plain_ms,
-            "fused_speedup_vs_plain": plain_ms / ms_per_step,
-            "max_abs_err_fused_vs_plain": fused_vs_plain,
-            "max_abs_err_vs_fp64_oracle": oracle_err,
-        },
-        "roofline": {
-            "bound": "valu",
-            "achieved": achieved,
-            "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": achieved / FP64_PEAK_TFLOPS,
-            "traffic": None,
-        },
-        "cpu_baseline": cpu,
-    }
+    line = bench_grad(args) if args.config == "grad" else bench_eri(args)
     print(json.dumps(line), flush=True)
 
 

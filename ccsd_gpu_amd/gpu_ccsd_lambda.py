@@ -124,7 +124,14 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     return imds
 
 
-def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=True, precision="fp64"):
+def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=True, precision="fp64",
+                      homogeneous=False):
+    """One lambda update l -> F(l) (the reference's update_lambda_gpu).
+
+    The update is affine in (l1, l2): F(l) = A l + b. With homogeneous=True the
+    l-independent terms b (fov, the bare voov term and the bare t1 inside
+    mia1) are left out, giving A l -- used by the mixed-precision solver for
+    its correction equations."""
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
     if imds is None:
         imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
@@ -133,13 +140,14 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     fov = eris.fock[:nocc, nocc:]
     e_o = eris.mo_energy[:nocc]
     e_v = eris.mo_energy[nocc:] + getattr(mycc, "level_shift", 0.0)
+    inh = 0.0 if homogeneous else 1.0
 
     theta = t2 * 2 - t2.permute(0, 1, 3, 2)
     mba = C("klca,klcb->ba", l2, theta)
     mij = C("ikcd,jkcd->ij", l2, theta)
     mba1 = C("jc,jb->bc", l1, t1) + mba
     mij1 = C("kb,jb->kj", l1, t1) + mij
-    mia1 = t1 + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
+    mia1 = inh * t1 + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
     mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
 
     if use_cpu_vvvv:
@@ -151,7 +159,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     w1 = imds.w1 - torch.diag(e_v)
     w2 = imds.w2 - torch.diag(e_o)
-    l1new = l1new + fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
+    l1new = l1new + inh * fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
     l1new = l1new + C("ijab,bj->ia", l2, imds.w3) * 2 - C("ijba,bj->ia", l2, imds.w3)
     l2new = l2new + torch.einsum("ia,jb->ijab", l1, imds.w4)
@@ -176,7 +184,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
-    l2new = l2new + voov.permute(1, 2, 0, 3) * 0.5
+    l2new = l2new + voov.permute(1, 2, 0, 3) * (0.5 * inh)
     l2new = l2new - C("bjic,ca->jiba", voov, mba1) - C("bjka,ik->jiba", voov, mij1)
     l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
     m4 = C("ijkl,aklb->ijab", l2tau, voov)
@@ -213,21 +221,64 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     max_cycle = mycc.max_cycle if max_cycle is None else max_cycle
     tol = mycc.conv_tol_normt if tol is None else tol
     nocc, nvir = t1.shape
-    imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
-    adiis = DIIS(getattr(mycc, "diis_space", 6)) if getattr(mycc, "diis", True) else None
+    # intermediates once in FP64: they enter every FP64 residual
+    imds = make_intermediates_gpu(mycc, t1, t2, eris, "fp64")
+    diis_space = getattr(mycc, "diis_space", 6)
+    use_diis = bool(getattr(mycc, "diis", True))
+
+    def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
+        """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
+        adiis = DIIS(diis_space) if use_diis else None
+        normt = None
+        best, best_it = float("inf"), 0
+        for it in range(ncyc):
+            if rhs is None:
+                n1, n2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec)
+            else:
+                n1, n2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec, homogeneous=True)
+                n1, n2 = n1 + rhs[0], n2 + rhs[1]
+            vec = amplitudes_to_vector(n1, n2)
+            normt = float(torch.linalg.norm(vec - amplitudes_to_vector(l1, l2)))
+            if adiis is not None:
+                vec = adiis.update(vec)
+            l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
+            if normt < tol_:
+                return True, l1, l2, normt, it + 1
+            if prec != "fp64":
+                # low precision has a noise floor: stop once the change norm stalls
+                if normt < 0.9 * best:
+                    best, best_it = normt, it
+                elif it - best_it >= 4:
+                    return False, l1, l2, normt, it + 1
+        return False, l1, l2, normt, ncyc
+
+    if precision == "fp64":
+        conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
+        mycc._lambda_iterations = iters
+        mycc._lambda_fp64_updates = iters
+        mycc._lambda_normt = normt
+        return conv, l1, l2
+
+    # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
+    lp_tol = max(tol, 1e-6 if precision == "fp32" else 1e-5)
+    _, l1, l2, normt, iters = fixed_point(l1, l2, precision, lp_tol, max_cycle)
+    total, fp64_updates = iters, 0
     conv = False
-    normt = None
-    for istep in range(max_cycle):
-        l1n, l2n = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision)
-        vec = amplitudes_to_vector(l1n, l2n)
-        normt = float(torch.linalg.norm(vec - amplitudes_to_vector(l1, l2)))
-        if adiis is not None:
-            vec = adiis.update(vec)
-        l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
+    for _round in range(8):
+        r1, r2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
+        fp64_updates += 1
+        r1, r2 = r1 - l1, r2 - l2
+        normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))
         if normt < tol:
             conv = True
             break
-    mycc._lambda_iterations = istep + 1
+        # correction: d = A d + r, to a relative accuracy well below the remaining residual
+        _, d1, d2, _, it = fixed_point(r1.clone(), r2.clone(), precision, max(normt * 1e-3, tol * 1e-2), max_cycle,
+                                       rhs=(r1, r2))
+        total += it
+        l1, l2 = l1 + d1, l2 + d2
+    mycc._lambda_iterations = total
+    mycc._lambda_fp64_updates = fp64_updates
     mycc._lambda_normt = normt
     return conv, l1, l2
 
