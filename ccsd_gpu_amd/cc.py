@@ -11,9 +11,12 @@ W-intermediates. Tensors are float64 torch tensors on the HIP device.
 import numpy as np
 import torch
 
+from .precision import contract, static_operands
 from .scf import DEV
 
-E = torch.einsum
+
+def E(eq, *ops):
+    return contract(eq, *ops)
 
 
 class ERIs:
@@ -220,6 +223,14 @@ class CCSD:
         e = ccsd_energy(t1, t2, eris)
         diis = DIIS(self.diis_space) if self.diis else None
         self.converged = False
+        with static_operands(eris.ovov, eris.ovoo, eris.ovvo, eris.oovv, eris.ovvv, eris.oooo, eris.vvvv):
+            e, t1, t2 = self._iterate(eris, t1, t2, e, diis, nocc, nvir)
+        self.e_corr = e
+        self.e_tot = self._scf.e_tot + e
+        self.t1, self.t2 = t1, t2
+        return e, t1, t2
+
+    def _iterate(self, eris, t1, t2, e, diis, nocc, nvir):
         for it in range(self.max_cycle):
             t1n, t2n = update_amps(t1, t2, eris)
             vec = amplitudes_to_vector(t1n, t2n)
@@ -233,9 +244,6 @@ class CCSD:
                 self.converged = True
                 break
             e = e_new
-        self.e_corr = e
-        self.e_tot = self._scf.e_tot + e
-        self.t1, self.t2 = t1, t2
         return e, t1, t2
 
     def solve_lambda(self, t1=None, t2=None, l1=None, l2=None, eris=None):

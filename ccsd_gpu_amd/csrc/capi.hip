@@ -21,6 +21,11 @@ hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const 
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
 size_t ccsd_c2s_term_size();
 hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
+hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
+                                      int M, int N, int K, long long sA, long long sB, long long sC, long long ldc,
+                                      int batch, int beta, hipStream_t stream);
+hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
+                                  hipStream_t stream);
 hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
                                   long long lda, long long ldb, long long ldc, long long sA, long long sB,
                                   long long sC, int batch, hipStream_t stream);
@@ -471,6 +476,29 @@ int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M
     hipError_t e = ccsd_launch_gemm_mixed(mode, A, B, C, M, N, K, lda, ldb, ldc, strideA, strideB, strideC, batch,
                                           (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_mixed_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld, void* stream)
+{
+    if (!x || !hi || !lo) return fail(CCSD_ERR_NULL, "null pointer");
+    if (rows < 0 || cols < 0 || ld < cols) return fail(CCSD_ERR_SHAPE, "bad split shape");
+    hipError_t e = ccsd_launch_split_bf16(x, hi, lo, rows, cols, ld, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "split_bf16_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C, int M, int N,
+                        int K, long long strideA, long long strideB, long long strideC, long long ldc, int batch,
+                        int beta, void* stream)
+{
+    if (!Ahi || !Alo || !Bhi || !Blo || !C) return fail(CCSD_ERR_NULL, "null pointer");
+    if (M < 0 || N < 0 || K < 0 || batch < 0 || ldc < N || (beta != 0 && beta != 1))
+        return fail(CCSD_ERR_SHAPE, "bad GEMM arguments");
+    if ((N + 127) / 128 > 65535 || batch > 65535) return fail(CCSD_ERR_SHAPE, "N or batch too large for one launch");
+    hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, strideA, strideB, strideC, ldc, batch,
+                                              beta, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
     return CCSD_OK;
 }
 

@@ -1,0 +1,184 @@
+// Split-BF16 GEMM on the CDNA4 matrix cores: C (+)= A B^T with FP32 accumulation.
+//
+// Operands are stored pre-split as two bfloat16 planes, x = hi + lo
+// (hi = bf16(x), lo = bf16(x - hi)), both K-contiguous:
+//   A{hi,lo}: [batch][M][K],  B{hi,lo}: [batch][N][K]   (the "NT" form)
+// and the product keeps the three significant terms
+//   C = A_hi B_hi^T + A_hi B_lo^T + A_lo B_hi^T
+// on v_mfma_f32_32x32x16_bf16 (about 16 significant bits per operand, FP32
+// accumulation). C is float64 [batch][M][N] (row stride ldc), overwritten
+// (beta = 0) or accumulated into (beta = 1).
+//
+// Block: 256 threads = 4 waves, 128x128 C tile, each wave 64x64 = 2x2 MFMA
+// 32x32 tiles; K staged by 32 through LDS, next tile prefetched into registers
+// while the current one is multiplied.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace {
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (80 bytes)
+constexpr int TILE = BM * LDK;              // bf16 elements of one operand plane tile
+
+__global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
+    const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
+    const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long sA, long long sB,
+    long long sC, long long ldc, int beta)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 lds[4 * TILE];   // Ahi, Alo, Bhi, Blo
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    // blockIdx.x runs over M tiles so that consecutive blocks share a B tile (L2 reuse)
+    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    const long long bz = blockIdx.z;
+    const __bf16* src[4] = {Ahi + bz * sA, Alo + bz * sA, Bhi + bz * sB, Blo + bz * sB};
+    const int rows[4] = {M, M, N, N};
+    const int base[4] = {m0, m0, n0, n0};
+
+    // each thread moves 2 chunks of 8 bf16 (16 bytes) per plane per K tile:
+    // chunk c = tid + 256*h, row = c / 4, kchunk = c % 4
+    u32x4 stage[4][2];
+    auto load_tile = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = tid + 256 * h;
+                const int r = c >> 2, kc = (c & 3) * 8;
+                const int gr = base[p] + r, gk = k0 + kc;
+                u32x4 v = {0u, 0u, 0u, 0u};
+                if (gr < rows[p]) {
+                    const __bf16* s = src[p] + (long long)gr * K + gk;
+                    if (gk + 8 <= K && ((reinterpret_cast<uintptr_t>(s) & 15) == 0)) {
+                        v = *reinterpret_cast<const u32x4*>(s);
+                    } else {
+                        const uint16_t* s16 = reinterpret_cast<const uint16_t*>(s);
+                        __attribute__((aligned(16))) uint16_t tmp[8];
+#pragma unroll
+                        for (int e = 0; e < 8; e++) tmp[e] = (gk + e < K) ? s16[e] : (uint16_t)0;
+                        v = *reinterpret_cast<const u32x4*>(tmp);
+                    }
+                }
+                stage[p][h] = v;
+            }
+    };
+    auto store_tile = [&]() {
+#pragma unroll
+        for (int p = 0; p < 4; p++)
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int c = tid + 256 * h;
+                const int r = c >> 2, kc = (c & 3) * 8;
+                *reinterpret_cast<u32x4*>(lds + p * TILE + r * LDK + kc) = stage[p][h];
+            }
+    };
+
+    v16f acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
+
+    const int half = lane >> 5, l32 = lane & 31;
+    load_tile(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        __syncthreads();           // previous tile fully consumed
+        store_tile();
+        __syncthreads();
+        if (k0 + BK < K) load_tile(k0 + BK);
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int ko = kk + half * 8;
+            v8bf ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = wm * 64 + i * 32 + l32;
+                ah[i] = *reinterpret_cast<const v8bf*>(lds + 0 * TILE + r * LDK + ko);
+                al[i] = *reinterpret_cast<const v8bf*>(lds + 1 * TILE + r * LDK + ko);
+                const int c = wn * 64 + i * 32 + l32;
+                bh[i] = *reinterpret_cast<const v8bf*>(lds + 2 * TILE + c * LDK + ko);
+                bl[i] = *reinterpret_cast<const v8bf*>(lds + 3 * TILE + c * LDK + ko);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+    }
+    // lane holds C[8*(e/4) + 4*half + e%4][l32] of each 32x32 tile
+    double* Cb = C + bz * sC;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int gc = n0 + wn * 64 + j * 32 + l32;
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+                if (gr < M && gc < N) {
+                    double* p = Cb + (long long)gr * ldc + gc;
+                    *p = beta ? *p + (double)acc[i][j][e] : (double)acc[i][j][e];
+                }
+            }
+        }
+}
+
+// x (float64, rows x cols, row stride ld) -> hi, lo (bfloat16, rows x cols contiguous)
+__global__ void split_bf16_kernel(const double* __restrict__ x, uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                  long long rows, long long cols, long long ld)
+{
+    const long long n = rows * cols;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long r = t / cols, c = t - r * cols;
+        const float f = (float)x[r * ld + c];
+        const uint16_t h = f2bf(f);
+        hi[t] = h;
+        lo[t] = f2bf(f - bf2f(h));
+    }
+}
+
+}  // namespace
+
+hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
+                                      int M, int N, int K, long long sA, long long sB, long long sC, long long ldc,
+                                      int batch, int beta, hipStream_t stream)
+{
+    if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
+    dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
+    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, grid, dim3(256), 0, stream, (const __bf16*)Ahi, (const __bf16*)Alo,
+                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, sA, sB, sC, ldc, beta);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
+                                  hipStream_t stream)
+{
+    const long long n = rows * cols;
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3((unsigned)nb), dim3(256), 0, stream, x, (uint16_t*)hi, (uint16_t*)lo, rows,
+                       cols, ld);
+    return hipGetLastError();
+}

@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
-from .precision import contract
+from .precision import contract, static_operands, vvvv_term
 from .scf import DEV
 
 
@@ -151,7 +151,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
 
     if use_cpu_vvvv:
-        l2new = C("ijcd,acbd->ijab", l2, eris.vvvv)
+        l2new = vvvv_term(l2, eris.vvvv, precision)
     else:
         l2new = torch.zeros_like(l2)
     l1new = C("ijab,jb->ia", l2new, t1) * 2 - C("jiab,jb->ia", l2new, t1)
@@ -252,20 +252,26 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
                     return False, l1, l2, normt, it + 1
         return False, l1, l2, normt, ncyc
 
-    if precision == "fp64":
-        conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
-        mycc._lambda_iterations = iters
-        mycc._lambda_fp64_updates = iters
-        mycc._lambda_normt = normt
-        return conv, l1, l2
+    with static_operands(t1, t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo, imds.w1,
+                         imds.w2, imds.w3, imds.w4, imds.wvvov, imds.wvooo, imds.wVOov, imds.wvOOv, imds.woooo):
+        if precision == "fp64":
+            conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
+            mycc._lambda_iterations = iters
+            mycc._lambda_fp64_updates = iters
+            mycc._lambda_normt = normt
+            return conv, l1, l2
+        # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
+        return _mixed_solve(mycc, fixed_point, update_lambda_gpu, t1, t2, l1, l2, eris, imds, use_cpu_vvvv,
+                            precision, tol, max_cycle)
 
-    # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
+
+def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle):
     lp_tol = max(tol, 1e-6 if precision == "fp32" else 1e-5)
     _, l1, l2, normt, iters = fixed_point(l1, l2, precision, lp_tol, max_cycle)
     total, fp64_updates = iters, 0
     conv = False
     for _round in range(8):
-        r1, r2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
+        r1, r2 = update(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
         fp64_updates += 1
         r1, r2 = r1 - l1, r2 - l2
         normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))

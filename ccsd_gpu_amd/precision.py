@@ -1,74 +1,137 @@
-"""Tensor contractions of the lambda / RDM path at a chosen precision.
+"""Tensor contractions of the lambda / RDM / CCSD code at a chosen precision.
 
     contract("ijcd,acbd->ijab", l2, vvvv, precision="fp64" | "fp32" | "bf16x3")
 
-fp64    torch.einsum in float64 (rocBLAS / hipBLASLt DGEMM): the plain path.
-fp32    the einsum is planned into one (batched) GEMM (operands permuted to
-        [batch, m, k] x [batch, k, n]) and run by the HIP MFMA GEMM kernel
-        (v_mfma_f32 FP32 inputs, FP32 accumulation); result back to float64.
-bf16x3  same plan, each operand split A = A_hi + A_lo into two bfloat16
-        parts and C = A_hi B_hi + A_hi B_lo + A_lo B_hi accumulated in FP32 on
-        the bf16 MFMA units.
+Every two-operand contraction is planned into one (batched) GEMM
+C[b] = A[b] B[b] with index groups batch / free-left / contracted / free-right.
+Each operand is viewed as a matrix in the memory order it already has where
+that is possible (a transposed view is passed to the GEMM instead of a copy);
+only operands whose indices are interleaved are permuted into a copy. Then:
+
+fp64    DGEMM (rocBLAS / hipBLASLt via torch.matmul): the plain path.
+bf16x3  contractions of at least MIXED_MIN_FLOPS run on the split-BF16 MFMA
+        kernel (bfloat16 hi/lo planes, three products, FP32 accumulation,
+        float64 out); smaller ones stay FP64. Operands registered with
+        `static_operands` (integrals and intermediates fixed during a solve)
+        are split once and reused.
+fp32    as bf16x3 with the FP32-input MFMA kernel.
 Three-operand expressions are contracted left to right.
 """
+from contextlib import contextmanager
+from math import prod
+
 import torch
 
 from . import mfma
 
+MIXED_MIN_FLOPS = 4e9
+_STATIC = None
 
-def _plan(eq):
-    lhs, out = eq.split("->")
-    a, b = lhs.split(",")
+
+@contextmanager
+def static_operands(*tensors):
+    """Within the block, prepared copies of these tensors are cached (they must not change)."""
+    global _STATIC
+    prev = _STATIC
+    _STATIC = {"ids": {id(t) for t in tensors if t is not None}, "keep": list(tensors), "cache": {}}
+    try:
+        yield
+    finally:
+        _STATIC = prev
+
+
+def _cached(X, key, make):
+    if _STATIC is not None and id(X) in _STATIC["ids"]:
+        k = (id(X),) + key
+        v = _STATIC["cache"].get(k)
+        if v is None:
+            v = make()
+            _STATIC["cache"][k] = v
+        return v
+    return make()
+
+
+def _groups(a, b, out):
     batch = [c for c in out if c in a and c in b]
     contr = [c for c in a if c in b and c not in out]
     left = [c for c in a if c not in b]
     right = [c for c in b if c not in a]
-    return a, b, out, batch, contr, left, right
+    return batch, left, contr, right
+
+
+def _view(X, s, g1, g2, g3, size):
+    """X (index string s) as a (n1, n2, n3) matrix batch in group order g1 g2 g3.
+
+    Returns (tensor, copied): a reshape view when s is g1+g2+g3, a transposed view
+    when s is g1+g3+g2, a permuted copy otherwise."""
+    n1, n2, n3 = prod(size[c] for c in g1), prod(size[c] for c in g2), prod(size[c] for c in g3)
+    if s == "".join(g1 + g2 + g3) and X.is_contiguous():
+        return X.reshape(n1, n2, n3), False
+    if s == "".join(g1 + g3 + g2) and X.is_contiguous():
+        return X.reshape(n1, n3, n2).transpose(1, 2), False
+    perm = [s.index(c) for c in g1 + g2 + g3]
+    return X.permute(perm).reshape(n1, n2, n3), True
+
+
+def _order_within(group, s):
+    return sorted(group, key=s.index)
 
 
 def _gemm_contract(eq, A, B, precision):
-    a, b, out, batch, contr, left, right = _plan(eq)
-    # indices present in only one operand and absent from the output are summed first
+    lhs, out = eq.split("->")
+    a, b = lhs.split(",")
+    # indices that appear in one operand only and not in the output are summed first
     for idx in [c for c in a if c not in b and c not in out]:
         A = A.sum(dim=a.index(idx))
         a = a.replace(idx, "")
     for idx in [c for c in b if c not in a and c not in out]:
         B = B.sum(dim=b.index(idx))
         b = b.replace(idx, "")
-    a, b, out, batch, contr, left, right = _plan(f"{a},{b}->{out}")
-    sa = dict(zip(a, A.shape))
-    sb = dict(zip(b, B.shape))
-    size = {**sa, **sb}
-    nb = 1
-    for c in batch:
-        nb *= size[c]
-    m = 1
-    for c in left:
-        m *= size[c]
-    k = 1
-    for c in contr:
-        k *= size[c]
-    n = 1
-    for c in right:
-        n *= size[c]
-    Ap = A.permute([a.index(c) for c in batch + left + contr]).reshape(nb, m, k)
-    Bp = B.permute([b.index(c) for c in batch + contr + right]).reshape(nb, k, n)
-    Cp = mfma.bmm(Ap, Bp, precision)
+    size = {**dict(zip(a, A.shape)), **dict(zip(b, B.shape))}
+    batch, left, contr, right = _groups(a, b, out)
+    batch = _order_within(batch, a)
+    left = _order_within(left, a)
+    right = _order_within(right, b)
+    # contracted order: follow A unless B then avoids a copy and A does not need one anyway
+    ca, cb = _order_within(contr, a), _order_within(contr, b)
+    contr = ca
+    if ca != cb:
+        a_ok = a in ("".join(batch + left + ca), "".join(batch + ca + left))
+        b_ok_cb = b in ("".join(batch + cb + right), "".join(batch + right + cb))
+        if b_ok_cb and not a_ok:
+            contr = cb
+    nb, m, k, n = prod(size[c] for c in batch), prod(size[c] for c in left), \
+        prod(size[c] for c in contr), prod(size[c] for c in right)
+    flops = 2.0 * nb * m * n * k
+    if precision == "bf16x3" and flops >= MIXED_MIN_FLOPS:
+        As = _cached(A, (a, "nt-a", tuple(batch + left + contr)),
+                     lambda: mfma.split(_view(A, a, batch, left, contr, size)[0]))
+        Bs = _cached(B, (b, "nt-b", tuple(batch + right + contr)),
+                     lambda: mfma.split(_view(B, b, batch, right, contr, size)[0]))
+        Cp = mfma.gemm_nt(As, Bs)
+    else:
+        Av = _cached(A, (a, "mk", tuple(batch + left + contr)),
+                     lambda: _view(A, a, batch, left, contr, size)[0])
+        Bv = _cached(B, (b, "kn", tuple(batch + contr + right)),
+                     lambda: _view(B, b, batch, contr, right, size)[0])
+        if precision == "fp32" and flops >= MIXED_MIN_FLOPS:
+            Cp = mfma.bmm(Av, Bv, "fp32")
+        elif nb == 1:
+            Cp = torch.mm(Av[0], Bv[0]).unsqueeze(0)
+        else:
+            Cp = torch.bmm(Av, Bv)
     C = Cp.reshape([size[c] for c in batch + left + right])
     order = batch + left + right
     return C.permute([order.index(c) for c in out])
 
 
 def contract(eq, *ops, precision="fp64"):
-    if precision == "fp64":
-        return torch.einsum(eq, *ops)
     lhs, out = eq.replace(" ", "").split("->")
     terms = lhs.split(",")
     if len(terms) == 1:
         return torch.einsum(eq, *ops)
     if len(terms) == 2:
         return _gemm_contract(f"{terms[0]},{terms[1]}->{out}", ops[0], ops[1], precision)
-    # left to right: keep every index still needed later or in the output
     cur, cur_t = ops[0], terms[0]
     for i in range(1, len(terms)):
         rest = "".join(terms[i + 1:]) + out
@@ -78,3 +141,25 @@ def contract(eq, *ops, precision="fp64"):
     if cur_t != out:
         cur = cur.permute([cur_t.index(c) for c in out])
     return cur
+
+
+def vvvv_term(l2, vvvv, precision):
+    """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update).
+
+    X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so the mixed path
+    evaluates only i <= j (half the o^2 v^4 work) and mirrors the rest."""
+    if precision == "fp64":
+        return contract("ijcd,acbd->ijab", l2, vvvv)
+    o, _, v, _ = l2.shape
+    iu, ju = torch.triu_indices(o, o, device=l2.device)
+    A = l2[iu, ju].reshape(1, -1, v * v)
+    if precision == "bf16x3":
+        B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
+        X = mfma.gemm_nt(mfma.split(A), B)[0]
+    else:
+        X = mfma.bmm(A, vvvv.permute(1, 3, 0, 2).reshape(1, v * v, v * v), precision)[0]
+    X = X.reshape(-1, v, v)
+    out = torch.empty_like(l2)
+    out[iu, ju] = X
+    out[ju, iu] = X.transpose(1, 2)
+    return out

@@ -48,6 +48,42 @@ def test_mixed_gemm_matches_fp64(gpu, mode, tol):
 
 
 @pytest.mark.gpu
+def test_bf16x3_nt_gemm_matches_fp64(gpu):
+    """Split-BF16 NT GEMM (pre-split planes) against FP64, ragged shapes, batch, beta."""
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    for (nb, m, k, n) in [(1, 128, 32, 128), (2, 100, 77, 130), (1, 1, 300, 7), (3, 257, 1000, 65),
+                          (1, 300, 64, 1000)]:
+        A = torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen)
+        B = torch.randn((nb, n, k), dtype=torch.float64, device="cuda", generator=gen)
+        ref = A @ B.transpose(1, 2)
+        C = mfma.gemm_nt(mfma.split(A), mfma.split(B))
+        err = float((C - ref).abs().max()) / k ** 0.5
+        print("bf16x3 nt", (nb, m, k, n), err, flush=True)
+        assert err < 2e-4
+        C2 = mfma.gemm_nt(mfma.split(A), mfma.split(B), out=C.clone(), beta=1)
+        assert float((C2 - 2 * ref).abs().max()) / k ** 0.5 < 4e-4
+
+
+@pytest.mark.gpu
+def test_vvvv_term_symmetric_half_matches_full(gpu):
+    from ccsd_gpu_amd.precision import static_operands, vvvv_term
+    gen = torch.Generator(device="cuda").manual_seed(3)
+    o, v = 6, 11
+    x = torch.randn((v, v, v, v), dtype=torch.float64, device="cuda", generator=gen)
+    vvvv = x + x.permute(1, 0, 2, 3)
+    vvvv = vvvv + vvvv.permute(0, 1, 3, 2)
+    vvvv = vvvv + vvvv.permute(2, 3, 0, 1)             # 8-fold symmetric (ac|bd)
+    l = torch.randn((o, o, v, v), dtype=torch.float64, device="cuda", generator=gen)
+    l2 = l + l.permute(1, 0, 3, 2)
+    ref = torch.einsum("ijcd,acbd->ijab", l2, vvvv)
+    with static_operands(vvvv):
+        for _ in range(2):
+            got = vvvv_term(l2, vvvv, "bf16x3")
+            assert float((got - ref).abs().max() / ref.abs().max()) < 1e-4
+
+
+@pytest.mark.gpu
 def test_contract_mixed_matches_einsum(gpu):
     from ccsd_gpu_amd.precision import contract
     gen = torch.Generator(device="cuda").manual_seed(9)
