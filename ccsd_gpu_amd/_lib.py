@@ -24,6 +24,7 @@ _SIGNATURES = {
     "ccsd_int2e_ip1_s2kl": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
     "ccsd_grad_ip1_gamma": (c_int, [c_void_p, c_void_p, c_void_p, c_double, c_int, c_int, c_void_p]),
     "ccsd_int1e": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "ccsd_plan_set_option": (c_int, [c_void_p, c_int, c_int]),
     "ccsd_gemm_mixed": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, ctypes.c_longlong,
                                 ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                 ctypes.c_longlong, c_int, c_void_p]),

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libccsd_gpu_amd.so")
-SOURCES = ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "capi.hip"]
+SOURCES = ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip", "capi.hip"]
 ARCH = os.environ.get("CCSD_GPU_ARCH", "gfx950")
 
 

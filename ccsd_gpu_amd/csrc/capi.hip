@@ -21,6 +21,8 @@ hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const 
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
 size_t ccsd_c2s_term_size();
 hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
+bool ccsd_fused_reg_supported(int li, int lj, int lk, int ll, int max_nckl);
+hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long sA, long long sB, long long sC, long long ldc,
                                       int batch, int beta, hipStream_t stream);
@@ -86,6 +88,7 @@ struct ccsd_plan {
     std::vector<PairClass> ket;                      // ksh >= lsh
     std::map<std::pair<int, int>, std::vector<PairClass>> bra_cache;
     bool has_c2s = false;
+    bool fused_reg = true;                           // register kernels for the fused gradient
     std::vector<C2sTermHost> c2s;                    // per spherical AO
     void* d_c2s = nullptr;
 };
@@ -115,6 +118,13 @@ std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>&
         c.max_np = std::max(c.max_np, a.nprim * b.nprim);
         c.max_nc = std::max(c.max_nc, a.nctr * b.nctr);
     }
+    // similar primitive counts next to each other (threads of a wave loop alike)
+    for (PairClass& c : cls)
+        std::stable_sort(c.pairs.begin(), c.pairs.end(), [p](const int2& x, const int2& y) {
+            const ShellInfo &a0 = p->shells[x.x], &a1 = p->shells[x.y];
+            const ShellInfo &b0 = p->shells[y.x], &b1 = p->shells[y.y];
+            return a0.nprim * a1.nprim * a0.nctr * a1.nctr > b0.nprim * b1.nprim * b0.nctr * b1.nctr;
+        });
     return cls;
 }
 
@@ -220,6 +230,12 @@ int run_classes(ccsd_plan* p, std::vector<PairClass>& bra, int deriv, double* ou
             a.gamma_scale = scale;
             const long long nblk = (long long)a.nbra * a.nket;
             if (nblk > 2147483647LL) return fail(CCSD_ERR_SHAPE, "too many quartets in one class launch");
+            if (!out && p->fused_reg && a.nbra <= 65535 &&
+                ccsd_fused_reg_supported(a.li, a.lj, a.lk, a.ll, a.max_nckl)) {
+                hipError_t e = ccsd_launch_fused_reg(a, stream);
+                if (e != hipSuccess) return hip_fail(e, "fused_ip1_reg_kernel launch");
+                continue;
+            }
             if (ccsd_eri_lds_bytes(a) > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
             hipError_t e = ccsd_launch_eri_class(a, stream);
             if (e != hipSuccess) return hip_fail(e, "eri_quartet_kernel launch");
@@ -500,6 +516,17 @@ int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const
                                               beta, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
     return CCSD_OK;
+}
+
+int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (option == 0) {
+        p->fused_reg = value != 0;
+        return CCSD_OK;
+    }
+    return fail(CCSD_ERR_SHAPE, "unknown option");
 }
 
 int ccsd_rys_roots(int n, const double* T, int count, double* r, double* w, void* stream)

@@ -62,6 +62,12 @@ int ccsd_int2e_ip1_s2kl(ccsd_plan* plan, double* out, int ish0, int ish1, int ca
 int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double scale, int ish0, int ish1,
                         void* stream);
 
+/* Plan options. option 0: 1 (default) lets ccsd_grad_ip1_gamma use the
+ * register-resident per-class kernels where a class fits, 0 forces the LDS
+ * kernel for every class (comparison and testing). No counterpart in the
+ * reference (one kernel there). */
+int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
+
 /* One-electron integrals in the Cartesian AO basis (row-major nao_c x nao_c
  * matrices, fully overwritten): kind 0 overlap, 1 kinetic, 2 nuclear attraction
  * (all nuclei), 3 <d/dr i|j> (3 matrices), 4 <d/dr i|T|j> (3), 5 <d/dr i|-Z_C/|r-C||j>

@@ -98,3 +98,21 @@ def test_eri_autograd_backward_vs_finite_difference(gpu):
     ref = np.zeros((mol.natm, 3))
     np.add.at(ref, mol.ao_atom_index(), -4.0 * per.T)
     assert np.abs(grad - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("make", [molecules.h2o_ccpvdz, molecules.ch4_ccpvdz, molecules.hf_sto3g])
+def test_register_kernels_equal_lds_kernel(gpu, make):
+    """The per-class register kernels and the generic LDS kernel give the same contraction."""
+    from ccsd_gpu_amd import _lib
+    mol = make()
+    g_t = torch.as_tensor(_random_gamma(mol, 21), device="cuda")
+    plan = _lib.get_plan(mol)
+    lib = _lib.load()
+    _lib.check(lib.ccsd_plan_set_option(plan, 0, 0), "set_option")
+    ref = contract_int2e_ip1(mol, g_t).cpu().numpy()
+    _lib.check(lib.ccsd_plan_set_option(plan, 0, 1), "set_option")
+    got = contract_int2e_ip1(mol, g_t).cpu().numpy()
+    scale = np.abs(ref).max()
+    print("reg vs lds", np.abs(got - ref).max() / scale, flush=True)
+    assert np.abs(got - ref).max() <= 1e-11 * scale
