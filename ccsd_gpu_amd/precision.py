@@ -104,15 +104,15 @@ def _gemm_contract(eq, A, B, precision):
         prod(size[c] for c in contr), prod(size[c] for c in right)
     flops = 2.0 * nb * m * n * k
     if precision == "bf16x3" and flops >= MIXED_MIN_FLOPS:
-        As = _cached(A, (a, "nt-a", tuple(batch + left + contr)),
+        As = _cached(A, (a, "nt-a", tuple(batch), tuple(left), tuple(contr)),
                      lambda: mfma.split(_view(A, a, batch, left, contr, size)[0]))
-        Bs = _cached(B, (b, "nt-b", tuple(batch + right + contr)),
+        Bs = _cached(B, (b, "nt-b", tuple(batch), tuple(right), tuple(contr)),
                      lambda: mfma.split(_view(B, b, batch, right, contr, size)[0]))
         Cp = mfma.gemm_nt(As, Bs)
     else:
-        Av = _cached(A, (a, "mk", tuple(batch + left + contr)),
+        Av = _cached(A, (a, "mk", tuple(batch), tuple(left), tuple(contr)),
                      lambda: _view(A, a, batch, left, contr, size)[0])
-        Bv = _cached(B, (b, "kn", tuple(batch + contr + right)),
+        Bv = _cached(B, (b, "kn", tuple(batch), tuple(contr), tuple(right)),
                      lambda: _view(B, b, batch, contr, right, size)[0])
         if precision == "fp32" and flops >= MIXED_MIN_FLOPS:
             Cp = mfma.bmm(Av, Bv, "fp32")

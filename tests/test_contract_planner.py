@@ -44,3 +44,26 @@ def test_planner_handles_transposed_and_permuted_views():
     for eq, ops in [("ijkl,klm->ijm", (x, y)), ("ijkl,klm->mij", (x, y)),
                     ("jilk,klm->ijm", (x.permute(1, 0, 3, 2), y)), ("ijkl,klm->ijm", (x, y.transpose(0, 1).contiguous().transpose(0, 1)))]:
         assert torch.allclose(contract(eq, *ops), torch.einsum(eq, *ops), rtol=1e-12, atol=1e-12)
+
+
+def test_static_cache_reuses_operands_in_several_roles():
+    """One static tensor used with different index groupings (CCSD update on random integrals)."""
+    from ccsd_gpu_amd import cc
+    o, v = 4, 7
+    g = torch.Generator().manual_seed(1)
+    R = lambda *s: torch.randn(s, dtype=torch.float64, generator=g) * 0.1  # noqa: E731
+
+    class Eris:
+        pass
+    e = Eris()
+    e.fock = torch.diag(torch.cat([torch.linspace(-2, -0.5, o), torch.linspace(0.2, 2, v)])).double()
+    e.mo_energy = torch.diagonal(e.fock).clone()
+    e.oooo, e.ovoo, e.ovvo = R(o, o, o, o), R(o, v, o, o), R(o, v, v, o)
+    e.oovv, e.ovov, e.ovvv, e.vvvv = R(o, o, v, v), R(o, v, o, v), R(o, v, v, v), R(v, v, v, v)
+    t1, t2 = R(o, v), R(o, o, v, v)
+    ref = cc.update_amps(t1, t2, e)
+    with static_operands(e.ovov, e.ovoo, e.ovvo, e.oovv, e.ovvv, e.oooo, e.vvvv):
+        for _ in range(2):
+            got = cc.update_amps(t1, t2, e)
+            assert torch.allclose(got[0], ref[0], rtol=1e-12, atol=1e-14)
+            assert torch.allclose(got[1], ref[1], rtol=1e-12, atol=1e-14)
