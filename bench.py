@@ -126,7 +126,7 @@ def bench_grad(args):
     mol = systems.water_cluster(nw, seed=1, cart=False)
     log(f"(H2O){nw}/cc-pVDZ nao={mol.nao} nbas={mol.nbas}")
     t0 = time.perf_counter()
-    mf, mycc = pipeline.setup(mol)
+    mf, mycc = pipeline.setup(mol, log=log)
     torch.cuda.synchronize()
     log(f"setup (RHF {mf.e_tot:.10f}, CCSD corr {mycc.e_corr:.10f}) {time.perf_counter() - t0:.1f} s")
     res = []

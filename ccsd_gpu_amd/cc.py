@@ -244,6 +244,7 @@ class CCSD:
                 self.converged = True
                 break
             e = e_new
+        self._iterations = it + 1
         return e, t1, t2
 
     def solve_lambda(self, t1=None, t2=None, l1=None, l2=None, eris=None):

@@ -22,12 +22,27 @@ from .gpu_ccsd_grad import compute_ml_ccsd_forces_gpu
 from .gpu_ccsd_lambda import solve_lambda_gpu
 
 
-def setup(mol, conv_tol_normt=1e-8):
-    mf = scf.RHF(mol)
+def setup(mol, conv_tol_normt=1e-8, log=None):
+    """RHF and CCSD on the GPU; `log` (callable) receives per-phase wall times."""
+    def tick(name, t0):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if log:
+            log(f"setup {name}: {t1 - t0:.2f} s")
+        return t1
+
+    t0 = time.perf_counter()
+    ints = scf.AOIntegrals(mol)
+    t0 = tick("AO integrals", t0)
+    mf = scf.RHF(mol, ints)
     mf.kernel()
+    t0 = tick("RHF", t0)
     mycc = cc.CCSD(mf)
     mycc.conv_tol_normt = conv_tol_normt
-    mycc.kernel()
+    eris = mycc.ao2mo()
+    t0 = tick("MO integrals", t0)
+    mycc.kernel(eris)
+    tick(f"CCSD ({getattr(mycc, '_iterations', '?')} iterations)", t0)
     return mf, mycc
 
 
