@@ -32,7 +32,7 @@ _SIGNATURES = {
                                 ctypes.c_longlong, c_void_p]),
     "ccsd_gemm_bf16x3_nt": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                     ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
-                                    c_int, c_int, c_void_p]),
+                                    ctypes.c_longlong, ctypes.c_longlong, c_int, c_int, c_void_p]),
     "ccsd_rys_roots": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
 }
 EXPORTED = tuple(_SIGNATURES)

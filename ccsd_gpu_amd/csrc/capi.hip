@@ -24,8 +24,8 @@ hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream
 bool ccsd_fused_reg_supported(int li, int lj, int lk, int ll, int max_nckl);
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
-                                      int M, int N, int K, long long sA, long long sB, long long sC, long long ldc,
-                                      int batch, int beta, hipStream_t stream);
+                                      int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
+                                      long long sC, long long ldc, int batch, int beta, hipStream_t stream);
 hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
                                   hipStream_t stream);
 hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
@@ -505,15 +505,15 @@ int ccsd_split_bf16(const double* x, void* hi, void* lo, long long rows, long lo
 }
 
 int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C, int M, int N,
-                        int K, long long strideA, long long strideB, long long strideC, long long ldc, int batch,
-                        int beta, void* stream)
+                        int K, long long lda, long long ldb, long long strideA, long long strideB, long long strideC,
+                        long long ldc, int batch, int beta, void* stream)
 {
     if (!Ahi || !Alo || !Bhi || !Blo || !C) return fail(CCSD_ERR_NULL, "null pointer");
-    if (M < 0 || N < 0 || K < 0 || batch < 0 || ldc < N || (beta != 0 && beta != 1))
+    if (M < 0 || N < 0 || K < 0 || batch < 0 || lda < K || ldb < K || ldc < N || (beta != 0 && beta != 1))
         return fail(CCSD_ERR_SHAPE, "bad GEMM arguments");
     if ((N + 127) / 128 > 65535 || batch > 65535) return fail(CCSD_ERR_SHAPE, "N or batch too large for one launch");
-    hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, strideA, strideB, strideC, ldc, batch,
-                                              beta, (hipStream_t)stream);
+    hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, strideA, strideB, strideC, ldc,
+                                              batch, beta, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
     return CCSD_OK;
 }

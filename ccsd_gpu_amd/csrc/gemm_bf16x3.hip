@@ -36,8 +36,8 @@ constexpr int TILE = BM * LDK;              // bf16 elements of one operand plan
 
 __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
-    const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long sA, long long sB,
-    long long sC, long long ldc, int beta)
+    const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long lda, long long ldb,
+    long long sA, long long sB, long long sC, long long ldc, int beta)
 {
     __shared__ __attribute__((aligned(16))) __bf16 lds[4 * TILE];   // Ahi, Alo, Bhi, Blo
     const int tid = threadIdx.x;
@@ -49,6 +49,7 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const __bf16* src[4] = {Ahi + bz * sA, Alo + bz * sA, Bhi + bz * sB, Blo + bz * sB};
     const int rows[4] = {M, M, N, N};
     const int base[4] = {m0, m0, n0, n0};
+    const long long ld[4] = {lda, lda, ldb, ldb};
 
     // each thread moves 2 chunks of 8 bf16 (16 bytes) per plane per K tile:
     // chunk c = tid + 256*h, row = c / 4, kchunk = c % 4
@@ -63,7 +64,7 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
                 const int gr = base[p] + r, gk = k0 + kc;
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (gr < rows[p]) {
-                    const __bf16* s = src[p] + (long long)gr * K + gk;
+                    const __bf16* s = src[p] + (long long)gr * ld[p] + gk;
                     if (gk + 8 <= K && ((reinterpret_cast<uintptr_t>(s) & 15) == 0)) {
                         v = *reinterpret_cast<const u32x4*>(s);
                     } else {
@@ -161,13 +162,13 @@ __global__ void split_bf16_kernel(const double* __restrict__ x, uint16_t* __rest
 }  // namespace
 
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
-                                      int M, int N, int K, long long sA, long long sB, long long sC, long long ldc,
-                                      int batch, int beta, hipStream_t stream)
+                                      int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
+                                      long long sC, long long ldc, int batch, int beta, hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
     dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
     hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, grid, dim3(256), 0, stream, (const __bf16*)Ahi, (const __bf16*)Alo,
-                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, sA, sB, sC, ldc, beta);
+                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta);
     return hipGetLastError();
 }
 

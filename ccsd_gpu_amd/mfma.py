@@ -60,12 +60,36 @@ def split(x):
     return Split(hi, lo)
 
 
-def gemm_nt(A, B, out=None, beta=0):
-    """C[b] = A[b] @ B[b]^T (+ C if beta) with A: Split (nb, m, k), B: Split (nb, n, k) -> float64 (nb, m, n)."""
+def k_splits(tiles, k, want=1024, min_chunk=1024):
+    """Number of K chunks for a GEMM with `tiles` output tiles: small outputs with a long
+    contracted dimension launch too few workgroups to fill 256 CUs, so K is cut into
+    chunks that run as a batch and are summed afterwards."""
+    if tiles * 4 >= want or k < 2 * min_chunk:
+        return 1
+    return max(1, min(-(-want // tiles), k // min_chunk))
+
+
+def _launch_nt(lib, A, B, C, a_off, b_off, m, n, kc, lda, ldb, sA, sB, sC, ldc, batch, beta):
+    ptr = _lib.c_void_p
+    e = A.hi.element_size()
+    _lib.check(lib.ccsd_gemm_bf16x3_nt(
+        ptr(A.hi.data_ptr() + a_off * e), ptr(A.lo.data_ptr() + a_off * e),
+        ptr(B.hi.data_ptr() + b_off * e), ptr(B.lo.data_ptr() + b_off * e),
+        ptr(C.data_ptr()), m, n, kc, lda, ldb, sA, sB, sC, ldc, batch, int(beta),
+        _lib.current_stream_ptr()), "ccsd_gemm_bf16x3_nt")
+
+
+def gemm_nt(A, B, out=None, beta=0, ksplit=None):
+    """C[b] = A[b] @ B[b]^T (+ C if beta) with A: Split (nb, m, k), B: Split (nb, n, k) -> float64 (nb, m, n).
+
+    `ksplit` chunks of K run as one batched launch into a workspace that is then
+    summed (None: chosen from the output tile count, see k_splits)."""
     nb, m, k = A.shape
     nb2, n, k2 = B.shape
     if nb != nb2 or k != k2:
         raise ValueError("gemm_nt shape mismatch")
+    if not (A.hi.is_contiguous() and A.lo.is_contiguous() and B.hi.is_contiguous() and B.lo.is_contiguous()):
+        raise ValueError("gemm_nt needs contiguous planes")
     if out is None:
         out = torch.empty((nb, m, n), dtype=torch.float64, device=A.hi.device)
         beta = 0
@@ -74,11 +98,25 @@ def gemm_nt(A, B, out=None, beta=0):
     if k == 0:
         return out if beta else out.zero_()
     lib = _lib.load()
+    if ksplit is None:
+        ksplit = k_splits(nb * -(-m // 128) * -(-n // 128), k) if nb <= 8 else 1
+    if ksplit > 1:
+        kc = -(-(-(-k // ksplit)) // 32) * 32          # chunk length, a multiple of the K tile
+        full, rest = divmod(k, kc)
+        ws = torch.empty((full + (rest > 0), m, n), dtype=torch.float64, device=out.device)
+        for b in range(nb):
+            if full:
+                _launch_nt(lib, A, B, ws, b * m * k, b * n * k, m, n, kc, k, k, kc, kc, m * n, n, full, 0)
+            if rest:
+                _launch_nt(lib, A, B, ws[full], b * m * k + full * kc, b * n * k + full * kc, m, n, rest,
+                           k, k, 0, 0, 0, n, 1, 0)
+            if beta:
+                out[b] += ws.sum(0)
+            else:
+                torch.sum(ws, 0, out=out[b])
+        return out
     for b0 in range(0, nb, 65535):
         b1 = min(nb, b0 + 65535)
-        _lib.check(lib.ccsd_gemm_bf16x3_nt(
-            _lib.c_void_p(A.hi[b0].data_ptr()), _lib.c_void_p(A.lo[b0].data_ptr()),
-            _lib.c_void_p(B.hi[b0].data_ptr()), _lib.c_void_p(B.lo[b0].data_ptr()),
-            _lib.c_void_p(out[b0].data_ptr()), m, n, k, m * k, n * k, m * n, n, b1 - b0, int(beta),
-            _lib.current_stream_ptr()), "ccsd_gemm_bf16x3_nt")
+        _launch_nt(lib, A, B, out[b0], b0 * m * k, b0 * n * k, m, n, k, k, k, m * k, n * k, m * n, n, b1 - b0,
+                   beta)
     return out

@@ -77,6 +77,27 @@ def _order_within(group, s):
     return sorted(group, key=s.index)
 
 
+def _dgemm(Av, Bv):
+    """FP64 batched product (nb, m, k) x (nb, k, n) on the BLAS library. Outputs with few
+    tiles and a long K are split along K (strided views of both operands, one batched
+    call, then a sum), otherwise such a product runs on a handful of workgroups."""
+    nb, m, k = Av.shape
+    n = Bv.shape[2]
+    s = mfma.k_splits(nb * -(-m // 64) * -(-n // 64), k, want=512, min_chunk=2048) if nb <= 8 else 1
+    if s <= 1:
+        return torch.mm(Av[0], Bv[0]).unsqueeze(0) if nb == 1 else torch.bmm(Av, Bv)
+    kc = k // s
+    out = torch.empty((nb, m, n), dtype=Av.dtype, device=Av.device)
+    for b in range(nb):
+        a, bb = Av[b], Bv[b]
+        a3 = a.as_strided((s, m, kc), (kc * a.stride(1), a.stride(0), a.stride(1)))
+        b3 = bb.as_strided((s, kc, n), (kc * bb.stride(0), bb.stride(0), bb.stride(1)))
+        torch.sum(torch.bmm(a3, b3), 0, out=out[b])
+        if s * kc < k:
+            out[b] += a[:, s * kc:] @ bb[s * kc:]
+    return out
+
+
 def _gemm_contract(eq, A, B, precision):
     lhs, out = eq.split("->")
     a, b = lhs.split(",")
@@ -116,10 +137,8 @@ def _gemm_contract(eq, A, B, precision):
                      lambda: _view(B, b, batch, contr, right, size)[0])
         if precision == "fp32" and flops >= MIXED_MIN_FLOPS:
             Cp = mfma.bmm(Av, Bv, "fp32")
-        elif nb == 1:
-            Cp = torch.mm(Av[0], Bv[0]).unsqueeze(0)
         else:
-            Cp = torch.bmm(Av, Bv)
+            Cp = _dgemm(Av, Bv)
     C = Cp.reshape([size[c] for c in batch + left + right])
     order = batch + left + right
     return C.permute([order.index(c) for c in out])

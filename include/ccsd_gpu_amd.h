@@ -93,15 +93,16 @@ int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M
  * contractions are FP64 CuPy einsums). */
 int ccsd_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld, void* stream);
 
-/* C[b] (+)= A[b] B[b]^T from split-BF16 operands (A planes [batch][M][K], B planes
- * [batch][N][K], both K-contiguous), three bf16 MFMA products per tile
+/* C[b] (+)= A[b] B[b]^T from split-BF16 operands (A planes [batch][M][K] with row
+ * stride lda, B planes [batch][N][K] with row stride ldb, both K-contiguous;
+ * strides in elements), three bf16 MFMA products per tile
  * (hi.hi + hi.lo + lo.hi), FP32 accumulation, float64 C [batch][M][N] with row
  * stride ldc; beta 0 overwrites, 1 accumulates. The mixed-precision engine for
  * the o^2 v^4 / o^3 v^3 contractions of the lambda and RDM code
  * (ccsd_gpu/gpu_ccsd_lambda.py:125, 132, 238-243; ccsd_gpu/gpu_ccsd_grad.py:222). */
 int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C, int M, int N,
-                        int K, long long strideA, long long strideB, long long strideC, long long ldc, int batch,
-                        int beta, void* stream);
+                        int K, long long lda, long long ldb, long long strideA, long long strideB, long long strideC,
+                        long long ldc, int batch, int beta, void* stream);
 
 /* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
  * arrays; r, w are (count, n)). Counterpart of rys_roots_simple

@@ -66,6 +66,25 @@ def test_bf16x3_nt_gemm_matches_fp64(gpu):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("nb,m,k,n,ksplit", [(1, 40, 70000, 40, None), (2, 50, 5000, 30, 3), (1, 7, 4099, 130, 5),
+                                             (1, 152, 30000, 152, None)])
+def test_bf16x3_nt_split_k(gpu, nb, m, k, n, ksplit):
+    """K split into chunks (one batched launch + ragged last chunk, summed) against FP64."""
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(k)
+    A = torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen)
+    B = torch.randn((nb, n, k), dtype=torch.float64, device="cuda", generator=gen)
+    ref = A @ B.transpose(1, 2)
+    As, Bs = mfma.split(A), mfma.split(B)
+    C = mfma.gemm_nt(As, Bs, ksplit=ksplit)
+    err = float((C - ref).abs().max()) / k ** 0.5
+    print("split-k", (nb, m, k, n, ksplit), err, flush=True)
+    assert err < 2e-4
+    C2 = mfma.gemm_nt(As, Bs, out=C.clone(), beta=1, ksplit=ksplit)
+    assert float((C2 - 2 * ref).abs().max()) / k ** 0.5 < 4e-4
+
+
+@pytest.mark.gpu
 def test_vvvv_term_symmetric_half_matches_full(gpu):
     from ccsd_gpu_amd.precision import static_operands, vvvv_term
     gen = torch.Generator(device="cuda").manual_seed(3)

@@ -67,3 +67,26 @@ def test_static_cache_reuses_operands_in_several_roles():
             got = cc.update_amps(t1, t2, e)
             assert torch.allclose(got[0], ref[0], rtol=1e-12, atol=1e-14)
             assert torch.allclose(got[1], ref[1], rtol=1e-12, atol=1e-14)
+
+
+@pytest.mark.parametrize("k", [8192, 10001])
+def test_split_k_dgemm_matches_mm(k):
+    """Small output, long contracted dimension: the K-split batched product (with a ragged
+    remainder and for a transposed left operand) equals the plain product."""
+    from ccsd_gpu_amd.precision import _dgemm
+    g = torch.Generator().manual_seed(k)
+    A = torch.randn(2, 3, k, dtype=torch.float64, generator=g)
+    At = torch.randn(2, k, 3, dtype=torch.float64, generator=g).transpose(1, 2)
+    B = torch.randn(2, k, 5, dtype=torch.float64, generator=g)
+    for a in (A, At, A[:1], At[:1]):
+        ref = torch.bmm(a, B[:a.shape[0]])
+        got = _dgemm(a, B[:a.shape[0]])
+        assert torch.allclose(got, ref, rtol=1e-12, atol=1e-11)
+
+
+def test_k_splits_policy():
+    from ccsd_gpu_amd.mfma import k_splits
+    assert k_splits(1, 1000) == 1                  # short K: no split
+    assert k_splits(4096, 10 ** 6) == 1            # enough tiles already
+    assert k_splits(1, 10 ** 6) == 976             # 1024 chunks wanted, limited by chunk >= 1024
+    assert k_splits(4, 10 ** 7) == 256
