@@ -16,6 +16,11 @@ reported in config, with the max |difference| of the two gradients.
 --config eri: the fused int2e_ip1 x G backward pass of the ERI autograd
 function on (H2O)4 (one step = one pass), value = ERI' quartets/s.
 
+--config batch: 64 seeded C/N/O/H molecules (3-12 heavy atoms, cc-pVDZ) in one
+plan; one step = one Schwarz-screened (1e-12) fused int2e_ip1 x Gamma pass over
+the whole batch with a separable density Gamma(D) per molecule; value = surviving
+quartets/s.
+
 Every line carries `roofline` (the dominant kernel: FP64 vector for the ERI
 kernels, time from HIP events on the launch stream) and `cpu_baseline`.
 """
@@ -118,6 +123,92 @@ def bench_eri(args):
         "roofline": roof, "cpu_baseline": cpu}
 
 
+def _sym_density(n, rng):
+    a = rng.standard_normal((n, n))
+    return (a + a.T) / np.sqrt(n)
+
+
+def timed_batch_pass(b, dms, reps):
+    """HIP-event time (ms) of one batched fused pass on torch's current stream, and the result."""
+    import torch
+    stream = torch.cuda.current_stream()
+    ev = []
+    de = None
+    for _ in range(reps):
+        a = torch.cuda.Event(enable_timing=True)
+        e = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        de = b.grad2e_separable(dms)
+        e.record(stream)
+        ev.append((a, e))
+    torch.cuda.synchronize()
+    return float(np.mean([a.elapsed_time(e) for a, e in ev])), de
+
+
+def bench_batch(args):
+    import torch
+    from ccsd_gpu_amd import flopcount, systems
+    from ccsd_gpu_amd.batch import MoleculeBatch
+    nmol = args.molecules
+    mols = systems.cnoh_batch(nmol, seed=0)
+    rng = np.random.default_rng(0)
+    dms = [_sym_density(m.nao_nr(), rng) for m in mols]
+    b = MoleculeBatch(mols)
+    log(f"{nmol} C/N/O/H molecules, nao {min(m.nao_nr() for m in mols)}-{max(m.nao_nr() for m in mols)}, "
+        f"{sum(m.natm for m in mols)} atoms, {int(b.mol_nbas.sum())} shells")
+    b.screen(args.thr)
+    t0 = time.perf_counter()
+    for _ in range(args.warmup):
+        b.grad2e_separable(dms)       # first call: Schwarz factors, work lists, per-class kernel choice
+    torch.cuda.synchronize()
+    log(f"warmup (incl. screening set-up and autotuning) {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        de = b.grad2e_separable(dms)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    de_s = torch.cat(de).clone()
+    surv = b.last_quartets()
+    kern_ms, _ = timed_batch_pass(b, dms, 2)
+    q = b.schwarz().cpu().numpy()
+    flops = 0.0
+    s0 = 0
+    for m in mols:
+        f, _ = flopcount.eri_work_screened(m, 1, q[s0:s0 + m.nbas, s0:s0 + m.nbas], args.thr)
+        flops += f
+        s0 += m.nbas
+    achieved = flops / (kern_ms * 1e-3) / 1e12
+    roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
+    # the same pass unscreened, for the error and gain of screening
+    b.screen(0.0)
+    b.grad2e_separable(dms)
+    full_ms, de_f = timed_batch_pass(b, dms, 1)
+    err = float((de_s - torch.cat(de_f)).abs().max())
+    net = max(float(d.sum(0).abs().max()) for d in de_f)
+    refq = 0
+    for m in mols:
+        ndc = int(m._bas[:, 3].sum())
+        refq += ndc * ndc * ndc * (ndc + 1) // 2
+    cpu = None
+    if not args.no_cpu:
+        rate, dt, nth = cpu_eri_rate(mols[-1])
+        cpu = {"value": surv / (flops / rate), "unit": "ERI' quartets/s", "cores": nth, "kind": "port",
+               "sample": f"FP64 C oracle int2e_ip1 over i-shells [0,3) of one molecule of the batch ({dt:.1f} s, "
+                         f"{rate / 1e9:.1f} GFLOP/s), scaled to the screened batch by the flop model"}
+    return {
+        "metric": METRIC, "value": surv / (ms * 1e-3), "unit": "ERI' quartets/s (Schwarz survivors)", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batched fused int2e_ip1 x Gamma(D) over {nmol} seeded C/N/O/H molecules "
+                               "(3-12 heavy atoms, cc-pVDZ), one launch per angular-momentum class for the batch",
+                   "schwarz_threshold": args.thr, "quartets_evaluated": surv,
+                   "quartets_unscreened": b.quartets_unscreened(), "quartets_reference_equivalent": refq,
+                   "unscreened_pass_ms": full_ms, "screened_pass_ms": kern_ms,
+                   "max_abs_diff_screened_vs_unscreened": err, "max_net_force_unscreened": net},
+        "roofline": roof, "cpu_baseline": cpu}
+
+
 def bench_grad(args):
     import torch
     from ccsd_gpu_amd import pipeline, systems
@@ -198,7 +289,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad", choices=["grad", "eri"])
+    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch"])
+    ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
+    ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--waters", type=int, default=0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -211,7 +304,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     _lib.load()
-    line = bench_grad(args) if args.config == "grad" else bench_eri(args)
+    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

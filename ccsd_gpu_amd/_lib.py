@@ -34,6 +34,12 @@ _SIGNATURES = {
                                     ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                     ctypes.c_longlong, ctypes.c_longlong, c_int, c_int, c_void_p]),
     "ccsd_rys_roots": (c_int, [c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "ccsd_plan_create_batch": (c_int, [ctypes.POINTER(c_void_p), c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                       c_void_p, c_int]),
+    "ccsd_plan_screen": (c_int, [c_void_p, c_double, c_void_p]),
+    "ccsd_schwarz": (c_int, [c_void_p, c_void_p, c_void_p]),
+    "ccsd_plan_quartets": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
+    "ccsd_grad_ip1_separable": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
 }
 EXPORTED = tuple(_SIGNATURES)
 

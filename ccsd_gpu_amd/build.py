@@ -2,17 +2,28 @@
 
     python -m ccsd_gpu_amd.build
 
-compiles ccsd_gpu_amd/csrc/*.hip with hipcc into ccsd_gpu_amd/libccsd_gpu_amd.so.
+compiles ccsd_gpu_amd/csrc/*.hip with hipcc, one object per translation unit
+in parallel (the register-resident kernels once per bra angular-momentum pair),
+into ccsd_gpu_amd/_build/ and links ccsd_gpu_amd/libccsd_gpu_amd.so.
 """
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libccsd_gpu_amd.so")
-SOURCES = ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip", "capi.hip"]
+# (source, object name, extra defines)
+UNITS = [(s, s.replace(".hip", ""), []) for s in
+         ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip",
+          "capi.hip"]]
+UNITS += [("fused_ip1_reg_tu.hip", f"fused_ip1_reg_l{i}{j}", [f"-DCCSD_REG_LI={i}", f"-DCCSD_REG_LJ={j}"])
+          for i in range(3) for j in range(3)]
 ARCH = os.environ.get("CCSD_GPU_ARCH", "gfx950")
+FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
 def hipcc():
@@ -22,16 +33,27 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=True):
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
-           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-           "-o", OUT + ".tmp"] + [os.path.join(CSRC, s) for s in SOURCES]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed")
+        raise RuntimeError("hipcc failed: " + " ".join(cmd))
+
+
+def build(verbose=True, jobs=None):
+    os.makedirs(OBJ, exist_ok=True)
+    cc = hipcc()
+    objs, cmds = [], []
+    for src, name, defs in UNITS:
+        obj = os.path.join(OBJ, name + ".o")
+        objs.append(obj)
+        cmds.append([cc] + FLAGS + defs + ["-c", os.path.join(CSRC, src), "-o", obj])
+    jobs = jobs or min(16, os.cpu_count() or 4)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", OUT + ".tmp"] + objs, verbose)
     os.replace(OUT + ".tmp", OUT)
     return OUT
 

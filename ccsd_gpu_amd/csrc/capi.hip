@@ -9,6 +9,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "eri_common.h"
@@ -21,8 +22,8 @@ hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const 
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
 size_t ccsd_c2s_term_size();
 hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
-bool ccsd_fused_reg_supported(int li, int lj, int lk, int ll, int max_nckl);
-hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, hipStream_t stream);
+int ccsd_fused_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_nckl);
+hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int waves_per_block, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
                                       long long sC, long long ldc, int batch, int beta, hipStream_t stream);
@@ -61,11 +62,38 @@ struct C2sTermHost {
     double c[6];
 };
 
+// Shell pairs of one angular-momentum class, grouped into segments: a segment
+// is a run of pairs of one molecule with the same primitive x contraction
+// product (lanes of a wave then loop alike), ordered by decreasing Schwarz
+// factor when screening is on, so that the survivors for any bra pair are a
+// prefix of each segment.
 struct PairClass {
     int l1, l2;
     std::vector<int2> pairs;
-    int max_np, max_nc;
+    std::vector<int> seg;          // segment s = pairs[seg[s], seg[s + 1])
+    std::vector<int> seg_mol;
+    int max_np = 0, max_nc = 0;
     int2* d_pairs = nullptr;
+};
+
+// Device work list of one (bra class, ket class) launch.
+struct ChunkList {
+    int4* d = nullptr;
+    int n = 0;
+    long long quartets = 0;
+    std::vector<long long> cumq;   // quartets in the first c chunks (autotuner samples)
+};
+
+// Kernel choice of one class in the fused gradient.
+struct Choice {
+    int kind = -1;   // 0 generic LDS kernel, 1 register kernel
+    int wpb = 4;     // waves per block (register kernel)
+    int chunk = 64;  // ket pairs per chunk
+};
+
+struct BraSet {
+    std::vector<PairClass> cls;
+    std::map<std::tuple<int, int, int, int>, ChunkList> chunks;  // (bc, kc, chunk, screened)
 };
 
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
@@ -85,46 +113,81 @@ struct ccsd_plan {
     double* d_env = nullptr;
     double* d_coords = nullptr;
     double* d_charges = nullptr;
-    std::vector<PairClass> ket;                      // ksh >= lsh
-    std::map<std::pair<int, int>, std::vector<PairClass>> bra_cache;
+    // molecules (one unless created by ccsd_plan_create_batch)
+    int nmol = 1;
+    std::vector<int> mol_shell0;                     // nmol + 1
+    std::vector<int> mol_ao0, mol_nao;               // Cartesian
+    int* d_mol_ao0 = nullptr;
+    int* d_mol_nao = nullptr;
+    long long* d_mol_dm_off = nullptr;
+    std::vector<PairClass> ket;                      // ksh >= lsh, same molecule
+    std::map<std::pair<int, int>, BraSet> bra_cache;
+    // fused-gradient kernel choice per (ish0, ish1, bra class, ket class); class
+    // numbering does not depend on the geometry or the screening threshold
+    std::map<std::tuple<int, int, int, int>, Choice> choice;
     bool has_c2s = false;
     bool fused_reg = true;                           // register kernels for the fused gradient
+    bool autotune = true;                            // time the kernel choices of each class once
+    // Schwarz screening of the fused gradient
+    double thr = 0.0;
+    bool q_valid = false;
+    std::vector<double> q;                           // nbas x nbas, symmetric
+    long long last_quartets = 0;
+    double* d_de_scratch = nullptr;                  // autotuner's throw-away gradient
     std::vector<C2sTermHost> c2s;                    // per spherical AO
     void* d_c2s = nullptr;
 };
 
 namespace {
 
-std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>& pairs)
+int prim_group(const ccsd_plan* p, const int2& pr)
+{
+    const ShellInfo &a = p->shells[pr.x], &b = p->shells[pr.y];
+    return a.nprim * b.nprim * a.nctr * b.nctr;
+}
+
+double pair_q(const ccsd_plan* p, const int2& pr) { return p->q[(size_t)pr.x * p->nbas + pr.y]; }
+
+// Classes by (l1, l2, several contraction columns or not); inside a class:
+// molecule, then primitive group (descending), then Schwarz factor (descending)
+// when `by_q`.
+std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>& pairs, bool by_q)
 {
     std::vector<PairClass> cls;
-    std::map<std::pair<int, int>, int> where;
+    std::map<std::tuple<int, int, int>, int> where;
     for (const int2& pr : pairs) {
         const ShellInfo &a = p->shells[pr.x], &b = p->shells[pr.y];
-        auto key = std::make_pair(a.l, b.l);
+        auto key = std::make_tuple(a.l, b.l, a.nctr * b.nctr > 1 ? 1 : 0);
         auto it = where.find(key);
         if (it == where.end()) {
-            where[key] = (int)cls.size();
+            it = where.emplace(key, (int)cls.size()).first;
             PairClass c;
             c.l1 = a.l;
             c.l2 = b.l;
-            c.max_np = 0;
-            c.max_nc = 0;
             cls.push_back(c);
-            it = where.find(key);
         }
         PairClass& c = cls[it->second];
         c.pairs.push_back(pr);
         c.max_np = std::max(c.max_np, a.nprim * b.nprim);
         c.max_nc = std::max(c.max_nc, a.nctr * b.nctr);
     }
-    // similar primitive counts next to each other (threads of a wave loop alike)
-    for (PairClass& c : cls)
-        std::stable_sort(c.pairs.begin(), c.pairs.end(), [p](const int2& x, const int2& y) {
-            const ShellInfo &a0 = p->shells[x.x], &a1 = p->shells[x.y];
-            const ShellInfo &b0 = p->shells[y.x], &b1 = p->shells[y.y];
-            return a0.nprim * a1.nprim * a0.nctr * a1.nctr > b0.nprim * b1.nprim * b0.nctr * b1.nctr;
+    for (PairClass& c : cls) {
+        std::stable_sort(c.pairs.begin(), c.pairs.end(), [p, by_q](const int2& x, const int2& y) {
+            const int mx = p->shells[x.x].mol, my = p->shells[y.x].mol;
+            if (mx != my) return mx < my;
+            const int gx = prim_group(p, x), gy = prim_group(p, y);
+            if (gx != gy) return gx > gy;
+            return by_q && pair_q(p, x) > pair_q(p, y);
         });
+        for (size_t t = 0; t < c.pairs.size(); t++) {
+            if (t == 0 || p->shells[c.pairs[t].x].mol != p->shells[c.pairs[t - 1].x].mol ||
+                prim_group(p, c.pairs[t]) != prim_group(p, c.pairs[t - 1])) {
+                c.seg.push_back((int)t);
+                c.seg_mol.push_back(p->shells[c.pairs[t].x].mol);
+            }
+        }
+        c.seg.push_back((int)c.pairs.size());
+    }
     return cls;
 }
 
@@ -144,18 +207,111 @@ void free_classes(std::vector<PairClass>& cls)
     cls.clear();
 }
 
-int get_bra(ccsd_plan* p, int ish0, int ish1, std::vector<PairClass>** out)
+void free_braset(BraSet& b)
+{
+    free_classes(b.cls);
+    for (auto& kv : b.chunks)
+        if (kv.second.d) (void)hipFree(kv.second.d);
+    b.chunks.clear();
+}
+
+void clear_bra_cache(ccsd_plan* p)
+{
+    for (auto& kv : p->bra_cache) free_braset(kv.second);
+    p->bra_cache.clear();
+}
+
+int build_ket(ccsd_plan* p)
+{
+    free_classes(p->ket);
+    std::vector<int2> kpairs;
+    for (int m = 0; m < p->nmol; m++)
+        for (int k = p->mol_shell0[m]; k < p->mol_shell0[m + 1]; k++)
+            for (int l = p->mol_shell0[m]; l <= k; l++) kpairs.push_back(make_int2(k, l));
+    p->ket = make_classes(p, kpairs, p->thr > 0.0 && p->q_valid);
+    return upload_classes(p->ket);
+}
+
+int get_bra(ccsd_plan* p, int ish0, int ish1, BraSet** out)
 {
     auto key = std::make_pair(ish0, ish1);
     auto it = p->bra_cache.find(key);
     if (it == p->bra_cache.end()) {
         std::vector<int2> pairs;
-        for (int i = ish0; i < ish1; i++)
-            for (int j = 0; j < p->nbas; j++) pairs.push_back(make_int2(i, j));
-        std::vector<PairClass> cls = make_classes(p, pairs);
-        int rc = upload_classes(cls);
-        if (rc) return rc;
-        it = p->bra_cache.emplace(key, std::move(cls)).first;
+        for (int i = ish0; i < ish1; i++) {
+            const int m = p->shells[i].mol;
+            for (int j = p->mol_shell0[m]; j < p->mol_shell0[m + 1]; j++) pairs.push_back(make_int2(i, j));
+        }
+        BraSet b;
+        b.cls = make_classes(p, pairs, false);
+        int rc = upload_classes(b.cls);
+        if (rc) {
+            free_braset(b);
+            return rc;
+        }
+        it = p->bra_cache.emplace(key, std::move(b)).first;
+    }
+    *out = &it->second;
+    return CCSD_OK;
+}
+
+// Work chunks of (bra class bc) x (ket class kc): per bra pair, the ket pairs of
+// its molecule, per segment the Schwarz survivors (a prefix), cut into runs of at
+// most `chunk`; most expensive chunks first so that the tail of the launch is short.
+int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screened, ChunkList** out)
+{
+    auto key = std::make_tuple(bc, kc, chunk, screened ? 1 : 0);
+    auto it = bs.chunks.find(key);
+    if (it == bs.chunks.end()) {
+        const PairClass& B = bs.cls[bc];
+        const PairClass& K = p->ket[kc];
+        std::vector<std::pair<long long, int4>> work;
+        long long quartets = 0;
+        for (int bq = 0; bq < (int)B.pairs.size(); bq++) {
+            const int2 bp = B.pairs[bq];
+            const int m = p->shells[bp.x].mol;
+            const long long cb = prim_group(p, bp);
+            const double qb = screened ? pair_q(p, bp) : 0.0;
+            for (size_t s = 0; s + 1 < K.seg.size(); s++) {
+                if (K.seg_mol[s] != m) continue;
+                const int k0 = K.seg[s];
+                int k1 = K.seg[s + 1];
+                if (screened) {
+                    // survivors: qb * q_kl >= thr, q_kl descending inside the segment
+                    int lo = k0, hi = k1;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) / 2;
+                        if (qb * pair_q(p, K.pairs[mid]) >= p->thr) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    k1 = lo;
+                }
+                const long long ck = prim_group(p, K.pairs[k0]);
+                for (int k = k0; k < k1; k += chunk) {
+                    const int ke = std::min(k1, k + chunk);
+                    work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
+                    quartets += ke - k;
+                }
+            }
+        }
+        if (work.size() <= 4000000)
+            std::stable_sort(work.begin(), work.end(),
+                             [](const std::pair<long long, int4>& x, const std::pair<long long, int4>& y) {
+                                 return x.first > y.first;
+                             });
+        std::vector<int4> h(work.size());
+        for (size_t t = 0; t < work.size(); t++) h[t] = work[t].second;
+        ChunkList cl;
+        cl.n = (int)h.size();
+        cl.quartets = quartets;
+        cl.cumq.resize(h.size() + 1);
+        cl.cumq[0] = 0;
+        for (size_t t = 0; t < h.size(); t++) cl.cumq[t + 1] = cl.cumq[t] + (h[t].z - h[t].y);
+        if (cl.n) {
+            HIP_TRY(hipMalloc(&cl.d, sizeof(int4) * h.size()));
+            HIP_TRY(hipMemcpy(cl.d, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+        }
+        it = bs.chunks.emplace(key, cl).first;
     }
     *out = &it->second;
     return CCSD_OK;
@@ -197,48 +353,237 @@ void tune_class(EriClassArgs& a)
     a.units_pass = std::max(1, std::min(nunits, red_budget / (a.groups * ncomp)));
 }
 
-int run_classes(ccsd_plan* p, std::vector<PairClass>& bra, int deriv, double* out, int i_base,
-                long long comp_stride, const double* gamma, double* de, double scale, hipStream_t stream)
+EriClassArgs class_args(const ccsd_plan* p, const PairClass& bc, const PairClass& kc, int deriv)
 {
-    const long long npair = (long long)p->nao_cart * (p->nao_cart + 1) / 2;
-    for (PairClass& bc : bra) {
-        for (PairClass& kc : p->ket) {
-            EriClassArgs a;
-            memset(&a, 0, sizeof(a));
-            a.shells = p->d_shells;
-            a.env = p->d_env;
-            a.coords = p->d_coords;
-            a.bra = bc.d_pairs;
-            a.ket = kc.d_pairs;
-            a.nbra = (int)bc.pairs.size();
-            a.nket = (int)kc.pairs.size();
-            a.li = bc.l1; a.lj = bc.l2; a.lk = kc.l1; a.ll = kc.l2;
-            a.deriv = deriv;
-            a.nroots = (a.li + a.lj + a.lk + a.ll + deriv) / 2 + 1;
-            a.max_nij = bc.max_np;
-            a.max_nkl = kc.max_np;
-            a.max_ncij = bc.max_nc;
-            a.max_nckl = kc.max_nc;
-            tune_class(a);
+    EriClassArgs a;
+    memset(&a, 0, sizeof(a));
+    a.shells = p->d_shells;
+    a.env = p->d_env;
+    a.coords = p->d_coords;
+    a.bra = bc.d_pairs;
+    a.ket = kc.d_pairs;
+    a.nbra = (int)bc.pairs.size();
+    a.nket = (int)kc.pairs.size();
+    a.li = bc.l1; a.lj = bc.l2; a.lk = kc.l1; a.ll = kc.l2;
+    a.deriv = deriv;
+    a.nroots = (a.li + a.lj + a.lk + a.ll + deriv) / 2 + 1;
+    a.max_nij = bc.max_np;
+    a.max_nkl = kc.max_np;
+    a.max_ncij = bc.max_nc;
+    a.max_nckl = kc.max_nc;
+    tune_class(a);
+    a.nao = p->nao_cart;
+    a.npair = (long long)p->nao_cart * (p->nao_cart + 1) / 2;
+    a.nbas = p->nbas;
+    a.mol_ao0 = p->d_mol_ao0;
+    a.mol_nao = p->d_mol_nao;
+    a.mol_dm_off = p->d_mol_dm_off;
+    return a;
+}
+
+int launch_choice(const EriClassArgs& a, const Choice& c, hipStream_t stream)
+{
+    if (c.kind == 1) {
+        hipError_t e = ccsd_launch_fused_reg(a, c.wpb, stream);
+        if (e != hipSuccess) return hip_fail(e, "fused_ip1_reg_kernel launch");
+        return CCSD_OK;
+    }
+    if (ccsd_eri_lds_bytes(a) > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
+    hipError_t e = ccsd_launch_eri_class(a, stream);
+    if (e != hipSuccess) return hip_fail(e, "eri_quartet_kernel launch");
+    return CCSD_OK;
+}
+
+// Kernel choice of one fused-gradient class: every candidate is timed once on a
+// sample of the class's real work (the first chunks of its list, about 2^18
+// quartets, into a throw-away gradient) and the fastest per quartet is kept.
+int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClassArgs a, bool screened,
+           hipStream_t stream, Choice* out)
+{
+    auto key = std::make_tuple(ish0, ish1, bc, kc);
+    auto it = p->choice.find(key);
+    if (it != p->choice.end()) {
+        *out = it->second;
+        return CCSD_OK;
+    }
+    std::vector<Choice> cand;
+    const bool reg = p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
+    if (reg) {
+        for (int w : {4, 2, 1}) {
+            Choice c;
+            c.kind = 1; c.wpb = w; c.chunk = 64;
+            cand.push_back(c);
+        }
+    }
+    // generic kernel: runs of 1, 4 or 16 quartets per wave; work lists kept below
+    // 4M entries (the bound counts every same-molecule quartet, screened or not)
+    const long long upper = (long long)a.nbra * a.nket;
+    for (int ch : {1, 4, 16}) {
+        if (upper / ch > 4000000 && ch != 16) continue;
+        Choice c;
+        c.kind = 0; c.wpb = 1; c.chunk = ch;
+        cand.push_back(c);
+    }
+    Choice best = cand[0];
+    if (p->autotune && cand.size() > 1) {
+        if (!p->d_de_scratch) HIP_TRY(hipMalloc(&p->d_de_scratch, sizeof(double) * 3 * p->natm));
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        double best_t = 1e300;
+        for (const Choice& c : cand) {
+            ChunkList* cl = nullptr;
+            int rc = get_chunks(p, bs, bc, kc, c.chunk, screened, &cl);
+            if (rc) return rc;
+            if (cl->n == 0) break;   // nothing to run in this class
+            const int ns = (int)(std::lower_bound(cl->cumq.begin() + 1, cl->cumq.end(), 262144LL) - cl->cumq.begin());
+            const int nsample = std::min(cl->n, std::max(1, ns));
+            EriClassArgs t = a;
+            t.chunks = cl->d;
+            t.nchunks = nsample;
+            t.de = p->d_de_scratch;
+            HIP_TRY(hipEventRecord(e0, stream));
+            rc = launch_choice(t, c, stream);
+            if (rc) return rc;
+            HIP_TRY(hipEventRecord(e1, stream));
+            HIP_TRY(hipEventSynchronize(e1));
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+            const double per_q = ms / (double)cl->cumq[nsample];
+            if (per_q < best_t) {
+                best_t = per_q;
+                best = c;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        // drop the work lists of the candidates not taken
+        for (const Choice& c : cand) {
+            if (c.chunk == best.chunk) continue;
+            auto ck = bs.chunks.find(std::make_tuple(bc, kc, c.chunk, screened ? 1 : 0));
+            if (ck != bs.chunks.end()) {
+                if (ck->second.d) (void)hipFree(ck->second.d);
+                bs.chunks.erase(ck);
+            }
+        }
+    }
+    p->choice[key] = best;
+    *out = best;
+    return CCSD_OK;
+}
+
+// Schwarz factors q[i][j] = max over functions a in i, b in j of sqrt((ab|ab)),
+// from the diagonal quartets of every same-molecule pair, on the device.
+int compute_q(ccsd_plan* p, hipStream_t stream)
+{
+    std::vector<int2> pairs;
+    for (int m = 0; m < p->nmol; m++)
+        for (int i = p->mol_shell0[m]; i < p->mol_shell0[m + 1]; i++)
+            for (int j = p->mol_shell0[m]; j <= i; j++) pairs.push_back(make_int2(i, j));
+    std::vector<PairClass> cls = make_classes(p, pairs, false);
+    int rc = upload_classes(cls);
+    unsigned long long* d_q = nullptr;
+    int4* d_ch = nullptr;
+    const size_t nq = (size_t)p->nbas * p->nbas;
+    hipError_t e = hipSuccess;
+    if (!rc) e = hipMalloc(&d_q, sizeof(unsigned long long) * nq);
+    if (!rc && e == hipSuccess) e = ccsd_launch_fill((double*)d_q, (long long)nq, 0.0, stream);
+    if (!rc && e == hipSuccess) e = hipMalloc(&d_ch, sizeof(int4) * std::max<size_t>(1, pairs.size()));
+    for (size_t c = 0; !rc && e == hipSuccess && c < cls.size(); c++) {
+        const PairClass& pc = cls[c];
+        std::vector<int4> h(pc.pairs.size());
+        for (size_t t = 0; t < h.size(); t++) h[t] = make_int4((int)t, (int)t, (int)t + 1, 0);
+        e = hipMemcpyAsync(d_ch, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) break;
+        EriClassArgs a = class_args(p, pc, pc, 0);
+        a.chunks = d_ch;
+        a.nchunks = (int)h.size();
+        a.qmax = d_q;
+        if (ccsd_eri_lds_bytes(a) > 65536) {
+            rc = fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
+            break;
+        }
+        e = ccsd_launch_eri_class(a, stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(stream);   // h is reused by the next class
+    }
+    std::vector<double> hq(nq);
+    if (!rc && e == hipSuccess)
+        e = hipMemcpyAsync(hq.data(), d_q, sizeof(double) * nq, hipMemcpyDeviceToHost, stream);
+    if (!rc && e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (!rc && e != hipSuccess) rc = hip_fail(e, "Schwarz factors");
+    if (d_q) (void)hipFree(d_q);
+    if (d_ch) (void)hipFree(d_ch);
+    free_classes(cls);
+    if (rc) return rc;
+    p->q.assign(nq, 0.0);
+    for (int i = 0; i < p->nbas; i++)
+        for (int j = 0; j <= i; j++) p->q[(size_t)i * p->nbas + j] = p->q[(size_t)j * p->nbas + i] = hq[(size_t)i * p->nbas + j];
+    p->q_valid = true;
+    return CCSD_OK;
+}
+
+// Screening state current for the geometry: factors computed and the ket classes
+// ordered by them (work lists and kernel choices rebuilt).
+int ensure_screening(ccsd_plan* p, hipStream_t stream)
+{
+    if (p->thr <= 0.0 || p->q_valid) return CCSD_OK;
+    int rc = compute_q(p, stream);
+    if (rc) return rc;
+    clear_bra_cache(p);
+    return build_ket(p);
+}
+
+int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long long comp_stride, hipStream_t stream)
+{
+    p->last_quartets = 0;
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], deriv);
+            ChunkList* cl = nullptr;
+            int rc = get_chunks(p, bs, bc, kc, 1, false, &cl);
+            if (rc) return rc;
+            a.chunks = cl->d;
+            a.nchunks = cl->n;
             a.out = out;
             a.i_base = i_base;
-            a.nao = p->nao_cart;
-            a.npair = npair;
             a.comp_stride = comp_stride;
+            Choice c;
+            c.kind = 0;
+            rc = launch_choice(a, c, stream);
+            if (rc) return rc;
+            p->last_quartets += cl->quartets;
+        }
+    }
+    return CCSD_OK;
+}
+
+// Fused gradient over the bra pairs of `bs`: explicit packed gamma (mode 0) or the
+// separable form from dm / pm (mode 1).
+int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
+              const double* gamma, const double* dm, const double* pm, hipStream_t stream)
+{
+    const bool screened = p->thr > 0.0;
+    p->last_quartets = 0;
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
+            a.gamma_mode = gamma_mode;
             a.gamma = gamma;
-            a.de = de;
+            a.dm = dm;
+            a.pm = pm;
             a.gamma_scale = scale;
-            const long long nblk = (long long)a.nbra * a.nket;
-            if (nblk > 2147483647LL) return fail(CCSD_ERR_SHAPE, "too many quartets in one class launch");
-            if (!out && p->fused_reg && a.nbra <= 65535 &&
-                ccsd_fused_reg_supported(a.li, a.lj, a.lk, a.ll, a.max_nckl)) {
-                hipError_t e = ccsd_launch_fused_reg(a, stream);
-                if (e != hipSuccess) return hip_fail(e, "fused_ip1_reg_kernel launch");
-                continue;
-            }
-            if (ccsd_eri_lds_bytes(a) > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
-            hipError_t e = ccsd_launch_eri_class(a, stream);
-            if (e != hipSuccess) return hip_fail(e, "eri_quartet_kernel launch");
+            Choice c;
+            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c);
+            if (rc) return rc;
+            ChunkList* cl = nullptr;
+            rc = get_chunks(p, bs, bc, kc, c.chunk, screened, &cl);
+            if (rc) return rc;
+            a.chunks = cl->d;
+            a.nchunks = cl->n;
+            a.de = de;
+            rc = launch_choice(a, c, stream);
+            if (rc) return rc;
+            p->last_quartets += cl->quartets;
         }
     }
     return CCSD_OK;
@@ -258,9 +603,10 @@ int eri_s2kl(ccsd_plan* p, double* out, int ish0, int ish1, int cart, int deriv,
     int rc = check_plan(p);
     if (rc) return rc;
     if (!out) return fail(CCSD_ERR_NULL, "null output pointer");
+    if (p->nmol != 1) return fail(CCSD_ERR_UNSUPPORTED, "integral tensors need a single-molecule plan");
     if (ish0 < 0 || ish1 > p->nbas || ish0 >= ish1) return fail(CCSD_ERR_SHAPE, "bad shell range");
     if (!cart && !p->has_c2s) return fail(CCSD_ERR_UNSUPPORTED, "plan has no cart2sph coefficients");
-    std::vector<PairClass>* bra = nullptr;
+    BraSet* bra = nullptr;
     rc = get_bra(p, ish0, ish1, &bra);
     if (rc) return rc;
     const int ncomp = deriv ? 3 : 1;
@@ -273,7 +619,7 @@ int eri_s2kl(ccsd_plan* p, double* out, int ish0, int ish1, int cart, int deriv,
     if (!cart) {
         HIP_TRY(hipMallocAsync((void**)&cart_buf, sizeof(double) * ncomp * comp_stride_c, stream));
     }
-    rc = run_classes(p, *bra, deriv, cart_buf, i0c, comp_stride_c, nullptr, nullptr, 0.0, stream);
+    rc = run_eri(p, *bra, deriv, cart_buf, i0c, comp_stride_c, stream);
     if (!rc && !cart) {
         const int i0s = p->shells[ish0].ao_sph;
         const int i1s = ish1 < p->nbas ? p->shells[ish1].ao_sph : p->nao_sph;
@@ -297,19 +643,25 @@ int eri_s2kl(ccsd_plan* p, double* out, int ish0, int ish1, int cart, int deriv,
     return rc;
 }
 
-}  // namespace
+int ccsd_plan_destroy_impl(ccsd_plan* p);
 
-extern "C" {
-
-const char* ccsd_last_error(void) { return g_last_error.c_str(); }
-
-int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, int nbas, const double* env, int nenv,
-                     const double* c2s)
+int create_plan(ccsd_plan** out, int nmol, const int* mol_nbas, const int* atm, int natm, const int* bas, int nbas,
+                const double* env, int nenv, const double* c2s)
 {
     if (!out || !atm || !bas || !env) return fail(CCSD_ERR_NULL, "null argument to ccsd_plan_create");
     if (natm <= 0 || nbas <= 0 || nenv <= 0) return fail(CCSD_ERR_SHAPE, "empty molecule");
     *out = nullptr;
+    std::vector<int> shell0(nmol + 1, 0);
+    for (int m = 0; m < nmol; m++) {
+        if (mol_nbas[m] <= 0) return fail(CCSD_ERR_SHAPE, "molecule without shells");
+        shell0[m + 1] = shell0[m] + mol_nbas[m];
+    }
+    if (shell0[nmol] != nbas) return fail(CCSD_ERR_SHAPE, "molecule shell counts do not add up to nbas");
     ccsd_plan* p = new ccsd_plan();
+    p->nmol = nmol;
+    p->mol_shell0 = shell0;
+    p->mol_ao0.assign(nmol, 0);
+    p->mol_nao.assign(nmol, 0);
     p->natm = natm;
     p->nbas = nbas;
     p->nenv = nenv;
@@ -328,6 +680,9 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
         s.pcoef = b[CCSD_PTR_COEFF];
         s.ao_cart = aoc;
         s.ao_sph = aos;
+        s.mol = (int)(std::upper_bound(shell0.begin(), shell0.end(), i) - shell0.begin()) - 1;
+        s.pad_ = 0;
+        if (i == shell0[s.mol]) p->mol_ao0[s.mol] = aoc;
         if (s.l < 0 || s.l > CCSD_LMAX) {
             delete p;
             return fail(CCSD_ERR_UNSUPPORTED, "angular momentum above 4");
@@ -348,6 +703,8 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
     }
     p->nao_cart = aoc;
     p->nao_sph = aos;
+    for (int m = 0; m < nmol; m++)
+        p->mol_nao[m] = (m + 1 < nmol ? p->mol_ao0[m + 1] : aoc) - p->mol_ao0[m];
     if (hipGetDevice(&p->device) != hipSuccess) {
         delete p;
         return fail(CCSD_ERR_HIP, "hipGetDevice failed");
@@ -356,28 +713,32 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
     if (hipMalloc(&p->d_shells, sizeof(ShellInfo) * nbas) != hipSuccess ||
         hipMalloc(&p->d_env, sizeof(double) * nenv) != hipSuccess ||
         hipMalloc(&p->d_coords, sizeof(double) * 3 * natm) != hipSuccess ||
-        hipMalloc(&p->d_charges, sizeof(double) * natm) != hipSuccess) {
-        ccsd_plan_destroy(p);
+        hipMalloc(&p->d_charges, sizeof(double) * natm) != hipSuccess ||
+        hipMalloc(&p->d_mol_ao0, sizeof(int) * nmol) != hipSuccess ||
+        hipMalloc(&p->d_mol_nao, sizeof(int) * nmol) != hipSuccess ||
+        hipMalloc(&p->d_mol_dm_off, sizeof(long long) * nmol) != hipSuccess) {
+        ccsd_plan_destroy_impl(p);
         return fail(CCSD_ERR_ALLOC, "device allocation for the basis plan failed");
     }
-    if (hipMemcpy(p->d_shells, p->shells.data(), sizeof(ShellInfo) * nbas, hipMemcpyHostToDevice) != hipSuccess) {
-        ccsd_plan_destroy(p);
+    std::vector<long long> dm_off(nmol, 0);
+    for (int m = 1; m < nmol; m++) dm_off[m] = dm_off[m - 1] + (long long)p->mol_nao[m - 1] * p->mol_nao[m - 1];
+    if (hipMemcpy(p->d_shells, p->shells.data(), sizeof(ShellInfo) * nbas, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_mol_ao0, p->mol_ao0.data(), sizeof(int) * nmol, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_mol_nao, p->mol_nao.data(), sizeof(int) * nmol, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(p->d_mol_dm_off, dm_off.data(), sizeof(long long) * nmol, hipMemcpyHostToDevice) != hipSuccess) {
+        ccsd_plan_destroy_impl(p);
         return fail(CCSD_ERR_HIP, "upload of shells failed");
     }
     rc = upload_geometry(p);
     if (rc) {
         std::string msg = g_last_error;
-        ccsd_plan_destroy(p);
+        ccsd_plan_destroy_impl(p);
         return fail(rc, msg);
     }
-    std::vector<int2> kpairs;
-    for (int k = 0; k < nbas; k++)
-        for (int l = 0; l <= k; l++) kpairs.push_back(make_int2(k, l));
-    p->ket = make_classes(p, kpairs);
-    rc = upload_classes(p->ket);
+    rc = build_ket(p);
     if (rc) {
         std::string msg = g_last_error;
-        ccsd_plan_destroy(p);
+        ccsd_plan_destroy_impl(p);
         return fail(rc, msg);
     }
     if (c2s) {
@@ -390,7 +751,7 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
                 const double v = c2s[(size_t)c * aos + s];
                 if (v != 0.0) {
                     if (t.n == 6) {
-                        ccsd_plan_destroy(p);
+                        ccsd_plan_destroy_impl(p);
                         return fail(CCSD_ERR_UNSUPPORTED, "cart2sph column with more than 6 terms");
                     }
                     t.idx[t.n] = c;
@@ -403,7 +764,7 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
         if (sizeof(C2sTermHost) != ccsd_c2s_term_size() ||
             hipMalloc(&p->d_c2s, sizeof(C2sTermHost) * aos) != hipSuccess ||
             hipMemcpy(p->d_c2s, p->c2s.data(), sizeof(C2sTermHost) * aos, hipMemcpyHostToDevice) != hipSuccess) {
-            ccsd_plan_destroy(p);
+            ccsd_plan_destroy_impl(p);
             return fail(CCSD_ERR_HIP, "upload of cart2sph terms failed");
         }
     }
@@ -411,11 +772,41 @@ int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, 
     return CCSD_OK;
 }
 
-int ccsd_plan_destroy(ccsd_plan* p)
+}  // namespace
+
+extern "C" {
+
+const char* ccsd_last_error(void) { return g_last_error.c_str(); }
+
+int ccsd_plan_create(ccsd_plan** out, const int* atm, int natm, const int* bas, int nbas, const double* env, int nenv,
+                     const double* c2s)
+{
+    return create_plan(out, 1, &nbas, atm, natm, bas, nbas, env, nenv, c2s);
+}
+
+int ccsd_plan_create_batch(ccsd_plan** out, int nmol, const int* mol_nbas, const int* atm, int natm, const int* bas,
+                           int nbas, const double* env, int nenv)
+{
+    if (!mol_nbas) return fail(CCSD_ERR_NULL, "null mol_nbas");
+    if (nmol <= 0) return fail(CCSD_ERR_SHAPE, "empty batch");
+    return create_plan(out, nmol, mol_nbas, atm, natm, bas, nbas, env, nenv, nullptr);
+}
+
+int ccsd_plan_destroy(ccsd_plan* p) { return ccsd_plan_destroy_impl(p); }
+
+}  // extern "C"
+
+namespace {
+
+int ccsd_plan_destroy_impl(ccsd_plan* p)
 {
     if (!p) return CCSD_OK;
     free_classes(p->ket);
-    for (auto& kv : p->bra_cache) free_classes(kv.second);
+    clear_bra_cache(p);
+    if (p->d_mol_ao0) (void)hipFree(p->d_mol_ao0);
+    if (p->d_mol_nao) (void)hipFree(p->d_mol_nao);
+    if (p->d_mol_dm_off) (void)hipFree(p->d_mol_dm_off);
+    if (p->d_de_scratch) (void)hipFree(p->d_de_scratch);
     if (p->d_shells) (void)hipFree(p->d_shells);
     if (p->d_env) (void)hipFree(p->d_env);
     if (p->d_coords) (void)hipFree(p->d_coords);
@@ -425,6 +816,10 @@ int ccsd_plan_destroy(ccsd_plan* p)
     return CCSD_OK;
 }
 
+}  // namespace
+
+extern "C" {
+
 int ccsd_plan_set_env(ccsd_plan* p, const double* env, int nenv)
 {
     int rc = check_plan(p);
@@ -432,6 +827,7 @@ int ccsd_plan_set_env(ccsd_plan* p, const double* env, int nenv)
     if (!env) return fail(CCSD_ERR_NULL, "null env");
     if (nenv != p->nenv) return fail(CCSD_ERR_SHAPE, "env length differs from the plan's");
     p->env.assign(env, env + nenv);
+    p->q_valid = false;     // Schwarz factors follow the geometry (recomputed at the next screened call)
     return upload_geometry(p);
 }
 
@@ -457,11 +853,68 @@ int ccsd_grad_ip1_gamma(ccsd_plan* p, double* de, const double* gamma, double sc
     int rc = check_plan(p);
     if (rc) return rc;
     if (!de || !gamma) return fail(CCSD_ERR_NULL, "null de or gamma");
+    if (p->nmol != 1) return fail(CCSD_ERR_UNSUPPORTED, "explicit gamma needs a single-molecule plan");
     if (ish0 < 0 || ish1 > p->nbas || ish0 >= ish1) return fail(CCSD_ERR_SHAPE, "bad shell range");
-    std::vector<PairClass>* bra = nullptr;
+    rc = ensure_screening(p, (hipStream_t)stream);
+    if (rc) return rc;
+    BraSet* bra = nullptr;
     rc = get_bra(p, ish0, ish1, &bra);
     if (rc) return rc;
-    return run_classes(p, *bra, 1, nullptr, 0, 0, gamma, de, scale, (hipStream_t)stream);
+    return run_fused(p, *bra, ish0, ish1, de, scale, 0, gamma, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int ccsd_grad_ip1_separable(ccsd_plan* p, double* de, const double* dm, const double* pm, double scale, void* stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!de || !dm || !pm) return fail(CCSD_ERR_NULL, "null de, dm or pm");
+    rc = ensure_screening(p, (hipStream_t)stream);
+    if (rc) return rc;
+    BraSet* bra = nullptr;
+    rc = get_bra(p, 0, p->nbas, &bra);
+    if (rc) return rc;
+    return run_fused(p, *bra, 0, p->nbas, de, scale, 1, nullptr, dm, pm, (hipStream_t)stream);
+}
+
+int ccsd_plan_screen(ccsd_plan* p, double thr, void* stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!(thr >= 0.0)) return fail(CCSD_ERR_SHAPE, "screening threshold must be >= 0");
+    if (thr == p->thr) return CCSD_OK;
+    p->thr = thr;
+    clear_bra_cache(p);
+    if (thr > 0.0 && !p->q_valid) {
+        rc = compute_q(p, (hipStream_t)stream);
+        if (rc) return rc;
+    }
+    return build_ket(p);
+}
+
+int ccsd_schwarz(ccsd_plan* p, double* q, void* stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!q) return fail(CCSD_ERR_NULL, "null output pointer");
+    if (!p->q_valid) {
+        rc = compute_q(p, (hipStream_t)stream);
+        if (rc) return rc;
+        if (p->thr > 0.0) {
+            clear_bra_cache(p);
+            rc = build_ket(p);
+            if (rc) return rc;
+        }
+    }
+    HIP_TRY(hipMemcpyAsync(q, p->q.data(), sizeof(double) * p->q.size(), hipMemcpyHostToDevice, (hipStream_t)stream));
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CCSD_OK;
+}
+
+int ccsd_plan_quartets(const ccsd_plan* p, long long* count)
+{
+    if (!p || !count) return fail(CCSD_ERR_NULL, "null argument");
+    *count = p->last_quartets;
+    return CCSD_OK;
 }
 
 int ccsd_int1e(ccsd_plan* p, int kind, double* out, void* stream)
@@ -524,6 +977,14 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
     if (rc) return rc;
     if (option == 0) {
         p->fused_reg = value != 0;
+        clear_bra_cache(p);
+        p->choice.clear();
+        return CCSD_OK;
+    }
+    if (option == 1) {
+        p->autotune = value != 0;
+        clear_bra_cache(p);
+        p->choice.clear();
         return CCSD_OK;
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");

@@ -29,7 +29,11 @@ struct ShellInfo {
     int l, nprim, nctr, atom;
     int pexp, pcoef;     // offsets into env
     int ao_cart, ao_sph; // first AO (contraction column 0) in each basis
+    int mol, pad_;       // molecule of the shell (multi-molecule plans)
 };
+
+// One unit of quartet work: bra pair x, ket pairs [y, z) of the class lists.
+// A chunk never mixes molecules, so all its quartets share the bra atom.
 
 // Launch description of one angular-momentum class of shell quartets.
 struct EriClassArgs {
@@ -53,8 +57,40 @@ struct EriClassArgs {
     int i_base, nao;
     long long npair;
     long long comp_stride;
-    // fused gradient (deriv == 1, out == nullptr): de[atom_i][3] -= sum ip1 * gamma
+    // fused gradient (deriv == 1, out == nullptr): de[atom_i][3] += scale * sum ip1 * gamma
     const double* gamma;       // Cartesian (nao, nao, npair) packed like the output
     double* de;
     double gamma_scale;
+    // work list
+    const int4* chunks;        // (bra index, ket begin, ket end, unused)
+    int nchunks;
+    // Schwarz mode (qmax != nullptr): qmax[ish * nbas + jsh] = max sqrt((ij|ij)) as IEEE bits
+    unsigned long long* qmax;
+    int nbas;
+    // separable two-particle density (gamma_mode 1), per molecule m, Cartesian AOs local to m:
+    // Gamma_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4,
+    // halved on the packed diagonal k == l like the explicit form
+    int gamma_mode;
+    const double* dm;
+    const double* pm;
+    const long long* mol_dm_off; // offset of molecule m's (nao_m x nao_m) block in dm / pm
+    const int* mol_ao0;          // first Cartesian AO of molecule m
+    const int* mol_nao;          // Cartesian AOs of molecule m
 };
+
+#ifdef __HIPCC__
+// Two-particle density element for Cartesian AOs (global indices) i j k l, k >= l,
+// of the packed (diagonal-halved) form, in either representation.
+__device__ __forceinline__ double gamma_packed(const EriClassArgs& a, int mol, int i, int j, int k, int l)
+{
+    if (a.gamma_mode == 0) return a.gamma[((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l];
+    const int o = a.mol_ao0[mol], n = a.mol_nao[mol];
+    const double* D = a.dm + a.mol_dm_off[mol];
+    const double* P = a.pm + a.mol_dm_off[mol];
+    i -= o; j -= o; k -= o; l -= o;
+    const double g = D[i * n + j] * P[k * n + l] + P[i * n + j] * D[k * n + l] -
+                     0.25 * (D[i * n + l] * P[k * n + j] + D[i * n + k] * P[l * n + j] + P[i * n + l] * D[j * n + k] +
+                             P[i * n + k] * D[j * n + l]);
+    return k == l ? 0.5 * g : g;
+}
+#endif

@@ -1,7 +1,8 @@
 // Rys-quadrature ERI kernels for gfx950 (CDNA4), FP64.
 //
-// One 64-lane wavefront evaluates one shell quartet (ish jsh | ksh lsh) of an
-// angular-momentum class; classes are launched separately with their own LDS
+// One 64-lane wavefront evaluates the shell quartets (ish jsh | ksh lsh) of one
+// work chunk (a bra pair and a run of ket pairs of one angular-momentum class)
+// one after the other; classes are launched separately with their own LDS
 // size. Work inside a quartet is spread over the wave in three phases:
 //
 //   A0  lanes = primitive quartets: Gaussian-product data, Boys argument T,
@@ -16,8 +17,10 @@
 //       partial sums kept in an LDS table owned lane-by-lane
 //
 // then a reduction over item groups and either a Cartesian s2kl store
-// (int2e_ip1_module.py:636-663 layout) or, for the fused gradient, a dot with
-// the two-particle density and one atomic add per wave into dE/dR.
+// (int2e_ip1_module.py:636-663 layout), a Schwarz maximum over the diagonal
+// (ab|ab) of a pair with itself, or, for the fused gradient, a dot with the
+// two-particle density (explicit or separable) and one atomic add per chunk
+// into dE/dR.
 #include "eri_common.h"
 
 #include "rys_device.h"
@@ -65,14 +68,12 @@ __host__ __device__ inline LdsLayout lds_layout(const EriClassArgs& a)
     return L;
 }
 
-__global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
+// One shell quartet (bra pair bq | ket pair kq); derivative sums for the fused
+// gradient accumulate into gdx, gdy, gdz (lane-private).
+__device__ __forceinline__ void quartet(const EriClassArgs& a, const LdsLayout& L, double* lds, int bq, int kq,
+                                        double& gdx, double& gdy, double& gdz)
 {
-    extern __shared__ double lds[];
-    const LdsLayout L = lds_layout(a);
     const int lane = threadIdx.x;
-    const long long q = blockIdx.x;
-    const int bq = (int)(q / a.nket);
-    const int kq = (int)(q - (long long)bq * a.nket);
     const int2 bp = a.bra[bq];
     const int2 kp = a.ket[kq];
     const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
@@ -140,9 +141,6 @@ __global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
     const int grp = lane / O, ol = lane - grp * O;
     const int npq = nij * nkl;
     const bool same_kl = (kp.x == kp.y);
-
-    // fused-gradient accumulators (lane-private)
-    double gdx = 0.0, gdy = 0.0, gdz = 0.0;
 
     for (int u0 = 0; u0 < nunits; u0 += a.units_pass) {
         const int u1 = min(nunits, u0 + a.units_pass);
@@ -305,27 +303,47 @@ __global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
             const int k = sk.ao_cart + c_k * nfk + fk;
             const int l = sl.ao_cart + c_l * nfl + fl;
             if (same_kl && l > k) continue;
-            const long long idx = ((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l;
             if (a.out) {
+                const long long idx = ((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l;
                 for (int c = 0; c < ncomp; c++) a.out[c * a.comp_stride + idx] = v[c];
+            } else if (a.qmax) {
+                // diagonal element (ab|ab) of a bra pair with itself: Schwarz factor
+                if (i == k && j == l) {
+                    const double qv = sqrt(fmax(v[0], 0.0));
+                    atomicMax(a.qmax + bp.x * a.nbas + bp.y, (unsigned long long)__double_as_longlong(qv));
+                }
             } else {
-                const double gm = a.gamma[idx];
+                const double gm = gamma_packed(a, si.mol, i, j, k, l);
                 gdx += v[0] * gm; gdy += v[1] * gm; gdz += v[2] * gm;
             }
         }
         __syncthreads();
     }
-    if (!a.out) {
+}
+
+// One block (one wave) per chunk: bra pair ch.x with ket pairs [ch.y, ch.z).
+__global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
+{
+    extern __shared__ double lds[];
+    const LdsLayout L = lds_layout(a);
+    const int4 ch = a.chunks[blockIdx.x];
+    double gdx = 0.0, gdy = 0.0, gdz = 0.0;
+    for (int kq = ch.y; kq < ch.z; kq++) {
+        __syncthreads();   // LDS of the previous quartet fully consumed
+        quartet(a, L, lds, ch.x, kq, gdx, gdy, gdz);
+    }
+    if (!a.out && !a.qmax) {
         // wave reduction of the three derivative sums, one atomic per component
         for (int off = 32; off > 0; off >>= 1) {
             gdx += __shfl_down(gdx, off, 64);
             gdy += __shfl_down(gdy, off, 64);
             gdz += __shfl_down(gdz, off, 64);
         }
-        if (lane == 0) {
-            atomicAdd(a.de + 3 * si.atom + 0, a.gamma_scale * gdx);
-            atomicAdd(a.de + 3 * si.atom + 1, a.gamma_scale * gdy);
-            atomicAdd(a.de + 3 * si.atom + 2, a.gamma_scale * gdz);
+        if (threadIdx.x == 0) {
+            const int atom = a.shells[a.bra[ch.x].x].atom;
+            atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * gdx);
+            atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * gdy);
+            atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * gdz);
         }
     }
 }
@@ -390,10 +408,9 @@ size_t ccsd_eri_lds_bytes(const EriClassArgs& a) { return sizeof(double) * ccsd:
 
 hipError_t ccsd_launch_eri_class(const EriClassArgs& a, hipStream_t stream)
 {
-    const long long nblk = (long long)a.nbra * a.nket;
-    if (nblk <= 0) return hipSuccess;
+    if (a.nchunks <= 0) return hipSuccess;
     const size_t lds = ccsd_eri_lds_bytes(a);
-    hipLaunchKernelGGL(ccsd::eri_quartet_kernel, dim3((unsigned)nblk), dim3(64), lds, stream, a);
+    hipLaunchKernelGGL(ccsd::eri_quartet_kernel, dim3((unsigned)a.nchunks), dim3(64), lds, stream, a);
     return hipGetLastError();
 }
 

@@ -1,0 +1,369 @@
+// Fused int2e_ip1 x Gamma contraction, register-resident, one class per template.
+//
+//   de[atom(i)][x] += scale * sum_{i, j, k>=l} ip1_x(ij|kl) Gamma[i][j][k(k+1)/2+l]
+//
+// One lane evaluates one shell quartet (ish jsh | ksh lsh): it loops over the
+// primitive quartets, evaluates the Rys roots, builds the three 1-D factors
+// (VRR on centre i, TRR i->k, HRR i->j and k->l; the same recurrences as the
+// LDS kernel) in registers with compile-time extents, and contracts the
+// derivative integrals with Gamma on the fly -- no integral is stored.
+//
+// Register budget: a lane holds Gamma for its quartet (all contraction
+// columns) plus one root's three 1-D factors. Classes whose Gamma does not fit
+// are split over P <= 4 waves ("parts"): part q contracts the Cartesian
+// functions [q NF / P, (q+1) NF / P) and recomputes the roots and 1-D factors,
+// which cost far less than the contraction. The part index is uniform over a
+// wave, so the P specialised bodies never diverge.
+//
+// General contractions (GC = 1): Gamma of every contraction-column combination
+// is loaded once per quartet and combined with the primitive coefficients of
+// each primitive quartet, so each primitive quartet's integrals are evaluated
+// once for all columns. GC = 0 launches hold only quartets whose shells are
+// all single-column (the host splits the classes), and the coefficients then
+// fold into the prefactor.
+//
+// A wave takes one (work chunk, part): one bra pair and up to 64 ket pairs, so
+// one atom receives the wave's three sums.
+#pragma once
+#include <utility>
+
+#include "eri_common.h"
+#include "rys_device.h"
+
+namespace ccsd_reg {
+
+template <int L>
+struct CartTab {
+    // (lx, ly, lz) of the Cartesian components, PySCF order
+    static constexpr int n = (L + 1) * (L + 2) / 2;
+    int x[n], y[n], z[n];
+    constexpr CartTab() : x(), y(), z()
+    {
+        int c = 0;
+        for (int lx = L; lx >= 0; lx--)
+            for (int ly = L - lx; ly >= 0; ly--) {
+                x[c] = lx;
+                y[c] = ly;
+                z[c] = L - lx - ly;
+                c++;
+            }
+    }
+};
+
+__device__ __forceinline__ double shell_norm_dev(int l)
+{
+    return l == 0 ? 0.28209479177387814347 : (l == 1 ? 0.48860251190291992159 : 1.0);
+}
+
+#ifndef CCSD_REG_BUDGET
+#define CCSD_REG_BUDGET 110
+#endif
+constexpr int kBudget = CCSD_REG_BUDGET;   // doubles a lane may hold for Gamma and the 1-D factors
+constexpr int kMaxParts = 4;   // beyond this the class stays on the LDS kernel
+
+constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
+// contraction columns a shell may bring into a register class: generally contracted
+// s shells (two columns), single-column shells otherwise
+constexpr int mcol(int l) { return l == 0 ? 2 : 1; }
+constexpr bool has_s(int li, int lj, int lk, int ll) { return li == 0 || lj == 0 || lk == 0 || ll == 0; }
+
+constexpr int need(int li, int lj, int lk, int ll, int gc, int p)
+{
+    const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
+    const int nfp = (nf + p - 1) / p;
+    const int gs = (li + 2) * (lj + 1) * (lk + 1) * (ll + 1);
+    const int m = gc ? mcol(li) * mcol(lj) * mcol(lk) * mcol(ll) : 1;
+    return (m > 1 ? m * nfp + nfp : nfp) + 3 * gs;
+}
+
+// Smallest part count that fits the budget (0: class not supported).
+constexpr int min_parts(int li, int lj, int lk, int ll, int gc)
+{
+    if (li > 2 || lj > 2 || lk > 2 || ll > 2) return 0;
+    const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
+    for (int p = 1; p <= kMaxParts && p <= nf; p *= 2)
+        if (need(li, lj, lk, ll, gc, p) <= kBudget) return p;
+    return 0;
+}
+
+// One direction: G[a][b][c][d], a <= LI+1, b <= LJ, c <= LK, d <= LL.
+template <int LI, int LJ, int LK, int LL>
+__device__ __forceinline__ void factor1d(double (&G)[LI + 2][LJ + 1][LK + 1][LL + 1], double seed, double c00, double cp,
+                                         double b00, double b10, double b01, double AB, double CD)
+{
+    constexpr int LIJ = LI + LJ + 1, LKL = LK + LL;
+    double v[LIJ + 1][LKL + 1];
+    v[0][0] = seed;
+#pragma unroll
+    for (int i = 0; i < LIJ; i++) v[i + 1][0] = c00 * v[i][0] + (i > 0 ? i * b10 * v[i - 1][0] : 0.0);
+#pragma unroll
+    for (int k = 0; k < LKL; k++)
+#pragma unroll
+        for (int i = 0; i <= LIJ; i++) {
+            double s = cp * v[i][k];
+            if (k > 0) s += k * b01 * v[i][k - 1];
+            if (i > 0) s += i * b00 * v[i - 1][k];
+            v[i][k + 1] = s;
+        }
+    // HRR i -> j: h[a][b][k], (a, b+1) = (a+1, b) - AB (a, b)
+    double h[LIJ + 1][LJ + 1][LKL + 1];
+#pragma unroll
+    for (int k = 0; k <= LKL; k++) {
+#pragma unroll
+        for (int a = 0; a <= LIJ; a++) h[a][0][k] = v[a][k];
+#pragma unroll
+        for (int b = 0; b < LJ; b++)
+#pragma unroll
+            for (int a = 0; a < LIJ - b; a++) h[a][b + 1][k] = h[a + 1][b][k] - AB * h[a][b][k];
+    }
+    // HRR k -> l
+#pragma unroll
+    for (int a = 0; a <= LI + 1; a++)
+#pragma unroll
+        for (int b = 0; b <= LJ; b++) {
+            double t[LKL + 1][LL + 1];
+#pragma unroll
+            for (int c = 0; c <= LKL; c++) t[c][0] = h[a][b][c];
+#pragma unroll
+            for (int e = 0; e < LL; e++)
+#pragma unroll
+                for (int c = 0; c < LKL - e; c++) t[c][e + 1] = t[c + 1][e] - CD * t[c][e];
+#pragma unroll
+            for (int c = 0; c <= LK; c++)
+#pragma unroll
+                for (int e = 0; e <= LL; e++) G[a][b][c][e] = t[c][e];
+        }
+}
+
+// Part Q of P of the quartet handled by this lane; returns the lane's three sums.
+template <int LI, int LJ, int LK, int LL, int GC, int P, int Q>
+__device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 ch, int lane, double& sx, double& sy,
+                                             double& sz)
+{
+    constexpr int NFI = ncart_c(LI), NFJ = ncart_c(LJ), NFK = ncart_c(LK), NFL = ncart_c(LL);
+    constexpr int NF = NFI * NFJ * NFK * NFL;
+    constexpr int F0 = NF * Q / P, F1 = NF * (Q + 1) / P, NFP = F1 - F0;
+    constexpr int NROOTS = (LI + LJ + LK + LL + 1) / 2 + 1;
+    constexpr int MIJ = GC ? mcol(LI) * mcol(LJ) : 1;
+    constexpr int MKL = GC ? mcol(LK) * mcol(LL) : 1;
+    constexpr CartTab<LI> ci_t{};
+    constexpr CartTab<LJ> cj_t{};
+    constexpr CartTab<LK> ck_t{};
+    constexpr CartTab<LL> cl_t{};
+
+    const int kq = ch.y + lane;
+    if (kq >= ch.z) return;
+    const int2 bp = a.bra[ch.x];
+    const int2 kp = a.ket[kq];
+    const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+    const ShellInfo sk = a.shells[kp.x], sl = a.shells[kp.y];
+    const double* RA = a.coords + 3 * si.atom;
+    const double* RB = a.coords + 3 * sj.atom;
+    const double* RC = a.coords + 3 * sk.atom;
+    const double* RD = a.coords + 3 * sl.atom;
+    const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
+    const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
+    const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    const double rcd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
+    const double norm =
+        34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) * shell_norm_dev(LL);
+    const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
+    const bool same_kl = kp.x == kp.y;
+    const double* ei = a.env + si.pexp;
+    const double* ej = a.env + sj.pexp;
+    const double* ek = a.env + sk.pexp;
+    const double* el = a.env + sl.pexp;
+    const double* cfi = a.env + si.pcoef;
+    const double* cfj = a.env + sj.pcoef;
+    const double* cfk = a.env + sk.pcoef;
+    const double* cfl = a.env + sl.pcoef;
+
+    // Gamma of this part for every contraction-column combination, loaded once
+    double gam[MIJ][MKL][NFP];
+#pragma unroll
+    for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+        for (int ckl = 0; ckl < MKL; ckl++)
+#pragma unroll
+            for (int t = 0; t < NFP; t++) {
+                const int f = F0 + t;
+                const int fi = f % NFI, r1 = f / NFI;
+                const int fj = r1 % NFJ, r2 = r1 / NFJ;
+                const int fk = r2 % NFK, fl = r2 / NFK;
+                double g = 0.0;
+                if (cij < ncij && ckl < nckl) {
+                    const int c_i = cij / sj.nctr, c_j = cij - c_i * sj.nctr;
+                    const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
+                    const int i = si.ao_cart + c_i * NFI + fi;
+                    const int j = sj.ao_cart + c_j * NFJ + fj;
+                    const int k = sk.ao_cart + c_k * NFK + fk;
+                    const int l = sl.ao_cart + c_l * NFL + fl;
+                    if (!(same_kl && l > k)) g = gamma_packed(a, si.mol, i, j, k, l);
+                }
+                gam[cij][ckl][t] = g;
+            }
+
+    for (int ip = 0; ip < si.nprim; ip++)
+        for (int jp = 0; jp < sj.nprim; jp++) {
+            const double ai = ei[ip], aj = ej[jp], aij = ai + aj;
+            const double Kab = exp(-ai * aj / aij * rab2);
+            const double Pc[3] = {RA[0] + AB[0] * (aj / aij), RA[1] + AB[1] * (aj / aij), RA[2] + AB[2] * (aj / aij)};
+            double cb[MIJ];
+#pragma unroll
+            for (int cij = 0; cij < MIJ; cij++) {
+                const int c_i = cij / sj.nctr, c_j = cij - c_i * sj.nctr;
+                cb[cij] = cij < ncij ? cfi[c_i * si.nprim + ip] * cfj[c_j * sj.nprim + jp] : 0.0;
+            }
+            for (int kp2 = 0; kp2 < sk.nprim; kp2++)
+                for (int lp = 0; lp < sl.nprim; lp++) {
+                    const double ak = ek[kp2], al = el[lp], akl = ak + al;
+                    const double Kcd = exp(-ak * al / akl * rcd2);
+                    const double Qc[3] = {RC[0] + CD[0] * (al / akl), RC[1] + CD[1] * (al / akl),
+                                          RC[2] + CD[2] * (al / akl)};
+                    const double PQ[3] = {Pc[0] - Qc[0], Pc[1] - Qc[1], Pc[2] - Qc[2]};
+                    const double rr = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+                    const double T = aij * akl / (aij + akl) * rr;
+                    double ck[MKL];
+#pragma unroll
+                    for (int ckl = 0; ckl < MKL; ckl++) {
+                        const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
+                        ck[ckl] = ckl < nckl ? cfk[c_k * sk.nprim + kp2] * cfl[c_l * sl.nprim + lp] : 0.0;
+                    }
+                    double pref = norm * Kab * Kcd / (aij * akl * sqrt(aij + akl));
+                    // Gamma of this primitive quartet (the columns' coefficients applied)
+                    double ge[NFP];
+                    if constexpr (MIJ * MKL == 1) {
+                        pref *= cb[0] * ck[0];
+#pragma unroll
+                        for (int t = 0; t < NFP; t++) ge[t] = gam[0][0][t];
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < NFP; t++) ge[t] = 0.0;
+#pragma unroll
+                        for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+                            for (int ckl = 0; ckl < MKL; ckl++) {
+                                const double cc = cb[cij] * ck[ckl];
+#pragma unroll
+                                for (int t = 0; t < NFP; t++) ge[t] += cc * gam[cij][ckl][t];
+                            }
+                    }
+                    double rt[NROOTS], wt[NROOTS];
+                    ccsd::rys_roots_dev(NROOTS, T, rt, wt);
+                    const double ai2 = 2.0 * ai;
+#pragma unroll
+                    for (int r = 0; r < NROOTS; r++) {
+                        const double a2 = rt[r] / (aij + akl);
+                        const double rt_aij = a2 * akl, rt_akl = a2 * aij;
+                        const double b00 = 0.5 * a2;
+                        const double b10 = 0.5 / aij * (1.0 - rt_aij);
+                        const double b01 = 0.5 / akl * (1.0 - rt_akl);
+                        double Gx[LI + 2][LJ + 1][LK + 1][LL + 1];
+                        double Gy[LI + 2][LJ + 1][LK + 1][LL + 1];
+                        double Gz[LI + 2][LJ + 1][LK + 1][LL + 1];
+                        factor1d<LI, LJ, LK, LL>(Gx, 1.0, (Pc[0] - RA[0]) - rt_aij * PQ[0],
+                                                 (Qc[0] - RC[0]) + rt_akl * PQ[0], b00, b10, b01, AB[0], CD[0]);
+                        factor1d<LI, LJ, LK, LL>(Gy, 1.0, (Pc[1] - RA[1]) - rt_aij * PQ[1],
+                                                 (Qc[1] - RC[1]) + rt_akl * PQ[1], b00, b10, b01, AB[1], CD[1]);
+                        factor1d<LI, LJ, LK, LL>(Gz, wt[r] * pref, (Pc[2] - RA[2]) - rt_aij * PQ[2],
+                                                 (Qc[2] - RC[2]) + rt_akl * PQ[2], b00, b10, b01, AB[2], CD[2]);
+#pragma unroll
+                        for (int t = 0; t < NFP; t++) {
+                            const int f = F0 + t;
+                            const int fi = f % NFI, r1 = f / NFI;
+                            const int fj = r1 % NFJ, r2 = r1 / NFJ;
+                            const int fk = r2 % NFK, fl = r2 / NFK;
+                            const int ix = ci_t.x[fi], iy = ci_t.y[fi], iz = ci_t.z[fi];
+                            const int jx = cj_t.x[fj], jy = cj_t.y[fj], jz = cj_t.z[fj];
+                            const int kx = ck_t.x[fk], ky = ck_t.y[fk], kz = ck_t.z[fk];
+                            const int lx = cl_t.x[fl], ly = cl_t.y[fl], lz = cl_t.z[fl];
+                            const double gx = Gx[ix][jx][kx][lx];
+                            const double gy = Gy[iy][jy][ky][ly];
+                            const double gz = Gz[iz][jz][kz][lz];
+                            double dx = -ai2 * Gx[ix + 1][jx][kx][lx];
+                            double dy = -ai2 * Gy[iy + 1][jy][ky][ly];
+                            double dz = -ai2 * Gz[iz + 1][jz][kz][lz];
+                            if (ix > 0) dx += ix * Gx[ix > 0 ? ix - 1 : 0][jx][kx][lx];
+                            if (iy > 0) dy += iy * Gy[iy > 0 ? iy - 1 : 0][jy][ky][ly];
+                            if (iz > 0) dz += iz * Gz[iz > 0 ? iz - 1 : 0][jz][kz][lz];
+                            const double g = ge[t];
+                            sx += g * dx * gy * gz;
+                            sy += g * gx * dy * gz;
+                            sz += g * gx * gy * dz;
+                        }
+                    }
+                }
+        }
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int P, int... Q>
+__device__ __forceinline__ void run_parts(const EriClassArgs& a, const int4 ch, int lane, int part, double& sx,
+                                          double& sy, double& sz, std::integer_sequence<int, Q...>)
+{
+    ((part == Q ? quartet_part<LI, LJ, LK, LL, GC, P, Q>(a, ch, lane, sx, sy, sz) : void()), ...);
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int P>
+__global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
+{
+    if constexpr (min_parts(LI, LJ, LK, LL, GC) == 0 || P < min_parts(LI, LJ, LK, LL, GC)) {
+        return;
+    } else {
+        const int lane = threadIdx.x & 63;
+        const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        const int c = gw / P;
+        if (c >= a.nchunks) return;   // whole waves leave; nothing below synchronizes the block
+        const int part = gw - c * P;
+        const int4 ch = a.chunks[c];
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        run_parts<LI, LJ, LK, LL, GC, P>(a, ch, lane, part, sx, sy, sz, std::make_integer_sequence<int, P>{});
+        // wave reduction (the chunk's bra pair, hence the atom of i, is shared by the wave)
+        for (int off = 32; off > 0; off >>= 1) {
+            sx += __shfl_down(sx, off, 64);
+            sy += __shfl_down(sy, off, 64);
+            sz += __shfl_down(sz, off, 64);
+        }
+        if (lane == 0) {
+            const int atom = a.shells[a.bra[ch.x].x].atom;
+            atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * sx);
+            atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * sy);
+            atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * sz);
+        }
+    }
+}
+
+template <int LI, int LJ, int LK, int LL, int GC>
+hipError_t launch(const EriClassArgs& a, int wpb, hipStream_t stream)
+{
+    constexpr int G = (GC && has_s(LI, LJ, LK, LL)) ? 1 : 0;
+    constexpr int P0 = min_parts(LI, LJ, LK, LL, G);
+    constexpr int P = P0 ? P0 : 1;
+    if (P0 == 0) return hipErrorInvalidValue;
+    const long long waves = (long long)a.nchunks * P;
+    const long long blocks = (waves + wpb - 1) / wpb;
+    if (blocks > 2147483647LL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fused_reg_kernel<LI, LJ, LK, LL, G, P>), dim3((unsigned)blocks), dim3(64 * wpb), 0, stream, a);
+    return hipGetLastError();
+}
+
+}  // namespace ccsd_reg
+
+// Launcher of the classes with bra angular momenta (LI, LJ); each pair is its own
+// object file (compiled in parallel).
+#define CCSD_REG_CASE(I, J, K, L)                                                                            \
+    case (K * 3 + L):                                                                                        \
+        return gc ? ccsd_reg::launch<I, J, K, L, 1>(a, wpb, stream) : ccsd_reg::launch<I, J, K, L, 0>(a, wpb, stream);
+#define CCSD_REG_L(I, J, K) CCSD_REG_CASE(I, J, K, 0) CCSD_REG_CASE(I, J, K, 1) CCSD_REG_CASE(I, J, K, 2)
+#define CCSD_REG_TU(I, J, NAME)                                                                              \
+    hipError_t NAME(const EriClassArgs& a, int gc, int wpb, hipStream_t stream)                              \
+    {                                                                                                        \
+        if (a.nchunks <= 0) return hipSuccess;                                                               \
+        if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;                                                 \
+        switch (a.lk * 3 + a.ll) {                                                                           \
+            CCSD_REG_L(I, J, 0)                                                                              \
+            CCSD_REG_L(I, J, 1)                                                                              \
+            CCSD_REG_L(I, J, 2)                                                                              \
+            default:                                                                                         \
+                return hipErrorInvalidValue;                                                                 \
+        }                                                                                                    \
+    }

@@ -77,3 +77,49 @@ def eri_work(mol, deriv, ish0=0, ish1=None):
     ndc_i = int(nctr[ish0:ish1].sum())
     ref_quartets = ndc_i * ndc * ndc * (ndc + 1) // 2
     return flops, quartets, ref_quartets
+
+
+def eri_work_screened(mol, deriv, q=None, thr=0.0):
+    """(flops, quartets) of a full fused pass over `mol` keeping the quartets with
+    q[i, j] * q[k, l] >= thr (every quartet when q is None or thr <= 0)."""
+    import numpy as np
+    bas = mol._bas
+    n = mol.nbas
+    l = bas[:, 1].astype(np.int64)
+    K = (bas[:, 2]).astype(np.int64)
+    C = (bas[:, 3]).astype(np.int64)
+    q = np.ones((n, n)) if q is None or thr <= 0 else np.asarray(q, dtype=np.float64)
+    thr = thr if thr > 0 else 0.0
+    ii, jj = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    bi, bj = ii.ravel(), jj.ravel()
+    kk, mm = np.tril_indices(n)
+    flops = 0.0
+    quartets = 0
+    for li in range(int(l.max()) + 1):
+        for lj in range(int(l.max()) + 1):
+            sb = (l[bi] == li) & (l[bj] == lj)
+            if not sb.any():
+                continue
+            qb = q[bi[sb], bj[sb]]
+            kb = (K[bi[sb]] * K[bj[sb]]).astype(np.float64)
+            cb = (C[bi[sb]] * C[bj[sb]]).astype(np.float64)
+            for lk in range(int(l.max()) + 1):
+                for ll in range(int(l.max()) + 1):
+                    sk = (l[kk] == lk) & (l[mm] == ll)
+                    if not sk.any():
+                        continue
+                    qk = q[kk[sk], mm[sk]]
+                    order = np.argsort(-qk, kind="stable")
+                    qs = qk[order]
+                    kq = (K[kk[sk]] * K[mm[sk]]).astype(np.float64)[order]
+                    kc = kq * (C[kk[sk]] * C[mm[sk]]).astype(np.float64)[order]
+                    cum_k = np.concatenate([[0.0], np.cumsum(kq)])
+                    cum_kc = np.concatenate([[0.0], np.cumsum(kc)])
+                    with np.errstate(divide="ignore"):
+                        cut = np.where(qb > 0, thr / np.where(qb > 0, qb, 1.0), np.inf)
+                    # survivors: qs >= cut, qs descending
+                    nsurv = np.searchsorted(-qs, -cut, side="right") if thr > 0 else np.full(len(qb), len(qs))
+                    base, per_combo = class_flops(li, lj, lk, ll, deriv)
+                    flops += float(np.sum(base * kb * cum_k[nsurv] + per_combo * kb * cb * cum_kc[nsurv]))
+                    quartets += int(nsurv.sum())
+    return flops, quartets

@@ -61,3 +61,24 @@ def sph_to_cart_4index(g_sph, c2s):
     t = torch.tensordot(t, c2s, dims=([2], [1])).permute(0, 1, 3, 2)     # (c, c, c, s)
     t = torch.tensordot(t, c2s, dims=([3], [1]))                         # (c, c, c, c)
     return t.contiguous()
+
+
+def set_screening(mol, thr):
+    """Schwarz screening of the fused gradient for `mol`'s plan: quartets with
+    q_ij q_kl < thr are skipped (thr = 0: none skipped, the default)."""
+    _lib.check(_lib.load().ccsd_plan_screen(_lib.get_plan(mol), float(thr), _lib.current_stream_ptr()),
+               "ccsd_plan_screen")
+
+
+def schwarz_factors(mol):
+    """(nbas, nbas) device tensor q_ij = max_{a in i, b in j} sqrt((ab|ab))."""
+    q = torch.empty((mol.nbas, mol.nbas), dtype=torch.float64, device="cuda")
+    _lib.check(_lib.load().ccsd_schwarz(_lib.get_plan(mol), _lib.ptr(q), _lib.current_stream_ptr()), "ccsd_schwarz")
+    return q
+
+
+def last_quartets(mol):
+    """Shell quartets evaluated by the last integral / fused call on `mol`'s plan."""
+    n = _lib.ctypes.c_longlong()
+    _lib.check(_lib.load().ccsd_plan_quartets(_lib.get_plan(mol), _lib.ctypes.byref(n)), "ccsd_plan_quartets")
+    return n.value

@@ -62,11 +62,45 @@ int ccsd_int2e_ip1_s2kl(ccsd_plan* plan, double* out, int ish0, int ish1, int ca
 int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double scale, int ish0, int ish1,
                         void* stream);
 
-/* Plan options. option 0: 1 (default) lets ccsd_grad_ip1_gamma use the
+/* Plan options (no counterpart in the reference, which has one kernel and no
+ * tuning). option 0: 1 (default) lets the fused gradient use the
  * register-resident per-class kernels where a class fits, 0 forces the LDS
- * kernel for every class (comparison and testing). No counterpart in the
- * reference (one kernel there). */
+ * kernel for every class (comparison and testing). option 1: 1 (default) times
+ * the kernel candidates of each class once per plan and keeps the fastest;
+ * 0 takes the first candidate (register kernel, 4 waves per block). */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
+
+/* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
+ * and env offsets global, as PySCF's gto.conc_env builds them), molecule m owning
+ * the next mol_nbas[m] shells. Shell pairs and quartets never mix molecules.
+ * Batch counterpart of ccsd_plan_create for many molecules of uneven size; the
+ * reference handles one molecule per call (ccsd_gpu/cuda/int2e_ip1_gpu.py:296). */
+int ccsd_plan_create_batch(ccsd_plan** plan, int nmol, const int* mol_nbas, const int* atm, int natm, const int* bas,
+                           int nbas, const double* env, int nenv);
+
+/* Schwarz screening of the fused gradient: quartets with q_ij * q_kl < thr are
+ * skipped, q_ij = max over functions a in shell i, b in shell j of sqrt((ab|ab)).
+ * thr = 0 (default) evaluates every quartet. The reference screens nothing
+ * (all nbas^2 * n_kl quartets, ccsd_gpu/cuda/int2e_ip1_gpu.py:199-214). */
+int ccsd_plan_screen(ccsd_plan* plan, double thr, void* stream);
+
+/* The Schwarz factors q (device, nbas x nbas, symmetric) of the plan's geometry. */
+int ccsd_schwarz(ccsd_plan* plan, double* q, void* stream);
+
+/* Shell quartets evaluated by the last integral or fused-gradient call. */
+int ccsd_plan_quartets(const ccsd_plan* plan, long long* count);
+
+/* Fused ERI-derivative contraction over all shells of every molecule of the plan
+ * with the separable two-particle density of Cartesian densities D and P
+ * (dm, pm: per molecule a row-major nao_m x nao_m block, blocks in molecule order):
+ *   Gamma_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4
+ *   de[atom(i)][x] += scale * sum_{i, j, k>=l} int2e_ip1[x](i j | k l) * Gamma'_ijkl
+ * with Gamma' halved on k == l (the packing of ccsd_grad_ip1_gamma). This is the
+ * density-matrix part of the reference's gradient assembly (the vhf1 J/K terms
+ * contracted with dm1p, ccsd_gpu/gpu_ccsd_grad.py:469-491) evaluated for many
+ * molecules at once without a four-index density. de: device (natm_total, 3). */
+int ccsd_grad_ip1_separable(ccsd_plan* plan, double* de, const double* dm, const double* pm, double scale,
+                            void* stream);
 
 /* One-electron integrals in the Cartesian AO basis (row-major nao_c x nao_c
  * matrices, fully overwritten): kind 0 overlap, 1 kinetic, 2 nuclear attraction
