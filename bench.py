@@ -224,6 +224,8 @@ def bench_grad(args):
     for s in range(args.warmup + args.steps):
         de, info = pipeline.ccsd_gradient(mf, mycc, "mixed", args.precision)
         log(f"mixed step {s}: {json.dumps(info)}")
+        hist = getattr(mycc, "_lambda_history", [])
+        log("lambda history: " + " ".join(f"{p[:2]}{'c' if c else ''}:{n:.1e}" for p, c, n in hist))
         if s >= args.warmup:
             res.append((de, info))
     wall = float(np.mean([i["total_s"] for _, i in res]))

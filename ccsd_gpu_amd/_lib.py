@@ -40,6 +40,10 @@ _SIGNATURES = {
     "ccsd_schwarz": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ccsd_plan_quartets": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
     "ccsd_grad_ip1_separable": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
+    "ccsd_grad_ip1_gamma_sep": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
+                                        c_void_p]),
+    "ccsd_sym_pairs": (c_int, [c_void_p, c_int, c_double, c_void_p]),
+    "ccsd_sum_slices": (c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int, c_int, c_void_p]),
 }
 EXPORTED = tuple(_SIGNATURES)
 

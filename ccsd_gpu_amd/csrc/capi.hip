@@ -29,6 +29,8 @@ hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const vo
                                       long long sC, long long ldc, int batch, int beta, hipStream_t stream);
 hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
                                   hipStream_t stream);
+hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
+hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
 hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
                                   long long lda, long long ldb, long long ldc, long long sA, long long sB,
                                   long long sC, int batch, hipStream_t stream);
@@ -863,6 +865,22 @@ int ccsd_grad_ip1_gamma(ccsd_plan* p, double* de, const double* gamma, double sc
     return run_fused(p, *bra, ish0, ish1, de, scale, 0, gamma, nullptr, nullptr, (hipStream_t)stream);
 }
 
+int ccsd_grad_ip1_gamma_sep(ccsd_plan* p, double* de, const double* gamma, const double* dm, const double* pm,
+                            double scale, int ish0, int ish1, void* stream)
+{
+    int rc = check_plan(p);
+    if (rc) return rc;
+    if (!de || !gamma || !dm || !pm) return fail(CCSD_ERR_NULL, "null de, gamma, dm or pm");
+    if (p->nmol != 1) return fail(CCSD_ERR_UNSUPPORTED, "explicit gamma needs a single-molecule plan");
+    if (ish0 < 0 || ish1 > p->nbas || ish0 >= ish1) return fail(CCSD_ERR_SHAPE, "bad shell range");
+    rc = ensure_screening(p, (hipStream_t)stream);
+    if (rc) return rc;
+    BraSet* bra = nullptr;
+    rc = get_bra(p, ish0, ish1, &bra);
+    if (rc) return rc;
+    return run_fused(p, *bra, ish0, ish1, de, scale, 2, gamma, dm, pm, (hipStream_t)stream);
+}
+
 int ccsd_grad_ip1_separable(ccsd_plan* p, double* de, const double* dm, const double* pm, double scale, void* stream)
 {
     int rc = check_plan(p);
@@ -945,6 +963,24 @@ int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M
     hipError_t e = ccsd_launch_gemm_mixed(mode, A, B, C, M, N, K, lda, ldb, ldc, strideA, strideB, strideC, batch,
                                           (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_mixed_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_sym_pairs(double* t, int n, double scale, void* stream)
+{
+    if (!t) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || n > 46340) return fail(CCSD_ERR_SHAPE, "bad dimension");
+    hipError_t e = ccsd_launch_sym_pairs(t, n, scale, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "sym_pairs_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int beta, void* stream)
+{
+    if (!ws || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || nslices < 0 || (beta != 0 && beta != 1)) return fail(CCSD_ERR_SHAPE, "bad slice-sum arguments");
+    hipError_t e = ccsd_launch_sum_slices(ws, out, n, nslices, beta, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "sum_slices_kernel");
     return CCSD_OK;
 }
 

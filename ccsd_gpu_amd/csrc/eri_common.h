@@ -67,9 +67,10 @@ struct EriClassArgs {
     // Schwarz mode (qmax != nullptr): qmax[ish * nbas + jsh] = max sqrt((ij|ij)) as IEEE bits
     unsigned long long* qmax;
     int nbas;
-    // separable two-particle density (gamma_mode 1), per molecule m, Cartesian AOs local to m:
-    // Gamma_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4,
-    // halved on the packed diagonal k == l like the explicit form
+    // gamma_mode 0: explicit packed gamma; 1: separable two-particle density of D and P,
+    // per molecule m, Cartesian AOs local to m:
+    //   Gamma_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4,
+    // halved on the packed diagonal k == l like the explicit form; 2: the sum of both
     int gamma_mode;
     const double* dm;
     const double* pm;
@@ -83,7 +84,9 @@ struct EriClassArgs {
 // of the packed (diagonal-halved) form, in either representation.
 __device__ __forceinline__ double gamma_packed(const EriClassArgs& a, int mol, int i, int j, int k, int l)
 {
-    if (a.gamma_mode == 0) return a.gamma[((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l];
+    const double ge = a.gamma_mode != 1 ? a.gamma[((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l]
+                                        : 0.0;
+    if (a.gamma_mode == 0) return ge;
     const int o = a.mol_ao0[mol], n = a.mol_nao[mol];
     const double* D = a.dm + a.mol_dm_off[mol];
     const double* P = a.pm + a.mol_dm_off[mol];
@@ -91,6 +94,6 @@ __device__ __forceinline__ double gamma_packed(const EriClassArgs& a, int mol, i
     const double g = D[i * n + j] * P[k * n + l] + P[i * n + j] * D[k * n + l] -
                      0.25 * (D[i * n + l] * P[k * n + j] + D[i * n + k] * P[l * n + j] + P[i * n + l] * D[j * n + k] +
                              P[i * n + k] * D[j * n + l]);
-    return k == l ? 0.5 * g : g;
+    return ge + (k == l ? 0.5 * g : g);
 }
 #endif

@@ -159,7 +159,28 @@ __global__ void split_bf16_kernel(const double* __restrict__ x, uint16_t* __rest
     }
 }
 
+// out[i] = beta * out[i] + sum_s ws[s * n + i]: the partial products of a K-split
+// GEMM summed in a fixed order (one pass, no atomics, no memset).
+__global__ void sum_slices_kernel(const double* __restrict__ ws, double* __restrict__ out, long long n, int S,
+                                  int beta)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        double s = beta ? out[t] : 0.0;
+        for (int q = 0; q < S; q++) s += ws[(long long)q * n + t];
+        out[t] = s;
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)nb), dim3(256), 0, stream, ws, out, n, S, beta);
+    return hipGetLastError();
+}
 
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,

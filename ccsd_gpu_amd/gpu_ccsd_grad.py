@@ -140,17 +140,35 @@ def _assemble_dm2_mo(d2):
 
 def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff):
     """Symmetrized AO 2-RDM, full (nao^4), in the AO basis of the columns of
-    `mo_coeff`'s rows (Cartesian when mo_coeff = cart2sph @ C)."""
-    dm2 = _assemble_dm2_mo(d2).permute(1, 0, 3, 2).contiguous()      # chemist's order
+    `mo_coeff`'s rows (Cartesian when mo_coeff = cart2sph @ C).
+
+    As the reference (gpu_ccsd_grad.py:272-326): the MO density in chemist's
+    order, C x C x C x C, then (ij) and (kl) symmetrised with weight 1/2 per
+    term. Here the symmetrisation runs first, in place on the MO tensor (it
+    commutes with the transform, and the reference's (pq)(rs) index swap before
+    the transform is one of the symmetrised permutations), and the transform is
+    four contiguous GEMMs, one index at a time, without permuted copies."""
+    dm2 = _assemble_dm2_mo(d2)
+    nmo = dm2.shape[0]
+    _lib_sym_pairs(dm2, 0.5)
     c = _dev(mo_coeff)
-    t = torch.tensordot(c, dm2, dims=([1], [0]))                       # (i q r s)
+    nao = c.shape[0]
+    t = (c @ dm2.reshape(nmo, -1)).reshape(nao, nmo, nmo, nmo)                 # (i q r s)
     del dm2
-    t = torch.tensordot(c, t, dims=([1], [1])).permute(1, 0, 2, 3)     # (i j r s)
-    t = torch.tensordot(t, c, dims=([2], [1])).permute(0, 1, 3, 2)     # (i j s k) -> (i j k s)
-    t = torch.tensordot(t, c, dims=([3], [1])).contiguous()            # (i j k l)
-    t = t + t.permute(1, 0, 2, 3)
-    t = t + t.permute(0, 1, 3, 2)
-    return t.mul_(0.5)
+    t = (t.reshape(-1, nmo) @ c.T).reshape(nao, nmo, nmo, nao)                 # (i q r l)
+    t = torch.matmul(c, t.reshape(nao, nmo, nmo * nao)).reshape(nao, nao, nmo, nao)   # (i j r l)
+    t = torch.matmul(c, t.reshape(nao * nao, nmo, nao))                        # (i j k l)
+    return t.reshape(nao, nao, nao, nao)
+
+
+def _lib_sym_pairs(t, scale):
+    from . import _lib
+    n = t.shape[0]
+    if t.dim() != 4 or any(d != n for d in t.shape) or not t.is_contiguous() or t.dtype != torch.float64:
+        raise ValueError("sym_pairs needs a contiguous float64 (n, n, n, n) tensor")
+    _lib.check(_lib.load().ccsd_sym_pairs(_lib.ptr(t), n, float(scale), _lib.current_stream_ptr()),
+               "ccsd_sym_pairs")
+    return t
 
 
 def _pack_halved(full):
@@ -291,10 +309,10 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     Dc = c2s @ hf_dm1 @ c2s.T
     Pc = c2s @ dm1p @ c2s.T
     if fused:
-        gam += _separable_gamma(Dc, Pc)
+        # one pass: the CCSD density explicitly, the D/P separable part formed in the kernel
         gp = _pack_halved(gam)
         del gam
-        contract_int2e_ip1(mol, gp, scale=-2.0, de=de2)
+        contract_int2e_ip1(mol, gp, scale=-2.0, de=de2, dm=Dc.contiguous(), pm=Pc.contiguous())
         del gp
     else:
         gp = _pack_halved(gam)

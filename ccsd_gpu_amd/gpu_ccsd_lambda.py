@@ -209,8 +209,39 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     return l1new, l2new
 
 
+class GraphedUpdate:
+    """l -> F(l) (or A l when homogeneous) captured once as a HIP graph and replayed.
+
+    The capture records every kernel of one lambda update (MFMA GEMMs, splits,
+    slice sums, elementwise work) on a side stream after two eager warm-up calls
+    (which also fill the static-operand cache); each call then copies (l1, l2)
+    into the graph's input buffers and replays. The returned tensors are the
+    graph's output buffers: valid until the next call."""
+
+    def __init__(self, fn, l1, l2):
+        self.l1 = l1.clone()
+        self.l2 = l2.clone()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2):
+                fn(self.l1, self.l2)
+        torch.cuda.current_stream().wait_stream(side)
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.o1, self.o2 = fn(self.l1, self.l2)
+        self.replays = 0
+
+    def __call__(self, l1, l2):
+        self.l1.copy_(l1)
+        self.l2.copy_(l2)
+        self.graph.replay()
+        self.replays += 1
+        return self.o1, self.o2
+
+
 def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=None, tol=None, verbose=0,
-               use_cpu_vvvv=True, precision="fp64"):
+               use_cpu_vvvv=True, precision="fp64", use_graph=None):
     _require_gpu()
     if eris is None:
         eris = mycc.ao2mo()
@@ -225,6 +256,22 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     imds = make_intermediates_gpu(mycc, t1, t2, eris, "fp64")
     diis_space = getattr(mycc, "diis_space", 6)
     use_diis = bool(getattr(mycc, "diis", True))
+    # the mixed-precision sweeps replay a captured HIP graph of the update
+    if use_graph is None:
+        use_graph = bool(getattr(mycc, "lambda_graph", True)) and precision != "fp64"
+    graphs = {}
+    history = []
+    mycc._lambda_history = history
+
+    def update(l1, l2, prec, homogeneous):
+        if not use_graph or prec == "fp64":
+            return update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec, homogeneous=homogeneous)
+        key = (prec, homogeneous)
+        if key not in graphs:
+            graphs[key] = GraphedUpdate(
+                lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec,
+                                               homogeneous=homogeneous), l1, l2)
+        return graphs[key](l1, l2)
 
     def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
         """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
@@ -233,12 +280,13 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
         best, best_it = float("inf"), 0
         for it in range(ncyc):
             if rhs is None:
-                n1, n2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec)
+                n1, n2 = update(l1, l2, prec, False)
             else:
-                n1, n2 = update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec, homogeneous=True)
+                n1, n2 = update(l1, l2, prec, True)
                 n1, n2 = n1 + rhs[0], n2 + rhs[1]
             vec = amplitudes_to_vector(n1, n2)
             normt = float(torch.linalg.norm(vec - amplitudes_to_vector(l1, l2)))
+            history.append((prec, rhs is not None, normt))
             if adiis is not None:
                 vec = adiis.update(vec)
             l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
@@ -261,8 +309,11 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
             mycc._lambda_normt = normt
             return conv, l1, l2
         # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
-        return _mixed_solve(mycc, fixed_point, update_lambda_gpu, t1, t2, l1, l2, eris, imds, use_cpu_vvvv,
-                            precision, tol, max_cycle)
+        out = _mixed_solve(mycc, fixed_point, update_lambda_gpu, t1, t2, l1, l2, eris, imds, use_cpu_vvvv,
+                           precision, tol, max_cycle)
+        mycc._lambda_graph_replays = sum(g.replays for g in graphs.values())
+        graphs.clear()
+        return out
 
 
 def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle):
@@ -275,6 +326,7 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
         fp64_updates += 1
         r1, r2 = r1 - l1, r2 - l2
         normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))
+        mycc._lambda_history.append(("fp64-residual", False, normt))
         if normt < tol:
             conv = True
             break

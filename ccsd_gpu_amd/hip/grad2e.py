@@ -16,10 +16,13 @@ import torch
 from .. import _lib
 
 
-def contract_int2e_ip1(mol, gamma_cart, scale=-2.0, b0=0, b1=None, de=None):
+def contract_int2e_ip1(mol, gamma_cart, scale=-2.0, b0=0, b1=None, de=None, dm=None, pm=None):
     """de[atom(i), x] += scale * sum_{i in [b0,b1), j, k>=l} ip1_x(ij|kl) gamma[i, j, kl].
 
-    Returns de as a (natm, 3) float64 tensor on the current device."""
+    With dm and pm (Cartesian nao_c x nao_c device matrices) the separable density
+    D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl)/4 (halved on
+    k == l) is added to gamma inside the kernel. Returns de as a (natm, 3) float64
+    tensor on the current device."""
     lib = _lib.load()
     b1 = mol.nbas if b1 is None else b1
     if not (0 <= b0 < b1 <= mol.nbas):
@@ -34,8 +37,17 @@ def contract_int2e_ip1(mol, gamma_cart, scale=-2.0, b0=0, b1=None, de=None):
     elif tuple(de.shape) != (mol.natm, 3) or de.dtype != torch.float64 or not de.is_contiguous():
         raise ValueError("de must be a contiguous float64 (natm, 3) tensor")
     plan = _lib.get_plan(mol)
-    _lib.check(lib.ccsd_grad_ip1_gamma(plan, _lib.ptr(de), _lib.ptr(gamma_cart), float(scale), b0, b1,
-                                       _lib.current_stream_ptr()), "ccsd_grad_ip1_gamma")
+    if dm is None:
+        _lib.check(lib.ccsd_grad_ip1_gamma(plan, _lib.ptr(de), _lib.ptr(gamma_cart), float(scale), b0, b1,
+                                           _lib.current_stream_ptr()), "ccsd_grad_ip1_gamma")
+        return de
+    pm = dm if pm is None else pm
+    for m in (dm, pm):
+        if tuple(m.shape) != (nao_c, nao_c) or m.dtype != torch.float64 or not m.is_contiguous() or not m.is_cuda:
+            raise ValueError(f"dm / pm must be contiguous float64 device tensors of shape {(nao_c, nao_c)}")
+    _lib.check(lib.ccsd_grad_ip1_gamma_sep(plan, _lib.ptr(de), _lib.ptr(gamma_cart), _lib.ptr(dm), _lib.ptr(pm),
+                                           float(scale), b0, b1, _lib.current_stream_ptr()),
+               "ccsd_grad_ip1_gamma_sep")
     return de
 
 

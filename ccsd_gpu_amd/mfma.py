@@ -60,6 +60,16 @@ def split(x):
     return Split(hi, lo)
 
 
+def sum_slices(ws, out, beta=0):
+    """out (+)= ws.sum(0) for a contiguous (S, ...) workspace and a contiguous out of
+    ws.shape[1:], by the library's fixed-order slice-sum kernel."""
+    if not (ws.is_contiguous() and out.is_contiguous()) or tuple(ws.shape[1:]) != tuple(out.shape):
+        raise ValueError("sum_slices needs contiguous ws (S, ...) and out of shape ws.shape[1:]")
+    _lib.check(_lib.load().ccsd_sum_slices(_lib.ptr(ws), _lib.ptr(out), out.numel(), ws.shape[0], int(beta),
+                                           _lib.current_stream_ptr()), "ccsd_sum_slices")
+    return out
+
+
 def k_splits(tiles, k, want=1024, min_chunk=1024):
     """Number of K chunks for a GEMM with `tiles` output tiles: small outputs with a long
     contracted dimension launch too few workgroups to fill 256 CUs, so K is cut into
@@ -110,10 +120,7 @@ def gemm_nt(A, B, out=None, beta=0, ksplit=None):
             if rest:
                 _launch_nt(lib, A, B, ws[full], b * m * k + full * kc, b * n * k + full * kc, m, n, rest,
                            k, k, 0, 0, 0, n, 1, 0)
-            if beta:
-                out[b] += ws.sum(0)
-            else:
-                torch.sum(ws, 0, out=out[b])
+            sum_slices(ws, out[b], beta)
         return out
     for b0 in range(0, nb, 65535):
         b1 = min(nb, b0 + 65535)

@@ -92,9 +92,13 @@ def _dgemm(Av, Bv):
         a, bb = Av[b], Bv[b]
         a3 = a.as_strided((s, m, kc), (kc * a.stride(1), a.stride(0), a.stride(1)))
         b3 = bb.as_strided((s, kc, n), (kc * bb.stride(0), bb.stride(0), bb.stride(1)))
-        torch.sum(torch.bmm(a3, b3), 0, out=out[b])
+        part = torch.bmm(a3, b3)
         if s * kc < k:
-            out[b] += a[:, s * kc:] @ bb[s * kc:]
+            part[0] += a[:, s * kc:] @ bb[s * kc:]
+        if out.is_cuda:
+            mfma.sum_slices(part, out[b])
+        else:
+            torch.sum(part, 0, out=out[b])
     return out
 
 

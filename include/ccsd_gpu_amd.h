@@ -102,6 +102,15 @@ int ccsd_plan_quartets(const ccsd_plan* plan, long long* count);
 int ccsd_grad_ip1_separable(ccsd_plan* plan, double* de, const double* dm, const double* pm, double scale,
                             void* stream);
 
+/* Both densities in one pass (single-molecule plans): Gamma' = gamma (explicit,
+ * packed as for ccsd_grad_ip1_gamma) + the separable form of dm, pm (nao_c x nao_c,
+ * Cartesian). This is the reference's whole two-electron gradient, the
+ * `-2 eri1 @ dm2buf` GEMV and the vhf1 J/K terms with dm1p (ccsd_gpu/gpu_ccsd_grad.py:
+ * 430-491), over i-shells [ish0, ish1) without forming eri1 or a four-index
+ * separable density. */
+int ccsd_grad_ip1_gamma_sep(ccsd_plan* plan, double* de, const double* gamma, const double* dm, const double* pm,
+                            double scale, int ish0, int ish1, void* stream);
+
 /* One-electron integrals in the Cartesian AO basis (row-major nao_c x nao_c
  * matrices, fully overwritten): kind 0 overlap, 1 kinetic, 2 nuclear attraction
  * (all nuclei), 3 <d/dr i|j> (3 matrices), 4 <d/dr i|T|j> (3), 5 <d/dr i|-Z_C/|r-C||j>
@@ -126,6 +135,19 @@ int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M
  * form of ccsd_gemm_bf16x3_nt. No counterpart in the reference (its
  * contractions are FP64 CuPy einsums). */
 int ccsd_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld, void* stream);
+
+/* out[i] = beta * out[i] + sum_{s < nslices} ws[s * n + i] (device arrays), in a
+ * fixed order: the reduction of a K-split GEMM's partial products. Launches one
+ * plain kernel (no memset), so it may sit inside a captured HIP graph. No
+ * counterpart in the reference (its einsums are single cuBLAS/cuTENSOR calls). */
+int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int beta, void* stream);
+
+/* In place on a dense (n, n, n, n) device tensor:
+ *   t[p][q][r][s] <- scale * (t[pqrs] + t[qprs] + t[pqsr] + t[qpsr]).
+ * The (ij)/(kl) symmetrisation of the two-particle density, applied in the MO
+ * basis before the AO transform (the reference symmetrises the AO tensor with
+ * two permuted copies, ccsd_gpu/gpu_ccsd_grad.py:324-326). */
+int ccsd_sym_pairs(double* t, int n, double scale, void* stream);
 
 /* C[b] (+)= A[b] B[b]^T from split-BF16 operands (A planes [batch][M][K] with row
  * stride lda, B planes [batch][N][K] with row stride ldb, both K-contiguous;

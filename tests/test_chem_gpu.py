@@ -228,3 +228,37 @@ def test_lambda_update_is_affine_with_homogeneous_part(gpu, h2o_pair):
     h1, h2 = update_lambda_gpu(mycc, mycc.t1, mycc.t2, L1, L2, eris, imds, homogeneous=True)
     assert float((f1 - z1 - h1).abs().max()) < 1e-12
     assert float((f2 - z2 - h2).abs().max()) < 1e-12
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_lambda_graph_replays_match_eager(gpu, h2o_pair, precision):
+    """The HIP-graph replay of the lambda update equals the eager update on every
+    replay, with eager work and synchronisations between the replays."""
+    from ccsd_gpu_amd.gpu_ccsd_lambda import GraphedUpdate, make_intermediates_gpu, update_lambda_gpu
+    from ccsd_gpu_amd.precision import static_operands
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    eris = mycc.ao2mo()
+    imds = make_intermediates_gpu(mycc, mycc.t1, mycc.t2, eris)
+    L1 = torch.as_tensor(l1, device="cuda")
+    L2 = torch.as_tensor(l2, device="cuda")
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    with static_operands(mycc.t1, mycc.t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo):
+        for homogeneous in (False, True):
+            def fn(a, b):
+                return update_lambda_gpu(mycc, mycc.t1, mycc.t2, a, b, eris, imds, True, precision,
+                                         homogeneous=homogeneous)
+            g = GraphedUpdate(fn, L1, L2)
+            for rep in range(4):
+                a = L1 * (1 + 0.1 * rep) + 1e-3 * torch.randn(L1.shape, dtype=torch.float64, device="cuda",
+                                                               generator=gen)
+                b = L2 * (1 - 0.05 * rep)
+                r1, r2 = fn(a, b)                       # eager
+                torch.cuda.synchronize()
+                g1, g2 = g(a, b)                        # replay
+                torch.cuda.synchronize()
+                e1 = float((g1 - r1).abs().max() / r1.abs().max())
+                e2 = float((g2 - r2).abs().max() / r2.abs().max())
+                print(precision, homogeneous, rep, e1, e2, flush=True)
+                assert e1 < 1e-12 and e2 < 1e-12

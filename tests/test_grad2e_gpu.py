@@ -116,3 +116,34 @@ def test_register_kernels_equal_lds_kernel(gpu, make):
     scale = np.abs(ref).max()
     print("reg vs lds", np.abs(got - ref).max() / scale, flush=True)
     assert np.abs(got - ref).max() <= 1e-11 * scale
+
+
+@pytest.mark.gpu
+def test_fused_explicit_plus_separable_in_one_pass(gpu):
+    """gamma + separable(D, P) in one kernel pass equals the explicit sum of both."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _separable_gamma
+    from ccsd_gpu_amd.hip.grad2e import contract_int2e_ip1, pack_s2kl
+    mol = molecules.h2o_ccpvdz(cart=False)
+    nc = mol.nao_cart()
+    gen = torch.Generator(device="cuda").manual_seed(21)
+    from ccsd_gpu_amd.autograd import symmetrize8
+    g = symmetrize8(torch.randn((nc,) * 4, dtype=torch.float64, device="cuda", generator=gen))
+    a = torch.randn((nc, nc), dtype=torch.float64, device="cuda", generator=gen)
+    b = torch.randn((nc, nc), dtype=torch.float64, device="cuda", generator=gen)
+    D, P = a + a.T, b + b.T
+    ref = contract_int2e_ip1(mol, pack_s2kl(g + _separable_gamma(D, P), halve_diag=True))
+    got = contract_int2e_ip1(mol, pack_s2kl(g, halve_diag=True), dm=D, pm=P)
+    err = float((got - ref).abs().max())
+    print("one-pass explicit+separable", err, flush=True)
+    assert err < 1e-10 * max(1.0, float(ref.abs().max()))
+
+
+@pytest.mark.gpu
+def test_sym_pairs_kernel_matches_torch(gpu):
+    from ccsd_gpu_amd.gpu_ccsd_grad import _lib_sym_pairs
+    gen = torch.Generator(device="cuda").manual_seed(22)
+    for n in (1, 5, 13):
+        t = torch.randn((n,) * 4, dtype=torch.float64, device="cuda", generator=gen)
+        ref = 0.5 * (t + t.permute(1, 0, 2, 3) + t.permute(0, 1, 3, 2) + t.permute(1, 0, 3, 2))
+        got = _lib_sym_pairs(t.clone(), 0.5)
+        assert float((got - ref).abs().max()) < 1e-14
