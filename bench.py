@@ -16,6 +16,10 @@ reported in config, with the max |difference| of the two gradients.
 --config eri: the fused int2e_ip1 x G backward pass of the ERI autograd
 function on (H2O)4 (one step = one pass), value = ERI' quartets/s.
 
+--config precision: the mixed-precision lambda on (H2O)6 (FP32 and split-BF16,
+with and without FP64-residual refinement); errors of l1, l2, the RDMs and the
+gradient against the FP64 lambda; value = split-BF16 lambda wall-s.
+
 --config batch: 64 seeded C/N/O/H molecules (3-12 heavy atoms, cc-pVDZ) in one
 plan; one step = one Schwarz-screened (1e-12) fused int2e_ip1 x Gamma pass over
 the whole batch with a separable density Gamma(D) per molecule; value = surviving
@@ -209,6 +213,65 @@ def bench_batch(args):
         "roofline": roof, "cpu_baseline": cpu}
 
 
+def bench_precision(args):
+    """Accuracy and speed of the mixed-precision lambda solve on (H2O)6 / cc-pVDZ
+    (nocc 30, nvir 114): l1, l2, the 1- and 2-RDMs and the gradient against the
+    FP64 lambda, for FP32 and split-BF16 MFMA contractions, with and without the
+    FP64-residual refinement."""
+    import torch
+    from ccsd_gpu_amd import pipeline, systems
+    from ccsd_gpu_amd.gpu_ccsd_grad import _gamma1_intermediates_gpu, _gamma2_intermediates_gpu, grad_elec_gpu
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    nw = args.waters or 6
+    mol = systems.water_cluster(nw, seed=1, cart=False)
+    mf, mycc = pipeline.setup(mol, log=log)
+    t1, t2 = mycc.t1, mycc.t2
+
+    def solve(prec, refine=True):
+        mycc.conv_tol_normt = 1e-8
+        mycc.lambda_refine = refine
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        l1, l2 = solve_lambda_gpu(mycc, t1, t2, precision=prec)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        mycc.lambda_refine = True
+        return l1.clone(), l2.clone(), dt, mycc._lambda_iterations, mycc._lambda_fp64_updates
+
+    def rdms(l1, l2, prec):
+        d1 = _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, prec)
+        d2 = _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, prec)
+        return [x for x in list(d1) + list(d2) if x is not None]
+
+    solve("fp64")                       # warm-up (static caches, library init)
+    l1r, l2r, t_ref, it_ref, _ = solve("fp64")
+    d_ref = rdms(l1r, l2r, "fp64")
+    g_ref = grad_elec_gpu(mycc, t1, t2, l1r, l2r, precision="fp64")
+    rows = {"fp64": {"lambda_s": t_ref, "iterations": it_ref}}
+    for prec in ("fp32", "bf16x3"):
+        for refine in (False, True):
+            solve(prec, refine)         # warm-up of this variant
+            l1, l2, dt, it, nf = solve(prec, refine)
+            d = rdms(l1, l2, prec)
+            g = grad_elec_gpu(mycc, t1, t2, l1, l2, precision=prec)
+            rows[prec + ("" if refine else "_no_refinement")] = {
+                "lambda_s": dt, "iterations": it, "fp64_updates": nf,
+                "max_err_l1": float((l1 - l1r).abs().max()), "max_err_l2": float((l2 - l2r).abs().max()),
+                "max_err_rdm1": max(float((a - b).abs().max()) for a, b in zip(d[:4], d_ref[:4])),
+                "max_err_rdm2": max(float((a - b).abs().max()) for a, b in zip(d[4:], d_ref[4:])),
+                "max_err_grad_Eh_per_Bohr": float(np.abs(g - g_ref).max())}
+            log(f"{prec} refine={refine}: {json.dumps(rows[prec + ('' if refine else '_no_refinement')])}")
+    best = rows["bf16x3"]
+    return {
+        "metric": METRIC, "value": best["lambda_s"], "unit": "s per lambda solve (split-BF16, refined to 1e-8)",
+        "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": best["lambda_s"] * 1e3, "higher_is_better": False,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64+bf16x3", "data": "synthetic",
+        "config": {"workload": f"CCSD lambda, (H2O){nw}/cc-pVDZ seeded, nocc {mycc.nocc}, nvir {mol.nao - mycc.nocc}, "
+                               "errors against the FP64 lambda",
+                   "speedup_bf16x3_vs_fp64": t_ref / best["lambda_s"], "results": rows},
+        "roofline": None, "cpu_baseline": None}
+
+
 def bench_grad(args):
     import torch
     from ccsd_gpu_amd import pipeline, systems
@@ -291,7 +354,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch"])
+    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision"])
     ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--waters", type=int, default=0)
@@ -306,7 +369,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     _lib.load()
-    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch}[args.config](args)
+    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

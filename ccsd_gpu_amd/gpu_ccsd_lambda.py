@@ -121,7 +121,23 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     imds.w1, imds.w2, imds.w3, imds.w4 = w1, w2, w3, w4
     imds.woooo, imds.wvooo = woooo, wvooo
     imds.wVOov, imds.wvOOv, imds.wvvov = wVOov, wvOOv, wvvov
+    _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0))
     return imds
+
+
+def _static_pieces(imds, t1, t2, eris, level_shift):
+    """Amplitude-only pieces of the lambda update, fixed during a solve (the reference
+    recomputes them in every update, gpu_ccsd_lambda.py:227-334)."""
+    nocc = t1.shape[0]
+    imds.theta = (t2 * 2 - t2.permute(0, 1, 3, 2)).contiguous()
+    imds.tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
+    imds.tmp_voov = (imds.wVOov * 2 + imds.wvOOv).contiguous()
+    imds.l2_const = (eris.ovvo.permute(0, 3, 1, 2) * 0.5).contiguous()     # voov.permute(1, 2, 0, 3) / 2
+    e_o = eris.mo_energy[:nocc]
+    e_v = eris.mo_energy[nocc:] + level_shift
+    eia = e_o[:, None] - e_v
+    imds.eia = eia
+    imds.inv_d2 = 1.0 / (eia[:, None, :, None] + eia[None, :, None, :])
 
 
 def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=True, precision="fp64",
@@ -142,7 +158,9 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     e_v = eris.mo_energy[nocc:] + getattr(mycc, "level_shift", 0.0)
     inh = 0.0 if homogeneous else 1.0
 
-    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
+    if not hasattr(imds, "theta"):
+        _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0))
+    theta = imds.theta
     mba = C("klca,klcb->ba", l2, theta)
     mij = C("ikcd,jkcd->ij", l2, theta)
     mba1 = C("jc,jb->bc", l1, t1) + mba
@@ -169,7 +187,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     l1new = l1new - C("iajk,kj->ia", ovoo, mij1) * 2 + C("jaik,kj->ia", ovoo, mij1)
     l2new = l2new - C("jbki,ka->jiba", ovoo, l1)
 
-    tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
+    tau = imds.tau
     l2tau = C("ijcd,klcd->ijkl", l2, tau)
     l2t1 = C("jidc,kc->ijkd", l2, t1)
 
@@ -184,7 +202,8 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
-    l2new = l2new + voov.permute(1, 2, 0, 3) * (0.5 * inh)
+    if inh:
+        l2new = l2new + imds.l2_const
     l2new = l2new - C("bjic,ca->jiba", voov, mba1) - C("bjka,ik->jiba", voov, mij1)
     l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
     m4 = C("ijkl,aklb->ijab", l2tau, voov)
@@ -192,7 +211,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     l1new = l1new + C("ijab,jb->ia", m4, t1) * 2 - C("ijba,jb->ia", m4, t1)
 
     l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
-    tmp_voov = imds.wVOov * 2 + imds.wvOOv
+    tmp_voov = imds.tmp_voov
     tmp = l2.permute(0, 2, 1, 3) - l2.permute(0, 3, 1, 2) * 0.5
     l2new = l2new + C("iakc,bjkc->jiba", tmp, tmp_voov)
     tmp = C("jkca,bikc->jiba", l2, imds.wvOOv)
@@ -202,10 +221,9 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     l2new = l2new + m3 * 0.5
     l1new = l1new + C("ijab,jb->ia", m3, t1) * 2 - C("ijba,jb->ia", m3, t1)
 
-    eia = e_o[:, None] - e_v
-    l1new = l1new / eia
+    l1new = l1new / imds.eia
     # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
-    l2new = (l2new + l2new.permute(1, 0, 3, 2)) / (eia[:, None, :, None] + eia[None, :, None, :])
+    l2new = (l2new + l2new.permute(1, 0, 3, 2)) * imds.inv_d2
     return l1new, l2new
 
 
@@ -301,7 +319,8 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
         return False, l1, l2, normt, ncyc
 
     with static_operands(t1, t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo, imds.w1,
-                         imds.w2, imds.w3, imds.w4, imds.wvvov, imds.wvooo, imds.wVOov, imds.wvOOv, imds.woooo):
+                         imds.w2, imds.w3, imds.w4, imds.wvvov, imds.wvooo, imds.wVOov, imds.wvOOv, imds.woooo,
+                         imds.theta, imds.tau, imds.tmp_voov):
         if precision == "fp64":
             conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
             mycc._lambda_iterations = iters
@@ -321,6 +340,12 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
     _, l1, l2, normt, iters = fixed_point(l1, l2, precision, lp_tol, max_cycle)
     total, fp64_updates = iters, 0
     conv = False
+    if not getattr(mycc, "lambda_refine", True):
+        # low-precision fixed point only (its noise floor; for accuracy studies)
+        mycc._lambda_iterations = total
+        mycc._lambda_fp64_updates = 0
+        mycc._lambda_normt = normt
+        return normt < tol, l1, l2
     for _round in range(8):
         r1, r2 = update(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
         fp64_updates += 1
