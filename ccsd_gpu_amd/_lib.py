@@ -43,6 +43,12 @@ _SIGNATURES = {
     "ccsd_grad_ip1_gamma_sep": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
                                         c_void_p]),
     "ccsd_sym_pairs": (c_int, [c_void_p, c_int, c_double, c_void_p]),
+    "ccsd_gemm_bf16x3_nt_scatter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                                            ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                            c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
+    "ccsd_split_bf16_strided": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    "ccsd_scatter_sum": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p,
+                                 c_double, c_int, c_void_p]),
     "ccsd_sum_slices": (c_int, [c_void_p, c_void_p, ctypes.c_longlong, c_int, c_int, c_void_p]),
 }
 EXPORTED = tuple(_SIGNATURES)

@@ -131,30 +131,41 @@ def update_amps(t1, t2, eris):
 
 
 class DIIS:
-    """Pulay DIIS on device vectors; error = difference of successive vectors."""
+    """Pulay DIIS on device vectors; error = difference of successive vectors.
+
+    Vectors and errors live in two (space, N) ring buffers on the device; each
+    update adds one row of the error overlap matrix (one GEMV, one host copy of
+    `space` numbers) and extrapolates with one GEMV over the stored vectors."""
 
     def __init__(self, space=6):
         self.space = space
-        self.xs, self.errs = [], []
+        self.X = self.E = None
+        self.B = np.zeros((space, space))
+        self.count = 0
         self.prev = None
 
     def update(self, x):
         if self.prev is None:
             self.prev = x
             return x
-        self.xs.append(x)
-        self.errs.append(x - self.prev)
-        self.xs, self.errs = self.xs[-self.space:], self.errs[-self.space:]
-        n = len(self.xs)
-        B = -torch.ones((n + 1, n + 1), dtype=torch.float64)
-        B[n, n] = 0
-        for i in range(n):
-            for j in range(i, n):
-                B[i, j] = B[j, i] = float(torch.dot(self.errs[i], self.errs[j]))
-        rhs = torch.zeros(n + 1, dtype=torch.float64)
-        rhs[n] = -1
-        c = torch.linalg.lstsq(B, rhs.unsqueeze(1)).solution[:n, 0]
-        out = sum(float(ci) * xi for ci, xi in zip(c, self.xs))
+        if self.X is None:
+            self.X = torch.empty((self.space, x.numel()), dtype=x.dtype, device=x.device)
+            self.E = torch.empty_like(self.X)
+        slot = self.count % self.space
+        self.count += 1
+        n = min(self.count, self.space)
+        self.X[slot].copy_(x)
+        torch.sub(x, self.prev, out=self.E[slot])
+        row = (self.E[:n] @ self.E[slot]).cpu().numpy()
+        self.B[slot, :n] = row
+        self.B[:n, slot] = row
+        A = -np.ones((n + 1, n + 1))
+        A[:n, :n] = self.B[:n, :n]
+        A[n, n] = 0.0
+        rhs = np.zeros(n + 1)
+        rhs[n] = -1.0
+        c = np.linalg.lstsq(A, rhs, rcond=None)[0][:n]
+        out = torch.as_tensor(c, dtype=x.dtype, device=x.device) @ self.X[:n]
         self.prev = out
         return out
 

@@ -26,7 +26,14 @@ int ccsd_fused_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_n
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int waves_per_block, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
-                                      long long sC, long long ldc, int batch, int beta, hipStream_t stream);
+                                      long long sC, long long ldc, int batch, int beta, double alpha,
+                                      const long long* off_b, const long long* off_r, const long long* off_c,
+                                      hipStream_t stream);
+hipError_t ccsd_launch_split_bf16_strided(const double* x, void* hi, void* lo, int ndim, const long long* sizes,
+                                          const long long* strides, hipStream_t stream);
+hipError_t ccsd_launch_scatter_sum(const double* ws, int S, int nb, int M, int N, double* C, const long long* off_b,
+                                   const long long* off_r, const long long* off_c, double alpha, int beta,
+                                   hipStream_t stream);
 hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
                                   hipStream_t stream);
 hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
@@ -1002,8 +1009,47 @@ int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const
         return fail(CCSD_ERR_SHAPE, "bad GEMM arguments");
     if ((N + 127) / 128 > 65535 || batch > 65535) return fail(CCSD_ERR_SHAPE, "N or batch too large for one launch");
     hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, strideA, strideB, strideC, ldc,
-                                              batch, beta, (hipStream_t)stream);
+                                              batch, beta, 1.0, nullptr, nullptr, nullptr, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_gemm_bf16x3_nt_scatter(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C, int M,
+                                int N, int K, long long lda, long long ldb, long long strideA, long long strideB,
+                                const long long* off_b, const long long* off_r, const long long* off_c, int batch,
+                                double alpha, int beta, void* stream)
+{
+    if (!Ahi || !Alo || !Bhi || !Blo || !C || !off_b || !off_r || !off_c) return fail(CCSD_ERR_NULL, "null pointer");
+    if (M < 0 || N < 0 || K < 0 || batch < 0 || lda < K || ldb < K || (beta != 0 && beta != 1))
+        return fail(CCSD_ERR_SHAPE, "bad GEMM arguments");
+    if ((N + 127) / 128 > 65535 || batch > 65535) return fail(CCSD_ERR_SHAPE, "N or batch too large for one launch");
+    hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, strideA, strideB, 0, 0, batch,
+                                              beta, alpha, off_b, off_r, off_c, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_split_bf16_strided(const double* x, void* hi, void* lo, int ndim, const long long* sizes,
+                            const long long* strides, void* stream)
+{
+    if (!x || !hi || !lo || !sizes || !strides) return fail(CCSD_ERR_NULL, "null pointer");
+    if (ndim < 1 || ndim > 8) return fail(CCSD_ERR_SHAPE, "ndim must be 1..8");
+    for (int d = 0; d < ndim; d++)
+        if (sizes[d] < 0 || strides[d] < 0) return fail(CCSD_ERR_SHAPE, "negative size or stride");
+    hipError_t e = ccsd_launch_split_bf16_strided(x, hi, lo, ndim, sizes, strides, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "split_bf16_strided_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_scatter_sum(const double* ws, int nslices, int batch, int M, int N, double* C, const long long* off_b,
+                     const long long* off_r, const long long* off_c, double alpha, int beta, void* stream)
+{
+    if (!ws || !C || !off_b || !off_r || !off_c) return fail(CCSD_ERR_NULL, "null pointer");
+    if (nslices < 0 || batch < 0 || M < 0 || N < 0 || (beta != 0 && beta != 1))
+        return fail(CCSD_ERR_SHAPE, "bad scatter-sum arguments");
+    hipError_t e = ccsd_launch_scatter_sum(ws, nslices, batch, M, N, C, off_b, off_r, off_c, alpha, beta,
+                                           (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "scatter_sum_kernel");
     return CCSD_OK;
 }
 

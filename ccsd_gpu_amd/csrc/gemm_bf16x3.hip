@@ -37,7 +37,8 @@ constexpr int TILE = BM * LDK;              // bf16 elements of one operand plan
 __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
     const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long lda, long long ldb,
-    long long sA, long long sB, long long sC, long long ldc, int beta)
+    long long sA, long long sB, long long sC, long long ldc, int beta, double alpha, const long long* __restrict__ off_b,
+    const long long* __restrict__ off_r, const long long* __restrict__ off_c)
 {
     __shared__ __attribute__((aligned(16))) __bf16 lds[4 * TILE];   // Ahi, Alo, Bhi, Blo
     const int tid = threadIdx.x;
@@ -127,19 +128,23 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
                 }
         }
     }
-    // lane holds C[8*(e/4) + 4*half + e%4][l32] of each 32x32 tile
-    double* Cb = C + bz * sC;
+    // lane holds C[8*(e/4) + 4*half + e%4][l32] of each 32x32 tile; the element (row, col)
+    // of batch b goes to C[b*sC + row*ldc + col], or to C[off_b[b] + off_r[row] + off_c[col]]
+    // when offset tables are given (a permuted destination, e.g. a term of a 4-index tensor)
+    double* Cb = off_r ? C + off_b[bz] : C + bz * sC;
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int gc = n0 + wn * 64 + j * 32 + l32;
+            const long long oc = (off_r && gc < N) ? off_c[gc] : (long long)gc;
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
                 if (gr < M && gc < N) {
-                    double* p = Cb + (long long)gr * ldc + gc;
-                    *p = beta ? *p + (double)acc[i][j][e] : (double)acc[i][j][e];
+                    double* p = Cb + (off_r ? off_r[gr] : (long long)gr * ldc) + oc;
+                    const double v = alpha * (double)acc[i][j][e];
+                    *p = beta ? *p + v : v;
                 }
             }
         }
@@ -156,6 +161,50 @@ __global__ void split_bf16_kernel(const double* __restrict__ x, uint16_t* __rest
         const uint16_t h = f2bf(f);
         hi[t] = h;
         lo[t] = f2bf(f - bf2f(h));
+    }
+}
+
+// x given as an N-d strided view (sizes, element strides, ndim <= 8) -> hi, lo planes
+// contiguous in the view's row-major order: the permutation of a GEMM operand and its
+// split in one pass.
+struct StridedView {
+    int ndim;
+    long long size[8];
+    long long stride[8];
+};
+
+__global__ void split_bf16_strided_kernel(const double* __restrict__ x, uint16_t* __restrict__ hi,
+                                          uint16_t* __restrict__ lo, long long n, const StridedView v)
+{
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        long long rem = t, off = 0;
+        for (int d = v.ndim - 1; d >= 0; d--) {
+            const long long q = rem / v.size[d];
+            off += (rem - q * v.size[d]) * v.stride[d];
+            rem = q;
+        }
+        const float f = (float)x[off];
+        const uint16_t h = f2bf(f);
+        hi[t] = h;
+        lo[t] = f2bf(f - bf2f(h));
+    }
+}
+
+// C[off_b[b] + off_r[r] + off_c[c]] = beta * C[...] + alpha * sum_s ws[s][b][r][c]: the
+// reduction of a K-split GEMM into a permuted destination (each element written once).
+__global__ void scatter_sum_kernel(const double* __restrict__ ws, int S, int nb, int M, int N, double* __restrict__ C,
+                                   const long long* __restrict__ off_b, const long long* __restrict__ off_r,
+                                   const long long* __restrict__ off_c, double alpha, int beta)
+{
+    const long long mn = (long long)M * N, total = mn * nb;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+         t += (long long)gridDim.x * blockDim.x) {
+        const long long b = t / mn, rc = t - b * mn;
+        const long long r = rc / N, c = rc - r * N;
+        double s = 0.0;
+        for (int q = 0; q < S; q++) s += ws[(long long)q * total + t];
+        double* p = C + off_b[b] + off_r[r] + off_c[c];
+        *p = beta ? *p + alpha * s : alpha * s;
     }
 }
 
@@ -182,14 +231,50 @@ hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, in
     return hipGetLastError();
 }
 
+hipError_t ccsd_launch_split_bf16_strided(const double* x, void* hi, void* lo, int ndim, const long long* sizes,
+                                          const long long* strides, hipStream_t stream)
+{
+    if (ndim < 1 || ndim > 8) return hipErrorInvalidValue;
+    StridedView v;
+    v.ndim = ndim;
+    long long n = 1;
+    for (int d = 0; d < 8; d++) {
+        v.size[d] = d < ndim ? sizes[d] : 1;
+        v.stride[d] = d < ndim ? strides[d] : 0;
+        if (d < ndim) n *= sizes[d];
+    }
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(split_bf16_strided_kernel, dim3((unsigned)nb), dim3(256), 0, stream, x, (uint16_t*)hi,
+                       (uint16_t*)lo, n, v);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_scatter_sum(const double* ws, int S, int nb, int M, int N, double* C, const long long* off_b,
+                                   const long long* off_r, const long long* off_c, double alpha, int beta,
+                                   hipStream_t stream)
+{
+    const long long total = (long long)nb * M * N;
+    if (total <= 0) return hipSuccess;
+    long long blocks = (total + 255) / 256;
+    if (blocks > 65536LL * 8) blocks = 65536LL * 8;
+    hipLaunchKernelGGL(scatter_sum_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, ws, S, nb, M, N, C, off_b,
+                       off_r, off_c, alpha, beta);
+    return hipGetLastError();
+}
+
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
-                                      long long sC, long long ldc, int batch, int beta, hipStream_t stream)
+                                      long long sC, long long ldc, int batch, int beta, double alpha,
+                                      const long long* off_b, const long long* off_r, const long long* off_c,
+                                      hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
     dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
     hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, grid, dim3(256), 0, stream, (const __bf16*)Ahi, (const __bf16*)Alo,
-                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta);
+                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta, alpha,
+                       off_b, off_r, off_c);
     return hipGetLastError();
 }
 

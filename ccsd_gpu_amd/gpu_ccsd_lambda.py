@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
-from .precision import contract, static_operands, vvvv_term
+from .precision import contract, contract_acc, static_operands, vvvv_term
 from .scf import DEV
 
 
@@ -149,6 +149,10 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     mia1) are left out, giving A l -- used by the mixed-precision solver for
     its correction equations."""
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+
+    def acc(out, eq, a, b, alpha=1.0):
+        return contract_acc(out, eq, a, b, alpha=alpha, precision=precision)
+
     if imds is None:
         imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
     t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
@@ -173,19 +177,20 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     else:
         l2new = torch.zeros_like(l2)
     l1new = C("ijab,jb->ia", l2new, t1) * 2 - C("jiab,jb->ia", l2new, t1)
-    l2new = l2new * 0.5
+    l2new.mul_(0.5)
 
     w1 = imds.w1 - torch.diag(e_v)
     w2 = imds.w2 - torch.diag(e_o)
     l1new = l1new + inh * fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
     l1new = l1new + C("ijab,bj->ia", l2, imds.w3) * 2 - C("ijba,bj->ia", l2, imds.w3)
-    l2new = l2new + torch.einsum("ia,jb->ijab", l1, imds.w4)
-    l2new = l2new + C("jibc,ca->jiba", l2, w1) - C("jk,kiba->jiba", w2, l2)
+    l2new.add_(torch.einsum("ia,jb->ijab", l1, imds.w4))
+    acc(l2new, "jibc,ca->jiba", l2, w1)
+    acc(l2new, "jk,kiba->jiba", w2, l2, -1.0)
 
     ovoo = eris.ovoo
     l1new = l1new - C("iajk,kj->ia", ovoo, mij1) * 2 + C("jaik,kj->ia", ovoo, mij1)
-    l2new = l2new - C("jbki,ka->jiba", ovoo, l1)
+    acc(l2new, "jbki,ka->jiba", ovoo, l1, -1.0)
 
     tau = imds.tau
     l2tau = C("ijcd,klcd->ijkl", l2, tau)
@@ -194,31 +199,33 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     l1new = l1new - C("jb,jiab->ia", l1, eris.oovv)
     ovvv = eris.ovvv
     l1new = l1new + C("iabc,bc->ia", ovvv, mba1) * 2 - C("ibca,bc->ia", ovvv, mba1)
-    l2new = l2new + C("jbac,ic->jiba", ovvv, l1)
+    acc(l2new, "jbac,ic->jiba", ovvv, l1)
     m4 = C("ijkd,kadb->ijab", l2t1, ovvv)
-    l2new = l2new - m4
+    l2new.sub_(m4)
     l1new = l1new - C("ijab,jb->ia", m4, t1) * 2 - C("ijab,ia->jb", m4, t1) * 2
     l1new = l1new + C("jiab,jb->ia", m4, t1) + C("jiab,ia->jb", m4, t1)
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
     if inh:
-        l2new = l2new + imds.l2_const
-    l2new = l2new - C("bjic,ca->jiba", voov, mba1) - C("bjka,ik->jiba", voov, mij1)
+        l2new.add_(imds.l2_const)
+    acc(l2new, "bjic,ca->jiba", voov, mba1, -1.0)
+    acc(l2new, "bjka,ik->jiba", voov, mij1, -1.0)
     l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
     m4 = C("ijkl,aklb->ijab", l2tau, voov)
-    l2new = l2new + m4 * 0.5
+    l2new.add_(m4, alpha=0.5)
     l1new = l1new + C("ijab,jb->ia", m4, t1) * 2 - C("ijba,jb->ia", m4, t1)
 
     l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
     tmp_voov = imds.tmp_voov
     tmp = l2.permute(0, 2, 1, 3) - l2.permute(0, 3, 1, 2) * 0.5
-    l2new = l2new + C("iakc,bjkc->jiba", tmp, tmp_voov)
+    acc(l2new, "iakc,bjkc->jiba", tmp, tmp_voov)
     tmp = C("jkca,bikc->jiba", l2, imds.wvOOv)
-    l2new = l2new + tmp + tmp.permute(1, 0, 2, 3) * 0.5
+    l2new.add_(tmp)
+    l2new.add_(tmp.permute(1, 0, 2, 3), alpha=0.5)
 
     m3 = C("ijkl,klab->ijab", imds.woooo, l2)
-    l2new = l2new + m3 * 0.5
+    l2new.add_(m3, alpha=0.5)
     l1new = l1new + C("ijab,jb->ia", m3, t1) * 2 - C("ijba,jb->ia", m3, t1)
 
     l1new = l1new / imds.eia
@@ -242,20 +249,35 @@ class GraphedUpdate:
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            for _ in range(2):
-                fn(self.l1, self.l2)
+            # the warm-up call is the first update itself: F(l1, l2)
+            w1, w2 = fn(self.l1, self.l2)
+            self.first = (w1.clone(), w2.clone())
         torch.cuda.current_stream().wait_stream(side)
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self.o1, self.o2 = fn(self.l1, self.l2)
         self.replays = 0
+        self._first_key = None
+
+    @staticmethod
+    def _key(l1, l2):
+        return (l1.data_ptr(), l1._version, l2.data_ptr(), l2._version)
 
     def __call__(self, l1, l2):
+        if self.first is not None and self._first_key == self._key(l1, l2):
+            out, self.first = self.first, None
+            return out
+        self.first = None
         self.l1.copy_(l1)
         self.l2.copy_(l2)
         self.graph.replay()
         self.replays += 1
         return self.o1, self.o2
+
+    def bind_first(self, l1, l2):
+        """The next call with exactly these (unmodified) tensors returns the warm-up result."""
+        self._first_key = self._key(l1, l2)
+        return self
 
 
 def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=None, tol=None, verbose=0,
@@ -288,25 +310,29 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
         if key not in graphs:
             graphs[key] = GraphedUpdate(
                 lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec,
-                                               homogeneous=homogeneous), l1, l2)
+                                               homogeneous=homogeneous), l1, l2).bind_first(l1, l2)
         return graphs[key](l1, l2)
 
     def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
         """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
         adiis = DIIS(diis_space) if use_diis else None
+        rhs_vec = amplitudes_to_vector(*rhs) if rhs is not None else None
         normt = None
         best, best_it = float("inf"), 0
+        prev = amplitudes_to_vector(l1, l2)
         for it in range(ncyc):
             if rhs is None:
                 n1, n2 = update(l1, l2, prec, False)
             else:
                 n1, n2 = update(l1, l2, prec, True)
-                n1, n2 = n1 + rhs[0], n2 + rhs[1]
             vec = amplitudes_to_vector(n1, n2)
-            normt = float(torch.linalg.norm(vec - amplitudes_to_vector(l1, l2)))
+            if rhs is not None:
+                vec += rhs_vec
+            normt = float(torch.linalg.norm(vec - prev))
             history.append((prec, rhs is not None, normt))
             if adiis is not None:
                 vec = adiis.update(vec)
+            prev = vec
             l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
             if normt < tol_:
                 return True, l1, l2, normt, it + 1

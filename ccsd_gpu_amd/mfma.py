@@ -1,4 +1,6 @@
-"""Batched GEMM on the MFMA units (HIP kernel gemm_mixed_kernel) for float64 tensors."""
+"""Batched GEMM on the MFMA units (HIP kernels) for float64 tensors."""
+import ctypes
+
 import torch
 
 from . import _lib
@@ -87,6 +89,63 @@ def _launch_nt(lib, A, B, C, a_off, b_off, m, n, kc, lda, ldb, sA, sB, sC, ldc, 
         ptr(B.hi.data_ptr() + b_off * e), ptr(B.lo.data_ptr() + b_off * e),
         ptr(C.data_ptr()), m, n, kc, lda, ldb, sA, sB, sC, ldc, batch, int(beta),
         _lib.current_stream_ptr()), "ccsd_gemm_bf16x3_nt")
+
+
+def split_view(x, shape):
+    """Split a float64 strided view (ndim <= 8, any strides, e.g. a permuted tensor)
+    into bf16 planes laid out in the view's row-major order, reshaped to `shape`
+    (nb, rows, cols): the operand's permutation and split in one pass."""
+    if x.dtype != torch.float64 or x.dim() > 8:
+        raise ValueError("split_view needs a float64 view of at most 8 dimensions")
+    n = x.numel()
+    hi = torch.empty(shape, dtype=torch.bfloat16, device=x.device)
+    lo = torch.empty_like(hi)
+    if hi.numel() != n:
+        raise ValueError("shape does not match the view's size")
+    if n:
+        nd = max(1, x.dim())
+        sizes = (ctypes.c_longlong * nd)(*(list(x.shape) or [1]))
+        strides = (ctypes.c_longlong * nd)(*(list(x.stride()) or [1]))
+        _lib.check(_lib.load().ccsd_split_bf16_strided(_lib.ptr(x), _lib.ptr(hi), _lib.ptr(lo), nd, sizes, strides,
+                                                        _lib.current_stream_ptr()), "ccsd_split_bf16_strided")
+    return Split(hi, lo)
+
+
+def gemm_nt_scatter(A, B, C, off_b, off_r, off_c, alpha=1.0, beta=1, ksplit=None):
+    """C[off_b[b] + off_r[r] + off_c[c]] (+)= alpha * (A[b] @ B[b]^T)[r, c] with A: Split
+    (nb, m, k), B: Split (nb, n, k), C any float64 device tensor whose storage the
+    int64 offset tables address (relative to C.data_ptr())."""
+    nb, m, k = A.shape
+    nb2, n, k2 = B.shape
+    if nb != nb2 or k != k2:
+        raise ValueError("gemm_nt_scatter shape mismatch")
+    if m == 0 or n == 0 or nb == 0:
+        return C
+    lib = _lib.load()
+    ptr = _lib.c_void_p
+    e = A.hi.element_size()
+    if ksplit is None:
+        ksplit = k_splits(nb * -(-m // 128) * -(-n // 128), k) if nb <= 8 else 1
+    if ksplit > 1 and k > 0:
+        kc = -(-(-(-k // ksplit)) // 32) * 32
+        full, rest = divmod(k, kc)
+        S = full + (rest > 0)
+        ws = torch.empty((S, nb, m, n), dtype=torch.float64, device=C.device)
+        for b in range(nb):
+            if full:
+                _launch_nt(lib, A, B, ws[0, b], b * m * k, b * n * k, m, n, kc, k, k, kc, kc, nb * m * n, n, full, 0)
+            if rest:
+                _launch_nt(lib, A, B, ws[full, b], b * m * k + full * kc, b * n * k + full * kc, m, n, rest,
+                           k, k, 0, 0, 0, n, 1, 0)
+        _lib.check(lib.ccsd_scatter_sum(_lib.ptr(ws), S, nb, m, n, _lib.ptr(C), _lib.ptr(off_b), _lib.ptr(off_r),
+                                        _lib.ptr(off_c), float(alpha), int(beta), _lib.current_stream_ptr()),
+                   "ccsd_scatter_sum")
+        return C
+    _lib.check(lib.ccsd_gemm_bf16x3_nt_scatter(
+        ptr(A.hi.data_ptr()), ptr(A.lo.data_ptr()), ptr(B.hi.data_ptr()), ptr(B.lo.data_ptr()), _lib.ptr(C),
+        m, n, k, k, k, m * k, n * k, _lib.ptr(off_b), _lib.ptr(off_r), _lib.ptr(off_c), nb, float(alpha), int(beta),
+        _lib.current_stream_ptr()), "ccsd_gemm_bf16x3_nt_scatter")
+    return C
 
 
 def gemm_nt(A, B, out=None, beta=0, ksplit=None):

@@ -102,7 +102,33 @@ def _dgemm(Av, Bv):
     return out
 
 
-def _gemm_contract(eq, A, B, precision):
+def _split_operand(X, s, g1, g2, g3, size, tag):
+    """bf16 planes of X (index string s) as an (n1, n2, n3) matrix batch in group order
+    g1 g2 g3, permuted and split in one pass; cached for static operands."""
+    n1, n2, n3 = prod(size[c] for c in g1), prod(size[c] for c in g2), prod(size[c] for c in g3)
+    perm = [s.index(c) for c in g1 + g2 + g3]
+    return _cached(X, (s, tag, tuple(g1), tuple(g2), tuple(g3)),
+                   lambda: mfma.split_view(X.permute(perm), (n1, n2, n3)))
+
+
+_OFFSETS = {}
+
+
+def _offsets(out, out_idx, group, size):
+    """int64 device offsets (elements) of the row-major enumeration of `group`'s indices
+    inside `out` (indices labelled by out_idx)."""
+    strides = tuple(out.stride(out_idx.index(c)) for c in group)
+    key = (tuple(group), tuple(size[c] for c in group), strides, out.device)
+    v = _OFFSETS.get(key)
+    if v is None:
+        v = torch.zeros(1, dtype=torch.int64, device=out.device)
+        for c, st in zip(group, strides):
+            v = (v[:, None] + torch.arange(size[c], dtype=torch.int64, device=out.device) * st).reshape(-1)
+        _OFFSETS[key] = v
+    return v
+
+
+def _prepare(eq, A, B):
     lhs, out = eq.split("->")
     a, b = lhs.split(",")
     # indices that appear in one operand only and not in the output are summed first
@@ -128,11 +154,14 @@ def _gemm_contract(eq, A, B, precision):
     nb, m, k, n = prod(size[c] for c in batch), prod(size[c] for c in left), \
         prod(size[c] for c in contr), prod(size[c] for c in right)
     flops = 2.0 * nb * m * n * k
+    return A, B, a, b, out, size, batch, left, contr, right, flops
+
+
+def _gemm_contract(eq, A, B, precision):
+    A, B, a, b, out, size, batch, left, contr, right, flops = _prepare(eq, A, B)
     if precision == "bf16x3" and flops >= MIXED_MIN_FLOPS:
-        As = _cached(A, (a, "nt-a", tuple(batch), tuple(left), tuple(contr)),
-                     lambda: mfma.split(_view(A, a, batch, left, contr, size)[0]))
-        Bs = _cached(B, (b, "nt-b", tuple(batch), tuple(right), tuple(contr)),
-                     lambda: mfma.split(_view(B, b, batch, right, contr, size)[0]))
+        As = _split_operand(A, a, batch, left, contr, size, "nt-a")
+        Bs = _split_operand(B, b, batch, right, contr, size, "nt-b")
         Cp = mfma.gemm_nt(As, Bs)
     else:
         Av = _cached(A, (a, "mk", tuple(batch), tuple(left), tuple(contr)),
@@ -146,6 +175,24 @@ def _gemm_contract(eq, A, B, precision):
     C = Cp.reshape([size[c] for c in batch + left + right])
     order = batch + left + right
     return C.permute([order.index(c) for c in out])
+
+
+def contract_acc(out, eq, A, B, alpha=1.0, precision="fp64"):
+    """out += alpha * einsum(eq, A, B), in place; eq's output labels out's indices.
+
+    On the split-BF16 path the GEMM epilogue adds the term straight into `out`
+    through offset tables (no product tensor, no permuted copy); otherwise the
+    product is formed and added."""
+    lhs, oidx = eq.replace(" ", "").split("->")
+    if precision == "bf16x3" and out.is_cuda:
+        A2, B2, a, b, o, size, batch, left, contr, right, flops = _prepare(f"{lhs}->{oidx}", A, B)
+        if flops >= MIXED_MIN_FLOPS:
+            As = _split_operand(A2, a, batch, left, contr, size, "nt-a")
+            Bs = _split_operand(B2, b, batch, right, contr, size, "nt-b")
+            mfma.gemm_nt_scatter(As, Bs, out, _offsets(out, oidx, batch, size), _offsets(out, oidx, left, size),
+                                 _offsets(out, oidx, right, size), alpha=alpha, beta=1)
+            return out
+    return out.add_(contract(eq, A, B, precision=precision), alpha=alpha)
 
 
 def contract(eq, *ops, precision="fp64"):

@@ -163,6 +163,29 @@ int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const
 /* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
  * arrays; r, w are (count, n)). Counterpart of rys_roots_simple
  * (ccsd_gpu/cuda/int2e_ip1_module.py:192-244), n = 1..9. */
+/* ccsd_gemm_bf16x3_nt with a scattered destination: element (row, col) of batch b
+ * is written to C[off_b[b] + off_r[row] + off_c[col]] (offset tables in device
+ * memory, elements), as C = alpha * AB^T (beta 0) or C += alpha * AB^T (beta 1).
+ * Accumulates a contraction term straight into a four-index tensor whose index
+ * order differs from the GEMM's (the lambda update's `l2new += term.permute(...)`,
+ * ccsd_gpu/gpu_ccsd_lambda.py:248-323), without the permuted copy. */
+int ccsd_gemm_bf16x3_nt_scatter(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C, int M,
+                                int N, int K, long long lda, long long ldb, long long strideA, long long strideB,
+                                const long long* off_b, const long long* off_r, const long long* off_c, int batch,
+                                double alpha, int beta, void* stream);
+
+/* ccsd_split_bf16 from an N-d strided view (ndim <= 8; sizes, element strides as
+ * host arrays): the planes are contiguous in the view's row-major order, so an
+ * operand is permuted and split in one pass. */
+int ccsd_split_bf16_strided(const double* x, void* hi, void* lo, int ndim, const long long* sizes,
+                            const long long* strides, void* stream);
+
+/* C[off_b[b] + off_r[r] + off_c[c]] = beta * C[...] + alpha * sum_s ws[s][b][r][c]
+ * (ws: (nslices, batch, M, N) contiguous): the K-split reduction into a scattered
+ * destination. */
+int ccsd_scatter_sum(const double* ws, int nslices, int batch, int M, int N, double* C, const long long* off_b,
+                     const long long* off_r, const long long* off_c, double alpha, int beta, void* stream);
+
 int ccsd_rys_roots(int n, const double* T, int count, double* r, double* w, void* stream);
 
 #ifdef __cplusplus

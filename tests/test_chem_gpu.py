@@ -262,3 +262,38 @@ def test_lambda_graph_replays_match_eager(gpu, h2o_pair, precision):
                 e2 = float((g2 - r2).abs().max() / r2.abs().max())
                 print(precision, homogeneous, rep, e1, e2, flush=True)
                 assert e1 < 1e-12 and e2 < 1e-12
+
+
+@pytest.mark.gpu
+def test_split_view_equals_split_of_permuted_copy(gpu):
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(13)
+    x = torch.randn((5, 7, 3, 9), dtype=torch.float64, device="cuda", generator=gen)
+    for perm in [(0, 1, 2, 3), (2, 0, 3, 1), (3, 2, 1, 0)]:
+        v = x.permute(perm)
+        shape = (1, v.shape[0] * v.shape[1], v.shape[2] * v.shape[3])
+        a = mfma.split_view(v, shape)
+        b = mfma.split(v.contiguous().reshape(shape))
+        assert torch.equal(a.hi, b.hi) and torch.equal(a.lo, b.lo)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("eq,shapes", [
+    ("iakc,bjkc->jiba", [(6, 9, 7, 8), (9, 6, 7, 8)]),
+    ("jibc,ca->jiba", [(6, 5, 9, 11), (11, 9)]),
+    ("bjic,ca->jiba", [(9, 6, 5, 11), (11, 9)]),
+    ("klca,klcb->ba", [(40, 40, 30, 9), (40, 40, 30, 8)]),        # long K: split-K into the scatter
+])
+def test_contract_acc_scatter_matches_einsum(gpu, eq, shapes, monkeypatch):
+    from ccsd_gpu_amd import precision
+    monkeypatch.setattr(precision, "MIXED_MIN_FLOPS", 0.0)
+    gen = torch.Generator(device="cuda").manual_seed(len(eq))
+    A = torch.randn(shapes[0], dtype=torch.float64, device="cuda", generator=gen)
+    B = torch.randn(shapes[1], dtype=torch.float64, device="cuda", generator=gen)
+    ref = torch.einsum(eq, A, B)
+    out0 = torch.randn(ref.shape, dtype=torch.float64, device="cuda", generator=gen)
+    got = precision.contract_acc(out0.clone(), eq, A, B, alpha=-0.5, precision="bf16x3")
+    want = out0 - 0.5 * ref
+    err = float((got - want).abs().max() / ref.abs().max())
+    print(eq, err, flush=True)
+    assert err < 2e-4
