@@ -39,6 +39,7 @@ _SIGNATURES = {
     "ccsd_plan_screen": (c_int, [c_void_p, c_double, c_void_p]),
     "ccsd_schwarz": (c_int, [c_void_p, c_void_p, c_void_p]),
     "ccsd_plan_quartets": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_longlong)]),
+    "ccsd_plan_last_launches": (c_int, [c_void_p, c_void_p, c_int, ctypes.POINTER(c_int)]),
     "ccsd_grad_ip1_separable": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p]),
     "ccsd_grad_ip1_gamma_sep": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
                                         c_void_p]),

@@ -6,6 +6,7 @@
 // pairs (ish in [ish0, ish1), all jsh) and ket pairs (ksh >= lsh) by angular
 // momentum class and launches one quartet kernel per (bra class, ket class).
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <string>
@@ -23,6 +24,7 @@ hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double*
 size_t ccsd_c2s_term_size();
 hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
 int ccsd_fused_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_nckl);
+size_t ccsd_fused_reg_lds(int max_nij, int wpb);
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int waves_per_block, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
@@ -142,6 +144,7 @@ struct ccsd_plan {
     bool q_valid = false;
     std::vector<double> q;                           // nbas x nbas, symmetric
     long long last_quartets = 0;
+    std::vector<std::array<long long, 10>> last_launches;   // see ccsd_plan_last_launches
     double* d_de_scratch = nullptr;                  // autotuner's throw-away gradient
     std::vector<C2sTermHost> c2s;                    // per spherical AO
     void* d_c2s = nullptr;
@@ -419,6 +422,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
     const bool reg = p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
     if (reg) {
         for (int w : {4, 2, 1}) {
+            if (ccsd_fused_reg_lds(a.max_nij, w) > 65536) continue;
             Choice c;
             c.kind = 1; c.wpb = w; c.chunk = 64;
             cand.push_back(c);
@@ -542,9 +546,18 @@ int ensure_screening(ccsd_plan* p, hipStream_t stream)
     return build_ket(p);
 }
 
+void record_launch(ccsd_plan* p, const EriClassArgs& a, const Choice& c, const ChunkList* cl)
+{
+    const int gc = (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0;
+    const int parts = c.kind == 1 ? ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) : 0;
+    p->last_launches.push_back({a.li, a.lj, a.lk, a.ll, gc, c.kind, c.kind == 1 ? c.wpb : 1, c.chunk, parts,
+                                cl->quartets});
+}
+
 int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long long comp_stride, hipStream_t stream)
 {
     p->last_quartets = 0;
+    p->last_launches.clear();
     for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], deriv);
@@ -561,6 +574,7 @@ int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long l
             rc = launch_choice(a, c, stream);
             if (rc) return rc;
             p->last_quartets += cl->quartets;
+            record_launch(p, a, c, cl);
         }
     }
     return CCSD_OK;
@@ -573,6 +587,7 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
 {
     const bool screened = p->thr > 0.0;
     p->last_quartets = 0;
+    p->last_launches.clear();
     for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
@@ -593,6 +608,7 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             rc = launch_choice(a, c, stream);
             if (rc) return rc;
             p->last_quartets += cl->quartets;
+            record_launch(p, a, c, cl);
         }
     }
     return CCSD_OK;
@@ -932,6 +948,16 @@ int ccsd_schwarz(ccsd_plan* p, double* q, void* stream)
     }
     HIP_TRY(hipMemcpyAsync(q, p->q.data(), sizeof(double) * p->q.size(), hipMemcpyHostToDevice, (hipStream_t)stream));
     HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return CCSD_OK;
+}
+
+int ccsd_plan_last_launches(const ccsd_plan* p, long long* rows, int max_rows, int* nrows)
+{
+    if (!p || !nrows) return fail(CCSD_ERR_NULL, "null argument");
+    *nrows = (int)p->last_launches.size();
+    if (rows)
+        for (int r = 0; r < std::min(max_rows, *nrows); r++)
+            for (int c = 0; c < 10; c++) rows[10 * r + c] = p->last_launches[r][c];
     return CCSD_OK;
 }
 

@@ -135,10 +135,45 @@ __device__ __forceinline__ void factor1d(double (&G)[LI + 2][LJ + 1][LK + 1][LL 
         }
 }
 
-// Part Q of P of the quartet handled by this lane; returns the lane's three sums.
+// Bra primitive-pair table of a wave (identical for its 64 lanes), kBraFields doubles
+// per pair: a_ij, P (3), norm * K_ab / a_ij, 2 a_i, 0.5 / a_ij, then the contraction-
+// column coefficient products c_i c_j (up to 4).
+constexpr int kBraFields = 11;
+
+template <int MIJ>
+__device__ __forceinline__ void fill_bra(const EriClassArgs& a, const ShellInfo& si, const ShellInfo& sj,
+                                         const double* RA, const double* AB, double norm, int lane, double* tab)
+{
+    const int nij = si.nprim * sj.nprim;
+    const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
+    const double* ei = a.env + si.pexp;
+    const double* ej = a.env + sj.pexp;
+    const double* cfi = a.env + si.pcoef;
+    const double* cfj = a.env + sj.pcoef;
+    const int ncij = si.nctr * sj.nctr;
+    for (int p = lane; p < nij; p += 64) {
+        const int ip = p / sj.nprim, jp = p - ip * sj.nprim;
+        const double ai = ei[ip], aj = ej[jp], aij = ai + aj, inv = 1.0 / aij;
+        double* e = tab + p * kBraFields;
+        e[0] = aij;
+        e[1] = RA[0] + AB[0] * (aj * inv);
+        e[2] = RA[1] + AB[1] * (aj * inv);
+        e[3] = RA[2] + AB[2] * (aj * inv);
+        e[4] = norm * exp(-ai * aj * inv * rab2) * inv;
+        e[5] = 2.0 * ai;
+        e[6] = 0.5 * inv;
+#pragma unroll
+        for (int cij = 0; cij < MIJ; cij++) {
+            const int c_i = cij / sj.nctr, c_j = cij - c_i * sj.nctr;
+            e[7 + cij] = cij < ncij ? cfi[c_i * si.nprim + ip] * cfj[c_j * sj.nprim + jp] : 0.0;
+        }
+    }
+}
+
+// Part Q of P of the quartet handled by this lane; adds to the lane's three sums.
 template <int LI, int LJ, int LK, int LL, int GC, int P, int Q>
-__device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 ch, int lane, double& sx, double& sy,
-                                             double& sz)
+__device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 ch, int lane, const double* tab,
+                                             double& sx, double& sy, double& sz)
 {
     constexpr int NFI = ncart_c(LI), NFJ = ncart_c(LJ), NFK = ncart_c(LK), NFL = ncart_c(LL);
     constexpr int NF = NFI * NFJ * NFK * NFL;
@@ -163,18 +198,12 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
     const double* RD = a.coords + 3 * sl.atom;
     const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
     const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
-    const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
     const double rcd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
-    const double norm =
-        34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) * shell_norm_dev(LL);
+    const int nij = si.nprim * sj.nprim;
     const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
     const bool same_kl = kp.x == kp.y;
-    const double* ei = a.env + si.pexp;
-    const double* ej = a.env + sj.pexp;
     const double* ek = a.env + sk.pexp;
     const double* el = a.env + sl.pexp;
-    const double* cfi = a.env + si.pcoef;
-    const double* cfj = a.env + sj.pcoef;
     const double* cfk = a.env + sk.pcoef;
     const double* cfl = a.env + sl.pcoef;
 
@@ -203,120 +232,134 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
                 gam[cij][ckl][t] = g;
             }
 
-    for (int ip = 0; ip < si.nprim; ip++)
-        for (int jp = 0; jp < sj.nprim; jp++) {
-            const double ai = ei[ip], aj = ej[jp], aij = ai + aj;
-            const double Kab = exp(-ai * aj / aij * rab2);
-            const double Pc[3] = {RA[0] + AB[0] * (aj / aij), RA[1] + AB[1] * (aj / aij), RA[2] + AB[2] * (aj / aij)};
-            double cb[MIJ];
+    for (int kp2 = 0; kp2 < sk.nprim; kp2++)
+        for (int lp = 0; lp < sl.nprim; lp++) {
+            // ket primitive pair: once per lane
+            const double ak = ek[kp2], al = el[lp], akl = ak + al, inv_kl = 1.0 / akl;
+            const double KcdN = exp(-ak * al * inv_kl * rcd2) * inv_kl;
+            const double Qc[3] = {RC[0] + CD[0] * (al * inv_kl), RC[1] + CD[1] * (al * inv_kl),
+                                  RC[2] + CD[2] * (al * inv_kl)};
+            const double QC[3] = {Qc[0] - RC[0], Qc[1] - RC[1], Qc[2] - RC[2]};
+            const double hkl = 0.5 * inv_kl;
+            double ck[MKL];
 #pragma unroll
-            for (int cij = 0; cij < MIJ; cij++) {
-                const int c_i = cij / sj.nctr, c_j = cij - c_i * sj.nctr;
-                cb[cij] = cij < ncij ? cfi[c_i * si.nprim + ip] * cfj[c_j * sj.nprim + jp] : 0.0;
+            for (int ckl = 0; ckl < MKL; ckl++) {
+                const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
+                ck[ckl] = ckl < nckl ? cfk[c_k * sk.nprim + kp2] * cfl[c_l * sl.nprim + lp] : 0.0;
             }
-            for (int kp2 = 0; kp2 < sk.nprim; kp2++)
-                for (int lp = 0; lp < sl.nprim; lp++) {
-                    const double ak = ek[kp2], al = el[lp], akl = ak + al;
-                    const double Kcd = exp(-ak * al / akl * rcd2);
-                    const double Qc[3] = {RC[0] + CD[0] * (al / akl), RC[1] + CD[1] * (al / akl),
-                                          RC[2] + CD[2] * (al / akl)};
-                    const double PQ[3] = {Pc[0] - Qc[0], Pc[1] - Qc[1], Pc[2] - Qc[2]};
-                    const double rr = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
-                    const double T = aij * akl / (aij + akl) * rr;
-                    double ck[MKL];
+            for (int p = 0; p < nij; p++) {
+                // bra primitive pair: broadcast reads of the wave's table
+                const double* e = tab + p * kBraFields;
+                const double aij = e[0];
+                const double Pc[3] = {e[1], e[2], e[3]};
+                const double ai2 = e[5], hij = e[6];
+                const double PQ[3] = {Pc[0] - Qc[0], Pc[1] - Qc[1], Pc[2] - Qc[2]};
+                const double PA[3] = {Pc[0] - RA[0], Pc[1] - RA[1], Pc[2] - RA[2]};
+                const double rr = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+                const double inv = 1.0 / (aij + akl);
+                const double T = aij * akl * inv * rr;
+                double pref = e[4] * KcdN * sqrt(inv);
+                double ge[NFP];
+                if constexpr (MIJ * MKL == 1) {
+                    pref *= e[7] * ck[0];
 #pragma unroll
-                    for (int ckl = 0; ckl < MKL; ckl++) {
-                        const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
-                        ck[ckl] = ckl < nckl ? cfk[c_k * sk.nprim + kp2] * cfl[c_l * sl.nprim + lp] : 0.0;
-                    }
-                    double pref = norm * Kab * Kcd / (aij * akl * sqrt(aij + akl));
-                    // Gamma of this primitive quartet (the columns' coefficients applied)
-                    double ge[NFP];
-                    if constexpr (MIJ * MKL == 1) {
-                        pref *= cb[0] * ck[0];
+                    for (int t = 0; t < NFP; t++) ge[t] = gam[0][0][t];
+                } else {
 #pragma unroll
-                        for (int t = 0; t < NFP; t++) ge[t] = gam[0][0][t];
-                    } else {
+                    for (int t = 0; t < NFP; t++) ge[t] = 0.0;
 #pragma unroll
-                        for (int t = 0; t < NFP; t++) ge[t] = 0.0;
+                    for (int cij = 0; cij < MIJ; cij++)
 #pragma unroll
-                        for (int cij = 0; cij < MIJ; cij++)
+                        for (int ckl = 0; ckl < MKL; ckl++) {
+                            const double cc = e[7 + cij] * ck[ckl];
 #pragma unroll
-                            for (int ckl = 0; ckl < MKL; ckl++) {
-                                const double cc = cb[cij] * ck[ckl];
-#pragma unroll
-                                for (int t = 0; t < NFP; t++) ge[t] += cc * gam[cij][ckl][t];
-                            }
-                    }
-                    double rt[NROOTS], wt[NROOTS];
-                    ccsd::rys_roots_dev(NROOTS, T, rt, wt);
-                    const double ai2 = 2.0 * ai;
-#pragma unroll
-                    for (int r = 0; r < NROOTS; r++) {
-                        const double a2 = rt[r] / (aij + akl);
-                        const double rt_aij = a2 * akl, rt_akl = a2 * aij;
-                        const double b00 = 0.5 * a2;
-                        const double b10 = 0.5 / aij * (1.0 - rt_aij);
-                        const double b01 = 0.5 / akl * (1.0 - rt_akl);
-                        double Gx[LI + 2][LJ + 1][LK + 1][LL + 1];
-                        double Gy[LI + 2][LJ + 1][LK + 1][LL + 1];
-                        double Gz[LI + 2][LJ + 1][LK + 1][LL + 1];
-                        factor1d<LI, LJ, LK, LL>(Gx, 1.0, (Pc[0] - RA[0]) - rt_aij * PQ[0],
-                                                 (Qc[0] - RC[0]) + rt_akl * PQ[0], b00, b10, b01, AB[0], CD[0]);
-                        factor1d<LI, LJ, LK, LL>(Gy, 1.0, (Pc[1] - RA[1]) - rt_aij * PQ[1],
-                                                 (Qc[1] - RC[1]) + rt_akl * PQ[1], b00, b10, b01, AB[1], CD[1]);
-                        factor1d<LI, LJ, LK, LL>(Gz, wt[r] * pref, (Pc[2] - RA[2]) - rt_aij * PQ[2],
-                                                 (Qc[2] - RC[2]) + rt_akl * PQ[2], b00, b10, b01, AB[2], CD[2]);
-#pragma unroll
-                        for (int t = 0; t < NFP; t++) {
-                            const int f = F0 + t;
-                            const int fi = f % NFI, r1 = f / NFI;
-                            const int fj = r1 % NFJ, r2 = r1 / NFJ;
-                            const int fk = r2 % NFK, fl = r2 / NFK;
-                            const int ix = ci_t.x[fi], iy = ci_t.y[fi], iz = ci_t.z[fi];
-                            const int jx = cj_t.x[fj], jy = cj_t.y[fj], jz = cj_t.z[fj];
-                            const int kx = ck_t.x[fk], ky = ck_t.y[fk], kz = ck_t.z[fk];
-                            const int lx = cl_t.x[fl], ly = cl_t.y[fl], lz = cl_t.z[fl];
-                            const double gx = Gx[ix][jx][kx][lx];
-                            const double gy = Gy[iy][jy][ky][ly];
-                            const double gz = Gz[iz][jz][kz][lz];
-                            double dx = -ai2 * Gx[ix + 1][jx][kx][lx];
-                            double dy = -ai2 * Gy[iy + 1][jy][ky][ly];
-                            double dz = -ai2 * Gz[iz + 1][jz][kz][lz];
-                            if (ix > 0) dx += ix * Gx[ix > 0 ? ix - 1 : 0][jx][kx][lx];
-                            if (iy > 0) dy += iy * Gy[iy > 0 ? iy - 1 : 0][jy][ky][ly];
-                            if (iz > 0) dz += iz * Gz[iz > 0 ? iz - 1 : 0][jz][kz][lz];
-                            const double g = ge[t];
-                            sx += g * dx * gy * gz;
-                            sy += g * gx * dy * gz;
-                            sz += g * gx * gy * dz;
+                            for (int t = 0; t < NFP; t++) ge[t] += cc * gam[cij][ckl][t];
                         }
+                }
+                double rt[NROOTS], wt[NROOTS];
+                ccsd::rys_roots_dev(NROOTS, T, rt, wt);
+#pragma unroll
+                for (int r = 0; r < NROOTS; r++) {
+                    const double a2 = rt[r] * inv;
+                    const double rt_aij = a2 * akl, rt_akl = a2 * aij;
+                    const double b00 = 0.5 * a2;
+                    const double b10 = hij * (1.0 - rt_aij);
+                    const double b01 = hkl * (1.0 - rt_akl);
+                    double Gx[LI + 2][LJ + 1][LK + 1][LL + 1];
+                    double Gy[LI + 2][LJ + 1][LK + 1][LL + 1];
+                    double Gz[LI + 2][LJ + 1][LK + 1][LL + 1];
+                    factor1d<LI, LJ, LK, LL>(Gx, 1.0, PA[0] - rt_aij * PQ[0], QC[0] + rt_akl * PQ[0], b00, b10, b01,
+                                             AB[0], CD[0]);
+                    factor1d<LI, LJ, LK, LL>(Gy, 1.0, PA[1] - rt_aij * PQ[1], QC[1] + rt_akl * PQ[1], b00, b10, b01,
+                                             AB[1], CD[1]);
+                    factor1d<LI, LJ, LK, LL>(Gz, wt[r] * pref, PA[2] - rt_aij * PQ[2], QC[2] + rt_akl * PQ[2], b00,
+                                             b10, b01, AB[2], CD[2]);
+#pragma unroll
+                    for (int t = 0; t < NFP; t++) {
+                        const int f = F0 + t;
+                        const int fi = f % NFI, r1 = f / NFI;
+                        const int fj = r1 % NFJ, r2 = r1 / NFJ;
+                        const int fk = r2 % NFK, fl = r2 / NFK;
+                        const int ix = ci_t.x[fi], iy = ci_t.y[fi], iz = ci_t.z[fi];
+                        const int jx = cj_t.x[fj], jy = cj_t.y[fj], jz = cj_t.z[fj];
+                        const int kx = ck_t.x[fk], ky = ck_t.y[fk], kz = ck_t.z[fk];
+                        const int lx = cl_t.x[fl], ly = cl_t.y[fl], lz = cl_t.z[fl];
+                        const double gx = Gx[ix][jx][kx][lx];
+                        const double gy = Gy[iy][jy][ky][ly];
+                        const double gz = Gz[iz][jz][kz][lz];
+                        double dx = -ai2 * Gx[ix + 1][jx][kx][lx];
+                        double dy = -ai2 * Gy[iy + 1][jy][ky][ly];
+                        double dz = -ai2 * Gz[iz + 1][jz][kz][lz];
+                        if (ix > 0) dx += ix * Gx[ix > 0 ? ix - 1 : 0][jx][kx][lx];
+                        if (iy > 0) dy += iy * Gy[iy > 0 ? iy - 1 : 0][jy][ky][ly];
+                        if (iz > 0) dz += iz * Gz[iz > 0 ? iz - 1 : 0][jz][kz][lz];
+                        const double g = ge[t];
+                        sx += g * dx * gy * gz;
+                        sy += g * gx * dy * gz;
+                        sz += g * gx * gy * dz;
                     }
                 }
+            }
         }
 }
 
 template <int LI, int LJ, int LK, int LL, int GC, int P, int... Q>
-__device__ __forceinline__ void run_parts(const EriClassArgs& a, const int4 ch, int lane, int part, double& sx,
-                                          double& sy, double& sz, std::integer_sequence<int, Q...>)
+__device__ __forceinline__ void run_parts(const EriClassArgs& a, const int4 ch, int lane, int part, const double* tab,
+                                          double& sx, double& sy, double& sz, std::integer_sequence<int, Q...>)
 {
-    ((part == Q ? quartet_part<LI, LJ, LK, LL, GC, P, Q>(a, ch, lane, sx, sy, sz) : void()), ...);
+    ((part == Q ? quartet_part<LI, LJ, LK, LL, GC, P, Q>(a, ch, lane, tab, sx, sy, sz) : void()), ...);
 }
 
+// Dynamic LDS: one bra table of max_nij pairs per wave of the block.
 template <int LI, int LJ, int LK, int LL, int GC, int P>
 __global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
 {
     if constexpr (min_parts(LI, LJ, LK, LL, GC) == 0 || P < min_parts(LI, LJ, LK, LL, GC)) {
         return;
     } else {
-        const int lane = threadIdx.x & 63;
-        const int gw = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+        extern __shared__ double lds_tab[];
+        constexpr int MIJ = GC ? mcol(LI) * mcol(LJ) : 1;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int gw = blockIdx.x * (blockDim.x >> 6) + wave;
         const int c = gw / P;
-        if (c >= a.nchunks) return;   // whole waves leave; nothing below synchronizes the block
+        const bool active = c < a.nchunks;   // every wave reaches the barrier below
         const int part = gw - c * P;
-        const int4 ch = a.chunks[c];
+        const int4 ch = active ? a.chunks[c] : make_int4(0, 0, 0, 0);
+        double* tab = lds_tab + (size_t)wave * a.max_nij * kBraFields;
+        const int2 bp = a.bra[ch.x];
+        const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+        if (active) {
+            const double* RA = a.coords + 3 * si.atom;
+            const double* RB = a.coords + 3 * sj.atom;
+            const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
+            const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
+                                shell_norm_dev(LL);
+            fill_bra<MIJ>(a, si, sj, RA, AB, norm, lane, tab);
+        }
+        __syncthreads();
+        if (!active) return;
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        run_parts<LI, LJ, LK, LL, GC, P>(a, ch, lane, part, sx, sy, sz, std::make_integer_sequence<int, P>{});
+        run_parts<LI, LJ, LK, LL, GC, P>(a, ch, lane, part, tab, sx, sy, sz, std::make_integer_sequence<int, P>{});
         // wave reduction (the chunk's bra pair, hence the atom of i, is shared by the wave)
         for (int off = 32; off > 0; off >>= 1) {
             sx += __shfl_down(sx, off, 64);
@@ -324,10 +367,9 @@ __global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
             sz += __shfl_down(sz, off, 64);
         }
         if (lane == 0) {
-            const int atom = a.shells[a.bra[ch.x].x].atom;
-            atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * sx);
-            atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * sy);
-            atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * sz);
+            atomicAdd(a.de + 3 * si.atom + 0, a.gamma_scale * sx);
+            atomicAdd(a.de + 3 * si.atom + 1, a.gamma_scale * sy);
+            atomicAdd(a.de + 3 * si.atom + 2, a.gamma_scale * sz);
         }
     }
 }
@@ -342,7 +384,10 @@ hipError_t launch(const EriClassArgs& a, int wpb, hipStream_t stream)
     const long long waves = (long long)a.nchunks * P;
     const long long blocks = (waves + wpb - 1) / wpb;
     if (blocks > 2147483647LL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((fused_reg_kernel<LI, LJ, LK, LL, G, P>), dim3((unsigned)blocks), dim3(64 * wpb), 0, stream, a);
+    const size_t lds = sizeof(double) * (size_t)wpb * a.max_nij * kBraFields;
+    if (lds > 65536) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((fused_reg_kernel<LI, LJ, LK, LL, G, P>), dim3((unsigned)blocks), dim3(64 * wpb), lds, stream,
+                       a);
     return hipGetLastError();
 }
 

@@ -18,6 +18,9 @@ int ccsd_fused_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_n
     return min_parts(li, lj, lk, ll, gc);
 }
 
+// Dynamic LDS of a register-kernel launch (the waves' bra primitive-pair tables).
+size_t ccsd_fused_reg_lds(int max_nij, int wpb) { return sizeof(double) * (size_t)wpb * max_nij * ccsd_reg::kBraFields; }
+
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int wpb, hipStream_t stream)
 {
     const int gc = (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0;

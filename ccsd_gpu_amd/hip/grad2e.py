@@ -94,3 +94,20 @@ def last_quartets(mol):
     n = _lib.ctypes.c_longlong()
     _lib.check(_lib.load().ccsd_plan_quartets(_lib.get_plan(mol), _lib.ctypes.byref(n)), "ccsd_plan_quartets")
     return n.value
+
+
+LAUNCH_FIELDS = ("li", "lj", "lk", "ll", "gc", "kernel", "waves_per_block", "chunk", "parts", "quartets")
+
+
+def last_launches(mol=None, plan=None):
+    """Launch records of the last integral / fused call on a plan (a molecule's
+    cached plan, or an explicit handle): one dict per launch, in launch order."""
+    import numpy as np
+    lib = _lib.load()
+    plan = _lib.get_plan(mol) if plan is None else plan
+    n = _lib.ctypes.c_int()
+    _lib.check(lib.ccsd_plan_last_launches(plan, None, 0, _lib.ctypes.byref(n)), "ccsd_plan_last_launches")
+    buf = np.zeros((max(1, n.value), 10), dtype=np.int64)
+    _lib.check(lib.ccsd_plan_last_launches(plan, buf.ctypes.data, n.value, _lib.ctypes.byref(n)),
+               "ccsd_plan_last_launches")
+    return [dict(zip(LAUNCH_FIELDS, map(int, row))) for row in buf[:n.value]]

@@ -90,6 +90,13 @@ int ccsd_schwarz(ccsd_plan* plan, double* q, void* stream);
 /* Shell quartets evaluated by the last integral or fused-gradient call. */
 int ccsd_plan_quartets(const ccsd_plan* plan, long long* count);
 
+/* The launches of the last integral or fused-gradient call, 10 numbers each, in
+ * launch order: li, lj, lk, ll, general-contraction variant (0/1), kernel (0 LDS,
+ * 1 register), waves per block, quartets per chunk, waves per quartet (register
+ * kernel), quartets. *nrows receives the count; up to max_rows rows are written
+ * to `rows` (may be NULL). No counterpart in the reference (profiling aid). */
+int ccsd_plan_last_launches(const ccsd_plan* plan, long long* rows, int max_rows, int* nrows);
+
 /* Fused ERI-derivative contraction over all shells of every molecule of the plan
  * with the separable two-particle density of Cartesian densities D and P
  * (dm, pm: per molecule a row-major nao_m x nao_m block, blocks in molecule order):
