@@ -10,10 +10,11 @@
 //
 // Register budget: a lane holds Gamma for its quartet (all contraction
 // columns) plus one root's three 1-D factors. Classes whose Gamma does not fit
-// are split over P <= 4 waves ("parts"): part q contracts the Cartesian
+// are split over P <= 8 waves ("parts"): part q contracts the Cartesian
 // functions [q NF / P, (q+1) NF / P) and recomputes the roots and 1-D factors,
-// which cost far less than the contraction. The part index is uniform over a
-// wave, so the P specialised bodies never diverge.
+// which cost far less than the contraction; only the 1-D factor entries the
+// part reads stay live. The part index is uniform over a wave, so the P
+// specialised bodies never diverge.
 //
 // General contractions (GC = 1): Gamma of every contraction-column combination
 // is loaded once per quartet and combined with the primitive coefficients of
@@ -59,7 +60,7 @@ __device__ __forceinline__ double shell_norm_dev(int l)
 #define CCSD_REG_BUDGET 110
 #endif
 constexpr int kBudget = CCSD_REG_BUDGET;   // doubles a lane may hold for Gamma and the 1-D factors
-constexpr int kMaxParts = 4;   // beyond this the class stays on the LDS kernel
+constexpr int kMaxParts = 8;   // beyond this the class stays on the LDS kernel
 
 constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
 // contraction columns a shell may bring into a register class: generally contracted
@@ -67,13 +68,51 @@ constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
 constexpr int mcol(int l) { return l == 0 ? 2 : 1; }
 constexpr bool has_s(int li, int lj, int lk, int ll) { return li == 0 || lj == 0 || lk == 0 || ll == 0; }
 
+// exponent of direction d of Cartesian component f of angular momentum l <= 2 (PySCF order)
+constexpr int kCartExp[3][6][3] = {
+    {{0, 0, 0}},
+    {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}},
+    {{2, 0, 0}, {1, 1, 0}, {1, 0, 1}, {0, 2, 0}, {0, 1, 1}, {0, 0, 2}}};
+constexpr int cart_exp(int l, int f, int d) { return kCartExp[l][f][d]; }
+
+// 1-D factor entries a part reads (the rest are dead and never held): per direction
+// the (a, b, c, d) of its functions, with a + 1 and a - 1 for the derivative.
+constexpr int g_live(int li, int lj, int lk, int ll, int f0, int f1)
+{
+    const int nfi = ncart_c(li), nfj = ncart_c(lj), nfk = ncart_c(lk);
+    const int sa = li + 2, sb = lj + 1, sc = lk + 1, sd = ll + 1;
+    int total = 0;
+    for (int d = 0; d < 3; d++) {
+        bool used[4 * 3 * 3 * 3] = {};
+        for (int f = f0; f < f1; f++) {
+            const int fi = f % nfi, r1 = f / nfi;
+            const int fj = r1 % nfj, r2 = r1 / nfj;
+            const int fk = r2 % nfk, fl = r2 / nfk;
+            const int ea = cart_exp(li, fi, d), eb = cart_exp(lj, fj, d);
+            const int ec = cart_exp(lk, fk, d), ed = cart_exp(ll, fl, d);
+            const int base = ((eb * sc) + ec) * sd + ed;
+            used[(ea + 1) * sb * sc * sd + base] = true;
+            used[ea * sb * sc * sd + base] = true;
+            if (ea > 0) used[(ea - 1) * sb * sc * sd + base] = true;
+        }
+        for (int u = 0; u < sa * sb * sc * sd; u++) total += used[u] ? 1 : 0;
+    }
+    return total;
+}
+
+// doubles a lane holds for part count p: Gamma of the part (with its contraction
+// columns) and the live 1-D factor entries, worst part
 constexpr int need(int li, int lj, int lk, int ll, int gc, int p)
 {
     const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
-    const int nfp = (nf + p - 1) / p;
-    const int gs = (li + 2) * (lj + 1) * (lk + 1) * (ll + 1);
     const int m = gc ? mcol(li) * mcol(lj) * mcol(lk) * mcol(ll) : 1;
-    return (m > 1 ? m * nfp + nfp : nfp) + 3 * gs;
+    int worst = 0;
+    for (int q = 0; q < p; q++) {
+        const int f0 = nf * q / p, f1 = nf * (q + 1) / p, nfp = f1 - f0;
+        const int v = (m > 1 ? m * nfp + nfp : nfp) + g_live(li, lj, lk, ll, f0, f1);
+        worst = v > worst ? v : worst;
+    }
+    return worst;
 }
 
 // Smallest part count that fits the budget (0: class not supported).
@@ -393,21 +432,20 @@ hipError_t launch(const EriClassArgs& a, int wpb, hipStream_t stream)
 
 }  // namespace ccsd_reg
 
-// Launcher of the classes with bra angular momenta (LI, LJ); each pair is its own
+// Launcher of the classes with angular momenta (LI, LJ, LK, *); each triple is its own
 // object file (compiled in parallel).
 #define CCSD_REG_CASE(I, J, K, L)                                                                            \
-    case (K * 3 + L):                                                                                        \
+    case L:                                                                                                  \
         return gc ? ccsd_reg::launch<I, J, K, L, 1>(a, wpb, stream) : ccsd_reg::launch<I, J, K, L, 0>(a, wpb, stream);
-#define CCSD_REG_L(I, J, K) CCSD_REG_CASE(I, J, K, 0) CCSD_REG_CASE(I, J, K, 1) CCSD_REG_CASE(I, J, K, 2)
-#define CCSD_REG_TU(I, J, NAME)                                                                              \
+#define CCSD_REG_TU(I, J, K, NAME)                                                                           \
     hipError_t NAME(const EriClassArgs& a, int gc, int wpb, hipStream_t stream)                              \
     {                                                                                                        \
         if (a.nchunks <= 0) return hipSuccess;                                                               \
         if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;                                                 \
-        switch (a.lk * 3 + a.ll) {                                                                           \
-            CCSD_REG_L(I, J, 0)                                                                              \
-            CCSD_REG_L(I, J, 1)                                                                              \
-            CCSD_REG_L(I, J, 2)                                                                              \
+        switch (a.ll) {                                                                                      \
+            CCSD_REG_CASE(I, J, K, 0)                                                                        \
+            CCSD_REG_CASE(I, J, K, 1)                                                                        \
+            CCSD_REG_CASE(I, J, K, 2)                                                                        \
             default:                                                                                         \
                 return hipErrorInvalidValue;                                                                 \
         }                                                                                                    \

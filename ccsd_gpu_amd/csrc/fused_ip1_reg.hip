@@ -1,11 +1,34 @@
 // Host side of the register-resident fused kernels: class support and dispatch
-// to the per-(l_i, l_j) object files.
+// to the per-(l_i, l_j, l_k) object files.
 #include "fused_ip1_reg.h"
 
-#define CCSD_REG_DECL(I, J) hipError_t ccsd_launch_fused_reg_l##I##J(const EriClassArgs&, int, int, hipStream_t);
-CCSD_REG_DECL(0, 0) CCSD_REG_DECL(0, 1) CCSD_REG_DECL(0, 2)
-CCSD_REG_DECL(1, 0) CCSD_REG_DECL(1, 1) CCSD_REG_DECL(1, 2)
-CCSD_REG_DECL(2, 0) CCSD_REG_DECL(2, 1) CCSD_REG_DECL(2, 2)
+hipError_t ccsd_launch_fused_reg_l000(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l001(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l002(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l010(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l011(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l012(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l020(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l021(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l022(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l100(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l101(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l102(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l110(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l111(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l112(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l120(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l121(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l122(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l200(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l201(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l202(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l210(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l211(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l212(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l220(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l221(const EriClassArgs&, int, int, hipStream_t);
+hipError_t ccsd_launch_fused_reg_l222(const EriClassArgs&, int, int, hipStream_t);
 
 // Waves per quartet ("parts") of the class's register kernel, 0 if it has none.
 // The kernels assume single-column shells except s shells with up to two columns.
@@ -24,16 +47,34 @@ size_t ccsd_fused_reg_lds(int max_nij, int wpb) { return sizeof(double) * (size_
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int wpb, hipStream_t stream)
 {
     const int gc = (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0;
-    switch (a.li * 3 + a.lj) {
-        case 0: return ccsd_launch_fused_reg_l00(a, gc, wpb, stream);
-        case 1: return ccsd_launch_fused_reg_l01(a, gc, wpb, stream);
-        case 2: return ccsd_launch_fused_reg_l02(a, gc, wpb, stream);
-        case 3: return ccsd_launch_fused_reg_l10(a, gc, wpb, stream);
-        case 4: return ccsd_launch_fused_reg_l11(a, gc, wpb, stream);
-        case 5: return ccsd_launch_fused_reg_l12(a, gc, wpb, stream);
-        case 6: return ccsd_launch_fused_reg_l20(a, gc, wpb, stream);
-        case 7: return ccsd_launch_fused_reg_l21(a, gc, wpb, stream);
-        case 8: return ccsd_launch_fused_reg_l22(a, gc, wpb, stream);
+    switch (a.li * 9 + a.lj * 3 + a.lk) {
+        case 0: return ccsd_launch_fused_reg_l000(a, gc, wpb, stream);
+        case 1: return ccsd_launch_fused_reg_l001(a, gc, wpb, stream);
+        case 2: return ccsd_launch_fused_reg_l002(a, gc, wpb, stream);
+        case 3: return ccsd_launch_fused_reg_l010(a, gc, wpb, stream);
+        case 4: return ccsd_launch_fused_reg_l011(a, gc, wpb, stream);
+        case 5: return ccsd_launch_fused_reg_l012(a, gc, wpb, stream);
+        case 6: return ccsd_launch_fused_reg_l020(a, gc, wpb, stream);
+        case 7: return ccsd_launch_fused_reg_l021(a, gc, wpb, stream);
+        case 8: return ccsd_launch_fused_reg_l022(a, gc, wpb, stream);
+        case 9: return ccsd_launch_fused_reg_l100(a, gc, wpb, stream);
+        case 10: return ccsd_launch_fused_reg_l101(a, gc, wpb, stream);
+        case 11: return ccsd_launch_fused_reg_l102(a, gc, wpb, stream);
+        case 12: return ccsd_launch_fused_reg_l110(a, gc, wpb, stream);
+        case 13: return ccsd_launch_fused_reg_l111(a, gc, wpb, stream);
+        case 14: return ccsd_launch_fused_reg_l112(a, gc, wpb, stream);
+        case 15: return ccsd_launch_fused_reg_l120(a, gc, wpb, stream);
+        case 16: return ccsd_launch_fused_reg_l121(a, gc, wpb, stream);
+        case 17: return ccsd_launch_fused_reg_l122(a, gc, wpb, stream);
+        case 18: return ccsd_launch_fused_reg_l200(a, gc, wpb, stream);
+        case 19: return ccsd_launch_fused_reg_l201(a, gc, wpb, stream);
+        case 20: return ccsd_launch_fused_reg_l202(a, gc, wpb, stream);
+        case 21: return ccsd_launch_fused_reg_l210(a, gc, wpb, stream);
+        case 22: return ccsd_launch_fused_reg_l211(a, gc, wpb, stream);
+        case 23: return ccsd_launch_fused_reg_l212(a, gc, wpb, stream);
+        case 24: return ccsd_launch_fused_reg_l220(a, gc, wpb, stream);
+        case 25: return ccsd_launch_fused_reg_l221(a, gc, wpb, stream);
+        case 26: return ccsd_launch_fused_reg_l222(a, gc, wpb, stream);
         default: return hipErrorInvalidValue;
     }
 }
