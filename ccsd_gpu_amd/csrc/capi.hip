@@ -83,9 +83,20 @@ struct PairClass {
     std::vector<int2> pairs;
     std::vector<int> seg;          // segment s = pairs[seg[s], seg[s + 1])
     std::vector<int> seg_mol;
+    std::vector<int> seg_eff;      // surviving primitive pairs of the segment's pairs
     int max_np = 0, max_nc = 0;
     int2* d_pairs = nullptr;
+    // ket classes: the surviving primitive pairs of every pair (kKetFields doubles each),
+    // pair t's entries at [pp_off[t], pp_off[t + 1])
+    int* d_pp_off = nullptr;
+    double* d_pp = nullptr;
 };
+
+// fields of a ket primitive-pair entry: a_kl, Q (3), K_cd / a_kl, 0.5 / a_kl, c_k c_l (4)
+constexpr int kKetFields = 10;
+// primitive pairs with a_1 a_2 / (a_1 + a_2) |R_12|^2 above this are not evaluated
+// (Gaussian-product factor below e^-50); the same cut as the register kernels' bra side
+constexpr double kPrimExpCut = 50.0;
 
 // Device work list of one (bra class, ket class) launch.
 struct ChunkList {
@@ -146,16 +157,62 @@ struct ccsd_plan {
     long long last_quartets = 0;
     std::vector<std::array<long long, 10>> last_launches;   // see ccsd_plan_last_launches
     double* d_de_scratch = nullptr;                  // autotuner's throw-away gradient
+    std::vector<double> coords_h;                    // natm x 3
+    std::vector<int> eff;                            // surviving primitive pairs of same-molecule shell pairs
+    std::vector<long long> mol_eff_off;
     std::vector<C2sTermHost> c2s;                    // per spherical AO
     void* d_c2s = nullptr;
 };
 
 namespace {
 
+int pair_eff(const ccsd_plan* p, int i, int j)
+{
+    const int m = p->shells[i].mol, s0 = p->mol_shell0[m], n = p->mol_shell0[m + 1] - s0;
+    return p->eff[p->mol_eff_off[m] + (long long)(i - s0) * n + (j - s0)];
+}
+
+// Primitive pairs of (i, j) past the Gaussian-product cut, for the plan's geometry.
+void compute_eff(ccsd_plan* p)
+{
+    p->mol_eff_off.assign(p->nmol + 1, 0);
+    for (int m = 0; m < p->nmol; m++) {
+        const long long n = p->mol_shell0[m + 1] - p->mol_shell0[m];
+        p->mol_eff_off[m + 1] = p->mol_eff_off[m] + n * n;
+    }
+    p->eff.assign(p->mol_eff_off[p->nmol], 0);
+    for (int m = 0; m < p->nmol; m++) {
+        const int s0 = p->mol_shell0[m], s1 = p->mol_shell0[m + 1], n = s1 - s0;
+        for (int i = s0; i < s1; i++)
+            for (int j = s0; j < s1; j++) {
+                const ShellInfo &a = p->shells[i], &b = p->shells[j];
+                const double* A = &p->coords_h[3 * a.atom];
+                const double* B = &p->coords_h[3 * b.atom];
+                const double r2 = (A[0] - B[0]) * (A[0] - B[0]) + (A[1] - B[1]) * (A[1] - B[1]) +
+                                  (A[2] - B[2]) * (A[2] - B[2]);
+                int c = 0;
+                for (int u = 0; u < a.nprim; u++)
+                    for (int v = 0; v < b.nprim; v++) {
+                        const double x = p->env[a.pexp + u], y = p->env[b.pexp + v];
+                        if (x * y / (x + y) * r2 <= kPrimExpCut) c++;
+                    }
+                p->eff[p->mol_eff_off[m] + (long long)(i - s0) * n + (j - s0)] = c;
+            }
+    }
+}
+
+// Grouping key of a pair inside a class: surviving primitive pairs in power-of-two
+// buckets (0, 1, 2-3, 4-7, ...) and contraction columns, so that the lanes of a wave
+// loop about alike without cutting the lists into many short segments.
 int prim_group(const ccsd_plan* p, const int2& pr)
 {
     const ShellInfo &a = p->shells[pr.x], &b = p->shells[pr.y];
-    return a.nprim * b.nprim * a.nctr * b.nctr;
+    int e = pair_eff(p, pr.x, pr.y), bucket = 0;
+    while (e) {
+        bucket++;
+        e >>= 1;
+    }
+    return bucket * 16 + a.nctr * b.nctr;
 }
 
 double pair_q(const ccsd_plan* p, const int2& pr) { return p->q[(size_t)pr.x * p->nbas + pr.y]; }
@@ -189,13 +246,15 @@ std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>&
             if (mx != my) return mx < my;
             const int gx = prim_group(p, x), gy = prim_group(p, y);
             if (gx != gy) return gx > gy;
-            return by_q && pair_q(p, x) > pair_q(p, y);
+            if (by_q) return pair_q(p, x) > pair_q(p, y);
+            return pair_eff(p, x.x, x.y) > pair_eff(p, y.x, y.y);
         });
         for (size_t t = 0; t < c.pairs.size(); t++) {
             if (t == 0 || p->shells[c.pairs[t].x].mol != p->shells[c.pairs[t - 1].x].mol ||
                 prim_group(p, c.pairs[t]) != prim_group(p, c.pairs[t - 1])) {
                 c.seg.push_back((int)t);
                 c.seg_mol.push_back(p->shells[c.pairs[t].x].mol);
+                c.seg_eff.push_back(pair_eff(p, c.pairs[t].x, c.pairs[t].y));
             }
         }
         c.seg.push_back((int)c.pairs.size());
@@ -214,9 +273,55 @@ int upload_classes(std::vector<PairClass>& cls)
 
 void free_classes(std::vector<PairClass>& cls)
 {
-    for (PairClass& c : cls)
+    for (PairClass& c : cls) {
         if (c.d_pairs) (void)hipFree(c.d_pairs);
+        if (c.d_pp_off) (void)hipFree(c.d_pp_off);
+        if (c.d_pp) (void)hipFree(c.d_pp);
+    }
     cls.clear();
+}
+
+// The surviving primitive pairs of every ket pair of every class, in device memory.
+int upload_ket_prims(ccsd_plan* p)
+{
+    for (PairClass& c : p->ket) {
+        std::vector<int> off(c.pairs.size() + 1, 0);
+        std::vector<double> pp;
+        for (size_t t = 0; t < c.pairs.size(); t++) {
+            const ShellInfo &k = p->shells[c.pairs[t].x], &l = p->shells[c.pairs[t].y];
+            const double* C = &p->coords_h[3 * k.atom];
+            const double* D = &p->coords_h[3 * l.atom];
+            const double CD[3] = {D[0] - C[0], D[1] - C[1], D[2] - C[2]};
+            const double r2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
+            const int nckl = k.nctr * l.nctr;
+            for (int u = 0; u < k.nprim; u++)
+                for (int v = 0; v < l.nprim; v++) {
+                    const double ak = p->env[k.pexp + u], al = p->env[l.pexp + v], akl = ak + al;
+                    const double mu = ak * al / akl * r2;
+                    if (mu > kPrimExpCut) continue;
+                    double e[kKetFields];
+                    e[0] = akl;
+                    for (int x = 0; x < 3; x++) e[1 + x] = C[x] + CD[x] * (al / akl);
+                    e[4] = exp(-mu) / akl;
+                    e[5] = 0.5 / akl;
+                    for (int q = 0; q < 4; q++) {
+                        if (q < nckl) {
+                            const int ck = q / l.nctr, cl = q - ck * l.nctr;
+                            e[6 + q] = p->env[k.pcoef + ck * k.nprim + u] * p->env[l.pcoef + cl * l.nprim + v];
+                        } else {
+                            e[6 + q] = 0.0;
+                        }
+                    }
+                    pp.insert(pp.end(), e, e + kKetFields);
+                }
+            off[t + 1] = (int)(pp.size() / kKetFields);
+        }
+        HIP_TRY(hipMalloc(&c.d_pp_off, sizeof(int) * off.size()));
+        HIP_TRY(hipMemcpy(c.d_pp_off, off.data(), sizeof(int) * off.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMalloc(&c.d_pp, sizeof(double) * std::max<size_t>(1, pp.size())));
+        if (!pp.empty()) HIP_TRY(hipMemcpy(c.d_pp, pp.data(), sizeof(double) * pp.size(), hipMemcpyHostToDevice));
+    }
+    return CCSD_OK;
 }
 
 void free_braset(BraSet& b)
@@ -241,7 +346,9 @@ int build_ket(ccsd_plan* p)
         for (int k = p->mol_shell0[m]; k < p->mol_shell0[m + 1]; k++)
             for (int l = p->mol_shell0[m]; l <= k; l++) kpairs.push_back(make_int2(k, l));
     p->ket = make_classes(p, kpairs, p->thr > 0.0 && p->q_valid);
-    return upload_classes(p->ket);
+    int rc = upload_classes(p->ket);
+    if (rc) return rc;
+    return upload_ket_prims(p);
 }
 
 int get_bra(ccsd_plan* p, int ish0, int ish1, BraSet** out)
@@ -270,9 +377,10 @@ int get_bra(ccsd_plan* p, int ish0, int ish1, BraSet** out)
 // Work chunks of (bra class bc) x (ket class kc): per bra pair, the ket pairs of
 // its molecule, per segment the Schwarz survivors (a prefix), cut into runs of at
 // most `chunk`; most expensive chunks first so that the tail of the launch is short.
-int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screened, ChunkList** out)
+int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screened, bool fused, ChunkList** out)
 {
-    auto key = std::make_tuple(bc, kc, chunk, screened ? 1 : 0);
+    // fused gradients also drop pairs without a primitive pair past the Gaussian-product cut
+    auto key = std::make_tuple(bc, kc, chunk, (screened ? 1 : 0) | (fused ? 2 : 0));
     auto it = bs.chunks.find(key);
     if (it == bs.chunks.end()) {
         const PairClass& B = bs.cls[bc];
@@ -282,10 +390,13 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screene
         for (int bq = 0; bq < (int)B.pairs.size(); bq++) {
             const int2 bp = B.pairs[bq];
             const int m = p->shells[bp.x].mol;
-            const long long cb = prim_group(p, bp);
+            const int eb = pair_eff(p, bp.x, bp.y);
+            if (fused && eb == 0) continue;
+            const long long cb = (long long)std::max(1, eb) * p->shells[bp.x].nctr * p->shells[bp.y].nctr;
             const double qb = screened ? pair_q(p, bp) : 0.0;
             for (size_t s = 0; s + 1 < K.seg.size(); s++) {
                 if (K.seg_mol[s] != m) continue;
+                if (fused && K.seg_eff[s] == 0) continue;
                 const int k0 = K.seg[s];
                 int k1 = K.seg[s + 1];
                 if (screened) {
@@ -298,7 +409,8 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screene
                     }
                     k1 = lo;
                 }
-                const long long ck = prim_group(p, K.pairs[k0]);
+                const long long ck = (long long)std::max(1, K.seg_eff[s]) * p->shells[K.pairs[k0].x].nctr *
+                                     p->shells[K.pairs[k0].y].nctr;
                 for (int k = k0; k < k1; k += chunk) {
                     const int ke = std::min(k1, k + chunk);
                     work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
@@ -339,6 +451,8 @@ int upload_geometry(ccsd_plan* p)
     }
     HIP_TRY(hipMemcpy(p->d_env, p->env.data(), sizeof(double) * p->nenv, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(p->d_coords, coords.data(), sizeof(double) * 3 * p->natm, hipMemcpyHostToDevice));
+    p->coords_h = coords;
+    compute_eff(p);
     std::vector<double> charges(p->natm);
     for (int a = 0; a < p->natm; a++) charges[a] = (double)p->atm[a * CCSD_ATM_SLOTS + 0];
     HIP_TRY(hipMemcpy(p->d_charges, charges.data(), sizeof(double) * p->natm, hipMemcpyHostToDevice));
@@ -390,6 +504,8 @@ EriClassArgs class_args(const ccsd_plan* p, const PairClass& bc, const PairClass
     a.mol_ao0 = p->d_mol_ao0;
     a.mol_nao = p->d_mol_nao;
     a.mol_dm_off = p->d_mol_dm_off;
+    a.ket_pp_off = kc.d_pp_off;
+    a.ket_pp = kc.d_pp;
     return a;
 }
 
@@ -446,7 +562,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
         double best_t = 1e300;
         for (const Choice& c : cand) {
             ChunkList* cl = nullptr;
-            int rc = get_chunks(p, bs, bc, kc, c.chunk, screened, &cl);
+            int rc = get_chunks(p, bs, bc, kc, c.chunk, screened, true, &cl);
             if (rc) return rc;
             if (cl->n == 0) break;   // nothing to run in this class
             const int ns = (int)(std::lower_bound(cl->cumq.begin() + 1, cl->cumq.end(), 262144LL) - cl->cumq.begin());
@@ -473,7 +589,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
         // drop the work lists of the candidates not taken
         for (const Choice& c : cand) {
             if (c.chunk == best.chunk) continue;
-            auto ck = bs.chunks.find(std::make_tuple(bc, kc, c.chunk, screened ? 1 : 0));
+            auto ck = bs.chunks.find(std::make_tuple(bc, kc, c.chunk, (screened ? 1 : 0) | 2));
             if (ck != bs.chunks.end()) {
                 if (ck->second.d) (void)hipFree(ck->second.d);
                 bs.chunks.erase(ck);
@@ -562,7 +678,7 @@ int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long l
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], deriv);
             ChunkList* cl = nullptr;
-            int rc = get_chunks(p, bs, bc, kc, 1, false, &cl);
+            int rc = get_chunks(p, bs, bc, kc, 1, false, false, &cl);
             if (rc) return rc;
             a.chunks = cl->d;
             a.nchunks = cl->n;
@@ -600,7 +716,7 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c);
             if (rc) return rc;
             ChunkList* cl = nullptr;
-            rc = get_chunks(p, bs, bc, kc, c.chunk, screened, &cl);
+            rc = get_chunks(p, bs, bc, kc, c.chunk, screened, true, &cl);
             if (rc) return rc;
             a.chunks = cl->d;
             a.nchunks = cl->n;
@@ -853,7 +969,11 @@ int ccsd_plan_set_env(ccsd_plan* p, const double* env, int nenv)
     if (nenv != p->nenv) return fail(CCSD_ERR_SHAPE, "env length differs from the plan's");
     p->env.assign(env, env + nenv);
     p->q_valid = false;     // Schwarz factors follow the geometry (recomputed at the next screened call)
-    return upload_geometry(p);
+    rc = upload_geometry(p);
+    if (rc) return rc;
+    // primitive cuts, segments and work lists follow the geometry as well
+    clear_bra_cache(p);
+    return build_ket(p);
 }
 
 int ccsd_plan_nao(const ccsd_plan* p, int cart)

@@ -77,6 +77,11 @@ struct EriClassArgs {
     const long long* mol_dm_off; // offset of molecule m's (nao_m x nao_m) block in dm / pm
     const int* mol_ao0;          // first Cartesian AO of molecule m
     const int* mol_nao;          // Cartesian AOs of molecule m
+    // ket primitive pairs past the Gaussian-product cut (register kernels): entries
+    // [ket_pp_off[kq], ket_pp_off[kq + 1]) of ket pair kq, 10 doubles each: a_kl, Q (3),
+    // K_cd / a_kl, 0.5 / a_kl, c_k c_l of up to 4 contraction-column combinations
+    const int* ket_pp_off;
+    const double* ket_pp;
 };
 
 #ifdef __HIPCC__

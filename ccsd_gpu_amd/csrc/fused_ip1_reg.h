@@ -179,9 +179,15 @@ __device__ __forceinline__ void factor1d(double (&G)[LI + 2][LJ + 1][LK + 1][LL 
 // column coefficient products c_i c_j (up to 4).
 constexpr int kBraFields = 11;
 
+// Primitive pairs with a_1 a_2 / (a_1 + a_2) |R_12|^2 above this (Gaussian-product
+// factor below e^-50 = 2e-22) contribute nothing at double precision and are skipped.
+constexpr double kPrimExpCut = 50.0;
+
+// Fills the wave's table with the bra primitive pairs that pass the cut, compacted
+// (ballot prefix sums); returns their number (uniform over the wave).
 template <int MIJ>
-__device__ __forceinline__ void fill_bra(const EriClassArgs& a, const ShellInfo& si, const ShellInfo& sj,
-                                         const double* RA, const double* AB, double norm, int lane, double* tab)
+__device__ __forceinline__ int fill_bra(const EriClassArgs& a, const ShellInfo& si, const ShellInfo& sj,
+                                        const double* RA, const double* AB, double norm, int lane, double* tab)
 {
     const int nij = si.nprim * sj.nprim;
     const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
@@ -190,15 +196,31 @@ __device__ __forceinline__ void fill_bra(const EriClassArgs& a, const ShellInfo&
     const double* cfi = a.env + si.pcoef;
     const double* cfj = a.env + sj.pcoef;
     const int ncij = si.nctr * sj.nctr;
-    for (int p = lane; p < nij; p += 64) {
-        const int ip = p / sj.nprim, jp = p - ip * sj.nprim;
-        const double ai = ei[ip], aj = ej[jp], aij = ai + aj, inv = 1.0 / aij;
-        double* e = tab + p * kBraFields;
+    int count = 0;
+    for (int p0 = 0; p0 < nij; p0 += 64) {
+        const int p = p0 + lane;
+        int ip = 0, jp = 0;
+        double ai = 0.0, aj = 0.0, aij = 1.0, inv = 1.0, mu = 0.0;
+        if (p < nij) {
+            ip = p / sj.nprim;
+            jp = p - ip * sj.nprim;
+            ai = ei[ip];
+            aj = ej[jp];
+            aij = ai + aj;
+            inv = 1.0 / aij;
+            mu = ai * aj * inv * rab2;
+        }
+        const bool keep = p < nij && mu <= kPrimExpCut;
+        const unsigned long long mask = __ballot(keep);
+        const int slot = count + __popcll(mask & ((1ull << lane) - 1ull));
+        count += __popcll(mask);
+        if (!keep) continue;
+        double* e = tab + slot * kBraFields;
         e[0] = aij;
         e[1] = RA[0] + AB[0] * (aj * inv);
         e[2] = RA[1] + AB[1] * (aj * inv);
         e[3] = RA[2] + AB[2] * (aj * inv);
-        e[4] = norm * exp(-ai * aj * inv * rab2) * inv;
+        e[4] = norm * exp(-mu) * inv;
         e[5] = 2.0 * ai;
         e[6] = 0.5 * inv;
 #pragma unroll
@@ -207,12 +229,13 @@ __device__ __forceinline__ void fill_bra(const EriClassArgs& a, const ShellInfo&
             e[7 + cij] = cij < ncij ? cfi[c_i * si.nprim + ip] * cfj[c_j * sj.nprim + jp] : 0.0;
         }
     }
+    return count;
 }
 
 // Part Q of P of the quartet handled by this lane; adds to the lane's three sums.
 template <int LI, int LJ, int LK, int LL, int GC, int P, int Q>
 __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 ch, int lane, const double* tab,
-                                             double& sx, double& sy, double& sz)
+                                             int nij, double& sx, double& sy, double& sz)
 {
     constexpr int NFI = ncart_c(LI), NFJ = ncart_c(LJ), NFK = ncart_c(LK), NFL = ncart_c(LL);
     constexpr int NF = NFI * NFJ * NFK * NFL;
@@ -237,14 +260,8 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
     const double* RD = a.coords + 3 * sl.atom;
     const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
     const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
-    const double rcd2 = CD[0] * CD[0] + CD[1] * CD[1] + CD[2] * CD[2];
-    const int nij = si.nprim * sj.nprim;
     const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
     const bool same_kl = kp.x == kp.y;
-    const double* ek = a.env + sk.pexp;
-    const double* el = a.env + sl.pexp;
-    const double* cfk = a.env + sk.pcoef;
-    const double* cfl = a.env + sl.pcoef;
 
     // Gamma of this part for every contraction-column combination, loaded once
     double gam[MIJ][MKL][NFP];
@@ -271,21 +288,18 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
                 gam[cij][ckl][t] = g;
             }
 
-    for (int kp2 = 0; kp2 < sk.nprim; kp2++)
-        for (int lp = 0; lp < sl.nprim; lp++) {
-            // ket primitive pair: once per lane
-            const double ak = ek[kp2], al = el[lp], akl = ak + al, inv_kl = 1.0 / akl;
-            const double KcdN = exp(-ak * al * inv_kl * rcd2) * inv_kl;
-            const double Qc[3] = {RC[0] + CD[0] * (al * inv_kl), RC[1] + CD[1] * (al * inv_kl),
-                                  RC[2] + CD[2] * (al * inv_kl)};
+    // ket primitive pairs past the cut, precomputed per ket pair (no divergent skips)
+    const int t1 = a.ket_pp_off[kq + 1];
+    for (int t = a.ket_pp_off[kq]; t < t1; t++) {
+            const double* kpd = a.ket_pp + (size_t)t * 10;
+            const double akl = kpd[0];
+            const double Qc[3] = {kpd[1], kpd[2], kpd[3]};
+            const double KcdN = kpd[4];
+            const double hkl = kpd[5];
             const double QC[3] = {Qc[0] - RC[0], Qc[1] - RC[1], Qc[2] - RC[2]};
-            const double hkl = 0.5 * inv_kl;
             double ck[MKL];
 #pragma unroll
-            for (int ckl = 0; ckl < MKL; ckl++) {
-                const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
-                ck[ckl] = ckl < nckl ? cfk[c_k * sk.nprim + kp2] * cfl[c_l * sl.nprim + lp] : 0.0;
-            }
+            for (int ckl = 0; ckl < MKL; ckl++) ck[ckl] = kpd[6 + ckl];
             for (int p = 0; p < nij; p++) {
                 // bra primitive pair: broadcast reads of the wave's table
                 const double* e = tab + p * kBraFields;
@@ -364,9 +378,9 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
 
 template <int LI, int LJ, int LK, int LL, int GC, int P, int... Q>
 __device__ __forceinline__ void run_parts(const EriClassArgs& a, const int4 ch, int lane, int part, const double* tab,
-                                          double& sx, double& sy, double& sz, std::integer_sequence<int, Q...>)
+                                          int nij, double& sx, double& sy, double& sz, std::integer_sequence<int, Q...>)
 {
-    ((part == Q ? quartet_part<LI, LJ, LK, LL, GC, P, Q>(a, ch, lane, tab, sx, sy, sz) : void()), ...);
+    ((part == Q ? quartet_part<LI, LJ, LK, LL, GC, P, Q>(a, ch, lane, tab, nij, sx, sy, sz) : void()), ...);
 }
 
 // Dynamic LDS: one bra table of max_nij pairs per wave of the block.
@@ -387,18 +401,20 @@ __global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
         double* tab = lds_tab + (size_t)wave * a.max_nij * kBraFields;
         const int2 bp = a.bra[ch.x];
         const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+        int nij = 0;
         if (active) {
             const double* RA = a.coords + 3 * si.atom;
             const double* RB = a.coords + 3 * sj.atom;
             const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
             const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
                                 shell_norm_dev(LL);
-            fill_bra<MIJ>(a, si, sj, RA, AB, norm, lane, tab);
+            nij = fill_bra<MIJ>(a, si, sj, RA, AB, norm, lane, tab);
         }
         __syncthreads();
         if (!active) return;
         double sx = 0.0, sy = 0.0, sz = 0.0;
-        run_parts<LI, LJ, LK, LL, GC, P>(a, ch, lane, part, tab, sx, sy, sz, std::make_integer_sequence<int, P>{});
+        run_parts<LI, LJ, LK, LL, GC, P>(a, ch, lane, part, tab, nij, sx, sy, sz,
+                                         std::make_integer_sequence<int, P>{});
         // wave reduction (the chunk's bra pair, hence the atom of i, is shared by the wave)
         for (int off = 32; off > 0; off >>= 1) {
             sx += __shfl_down(sx, off, 64);

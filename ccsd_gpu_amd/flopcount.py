@@ -18,7 +18,9 @@ and c contraction-column combinations, FP64 operations are counted as
 restate the reference kernel's arithmetic (ccsd_gpu/cuda/int2e_ip1_module.py:409-629).
 Quartet counts: this library evaluates contracted shells (general contractions
 once per primitive set); the reference decontracts (ccsd_gpu/cuda/int2e_ip1_gpu.py:43-79)
-and enumerates nbas_dc^2 * nbas_dc (nbas_dc+1)/2 quartets.
+and enumerates nbas_dc^2 * nbas_dc (nbas_dc+1)/2 quartets. K counts only the
+primitive pairs with a1 a2/(a1+a2) R^2 <= 50 (the kernels skip the others: their
+Gaussian-product factor is below e^-50).
 """
 from collections import defaultdict
 
@@ -43,18 +45,42 @@ def class_flops(li, lj, lk, ll, deriv, nroots=None):
     return base, per_combo
 
 
-def eri_work(mol, deriv, ish0=0, ish1=None):
-    """(flops, quartets, reference-equivalent quartets) of one s2kl pass over i-shells [ish0, ish1)."""
+PRIM_EXP_CUT = 50.0   # primitive pairs with a1 a2/(a1+a2) R^2 above this are skipped by the kernels
+
+
+def pair_prims(mol, cut=PRIM_EXP_CUT):
+    """(nbas, nbas) number of primitive pairs of each shell pair that pass the
+    Gaussian-product cut (what the fused kernels evaluate)."""
+    import numpy as np
+    bas, env = mol._bas, mol._env
+    coords = np.asarray(mol.atom_coords())
+    exps = [np.asarray(env[b[5]:b[5] + b[2]]) for b in bas]
+    n = mol.nbas
+    out = np.zeros((n, n), dtype=np.int64)
+    for i in range(n):
+        for j in range(n):
+            r2 = float(np.sum((coords[bas[i, 0]] - coords[bas[j, 0]]) ** 2))
+            a, b = exps[i][:, None], exps[j][None, :]
+            out[i, j] = int(np.count_nonzero(a * b / (a + b) * r2 <= cut))
+    return out
+
+
+def eri_work(mol, deriv, ish0=0, ish1=None, prim_cut=True):
+    """(flops, quartets, reference-equivalent quartets) of one s2kl pass over i-shells [ish0, ish1).
+
+    With prim_cut the primitive pairs below the Gaussian-product cut (skipped by the
+    fused kernels) are not counted."""
     ish1 = mol.nbas if ish1 is None else ish1
     bas = mol._bas
     l = bas[:, 1]
     nprim = bas[:, 2]
     nctr = bas[:, 3]
+    kp = pair_prims(mol) if prim_cut else None
     bra = defaultdict(lambda: [0, 0, 0])   # (li, lj) -> [count, sum K_ij, sum K_ij*c_ij]
     for i in range(ish0, ish1):
         for j in range(mol.nbas):
             e = bra[(int(l[i]), int(l[j]))]
-            kij = int(nprim[i] * nprim[j])
+            kij = int(kp[i, j]) if prim_cut else int(nprim[i] * nprim[j])
             e[0] += 1
             e[1] += kij
             e[2] += kij * int(nctr[i] * nctr[j])
@@ -62,7 +88,7 @@ def eri_work(mol, deriv, ish0=0, ish1=None):
     for k in range(mol.nbas):
         for m in range(k + 1):
             e = ket[(int(l[k]), int(l[m]))]
-            kkl = int(nprim[k] * nprim[m])
+            kkl = int(kp[k, m]) if prim_cut else int(nprim[k] * nprim[m])
             e[0] += 1
             e[1] += kkl
             e[2] += kkl * int(nctr[k] * nctr[m])
@@ -81,12 +107,13 @@ def eri_work(mol, deriv, ish0=0, ish1=None):
 
 def eri_work_screened(mol, deriv, q=None, thr=0.0):
     """(flops, quartets) of a full fused pass over `mol` keeping the quartets with
-    q[i, j] * q[k, l] >= thr (every quartet when q is None or thr <= 0)."""
+    q[i, j] * q[k, l] >= thr (every quartet when q is None or thr <= 0), primitive
+    pairs counted past the Gaussian-product cut."""
     import numpy as np
     bas = mol._bas
     n = mol.nbas
     l = bas[:, 1].astype(np.int64)
-    K = (bas[:, 2]).astype(np.int64)
+    KP = pair_prims(mol).astype(np.float64)       # primitive pairs past the cut
     C = (bas[:, 3]).astype(np.int64)
     q = np.ones((n, n)) if q is None or thr <= 0 else np.asarray(q, dtype=np.float64)
     thr = thr if thr > 0 else 0.0
@@ -101,7 +128,7 @@ def eri_work_screened(mol, deriv, q=None, thr=0.0):
             if not sb.any():
                 continue
             qb = q[bi[sb], bj[sb]]
-            kb = (K[bi[sb]] * K[bj[sb]]).astype(np.float64)
+            kb = KP[bi[sb], bj[sb]]
             cb = (C[bi[sb]] * C[bj[sb]]).astype(np.float64)
             for lk in range(int(l.max()) + 1):
                 for ll in range(int(l.max()) + 1):
@@ -111,7 +138,7 @@ def eri_work_screened(mol, deriv, q=None, thr=0.0):
                     qk = q[kk[sk], mm[sk]]
                     order = np.argsort(-qk, kind="stable")
                     qs = qk[order]
-                    kq = (K[kk[sk]] * K[mm[sk]]).astype(np.float64)[order]
+                    kq = KP[kk[sk], mm[sk]][order]
                     kc = kq * (C[kk[sk]] * C[mm[sk]]).astype(np.float64)[order]
                     cum_k = np.concatenate([[0.0], np.cumsum(kq)])
                     cum_kc = np.concatenate([[0.0], np.cumsum(kc)])
