@@ -158,6 +158,12 @@ struct ccsd_plan {
     std::vector<std::array<long long, 10>> last_launches;   // see ccsd_plan_last_launches
     double* d_de_scratch = nullptr;                  // autotuner's throw-away gradient
     std::vector<double> coords_h;                    // natm x 3
+    // side streams (default priority) that run independent class launches of a fused pass
+    // concurrently; forked from and joined back to the caller's stream with events
+    static constexpr int kSideStreams = 4;
+    hipStream_t side[kSideStreams] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
+    bool concurrent = true;
     std::vector<int> eff;                            // surviving primitive pairs of same-molecule shell pairs
     std::vector<long long> mol_eff_off;
     std::vector<C2sTermHost> c2s;                    // per spherical AO
@@ -698,12 +704,45 @@ int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long l
 
 // Fused gradient over the bra pairs of `bs`: explicit packed gamma (mode 0) or the
 // separable form from dm / pm (mode 1).
+int ensure_side_streams(ccsd_plan* p)
+{
+    if (p->fork_ev) return CCSD_OK;
+    for (int i = 0; i < ccsd_plan::kSideStreams; i++) {
+        HIP_TRY(hipStreamCreateWithFlags(&p->side[i], hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&p->join_ev[i], hipEventDisableTiming));
+    }
+    HIP_TRY(hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming));
+    return CCSD_OK;
+}
+
 int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
               const double* gamma, const double* dm, const double* pm, hipStream_t stream)
 {
     const bool screened = p->thr > 0.0;
     p->last_quartets = 0;
     p->last_launches.clear();
+    // kernel choices first (the autotuner synchronises), then the launches, spread over
+    // the side streams: they only add into de with atomics, so their order is free
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
+            a.gamma_mode = gamma_mode;
+            a.gamma = gamma;
+            a.dm = dm;
+            a.pm = pm;
+            a.gamma_scale = scale;
+            Choice c;
+            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c);
+            if (rc) return rc;
+        }
+    const bool conc = p->concurrent;
+    if (conc) {
+        int rc = ensure_side_streams(p);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(p->fork_ev, stream));
+        for (int i = 0; i < ccsd_plan::kSideStreams; i++) HIP_TRY(hipStreamWaitEvent(p->side[i], p->fork_ev, 0));
+    }
+    int nl = 0;
     for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
@@ -721,10 +760,19 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             a.chunks = cl->d;
             a.nchunks = cl->n;
             a.de = de;
-            rc = launch_choice(a, c, stream);
+            if (cl->n == 0) continue;
+            hipStream_t st = conc ? p->side[nl % ccsd_plan::kSideStreams] : stream;
+            nl++;
+            rc = launch_choice(a, c, st);
             if (rc) return rc;
             p->last_quartets += cl->quartets;
             record_launch(p, a, c, cl);
+        }
+    }
+    if (conc) {
+        for (int i = 0; i < ccsd_plan::kSideStreams; i++) {
+            HIP_TRY(hipEventRecord(p->join_ev[i], p->side[i]));
+            HIP_TRY(hipStreamWaitEvent(stream, p->join_ev[i], 0));
         }
     }
     return CCSD_OK;
@@ -948,6 +996,11 @@ int ccsd_plan_destroy_impl(ccsd_plan* p)
     if (p->d_mol_nao) (void)hipFree(p->d_mol_nao);
     if (p->d_mol_dm_off) (void)hipFree(p->d_mol_dm_off);
     if (p->d_de_scratch) (void)hipFree(p->d_de_scratch);
+    for (int i = 0; i < ccsd_plan::kSideStreams; i++) {
+        if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
+        if (p->join_ev[i]) (void)hipEventDestroy(p->join_ev[i]);
+    }
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
     if (p->d_shells) (void)hipFree(p->d_shells);
     if (p->d_env) (void)hipFree(p->d_env);
     if (p->d_coords) (void)hipFree(p->d_coords);
@@ -1213,6 +1266,10 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         p->autotune = value != 0;
         clear_bra_cache(p);
         p->choice.clear();
+        return CCSD_OK;
+    }
+    if (option == 2) {
+        p->concurrent = value != 0;
         return CCSD_OK;
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");

@@ -67,7 +67,9 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * register-resident per-class kernels where a class fits, 0 forces the LDS
  * kernel for every class (comparison and testing). option 1: 1 (default) times
  * the kernel candidates of each class once per plan and keeps the fastest;
- * 0 takes the first candidate (register kernel, 4 waves per block). */
+ * 0 takes the first candidate (register kernel, 4 waves per block). option 2:
+ * 1 (default) runs the class launches of a fused pass on four side streams
+ * (forked from and joined back to `stream` with events), 0 on `stream` alone. */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
