@@ -335,21 +335,19 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     dm1_tot = dm1 + hf_dm1
     mocc = mo[:, :nocc]
     zeta = zeta + (mocc * (mo_energy[:nocc] * 2.0)) @ mocc.T
+    # every term is a sum over the AO rows i of atom A (the reference's per-atom loop,
+    # gpu_ccsd_grad.py:554-583): form all row sums at once, then add them up by atom
     offs = mol.offset_nr_by_atom()
+    ao_atom = torch.as_tensor(np.repeat(np.arange(mol.natm), offs[:, 3] - offs[:, 2]), device=DEV)
+    sym_dm = dm1_tot + dm1_tot.T
+    w = im1 + im1.T - zeta - zeta.T - 2.0 * vhf_s1occ
+    rows = torch.einsum("xij,ij->ix", s1, w) + torch.einsum("xij,ij->ix", h1, sym_dm)   # (nao, 3)
+    de_all = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
+    de_all.index_add_(0, ao_atom, rows)
+    de_all += torch.einsum("axij,ij->ax", vrinv_all, sym_dm)
+    de_all += de2
     atoms = range(mol.natm) if atmlst is None else list(atmlst)
-    de = torch.zeros((len(atoms), 3), dtype=torch.float64, device=DEV)
-    for kk, ia in enumerate(atoms):
-        p0, p1_ = int(offs[ia, 2]), int(offs[ia, 3])
-        vrinv = vrinv_all[ia].clone()
-        vrinv[:, p0:p1_] += h1[:, p0:p1_]
-        h1ao = vrinv + vrinv.transpose(1, 2)
-        de[kk] += E("xij,ij->x", s1[:, p0:p1_], im1[p0:p1_])
-        de[kk] += E("xji,ij->x", s1[:, p0:p1_], im1[:, p0:p1_])
-        de[kk] += E("xij,ji->x", h1ao, dm1_tot)
-        de[kk] -= E("xij,ij->x", s1[:, p0:p1_], zeta[p0:p1_])
-        de[kk] -= E("xji,ij->x", s1[:, p0:p1_], zeta[:, p0:p1_])
-        de[kk] -= E("xij,ij->x", s1[:, p0:p1_], vhf_s1occ[p0:p1_]) * 2
-        de[kk] += de2[ia]
+    de = de_all[list(atoms)]
     t0 = mark("onee", t0)
     return de.cpu().numpy()
 
