@@ -2,9 +2,10 @@
 //
 // The reference evaluates these on the CPU through PySCF (hcore_generator,
 // get_ovlp, int1e_ipovlp / ipkin / ipnuc / iprinv at ccsd_gpu/gpu_ccsd_grad.py:554-583).
-// One thread per ordered shell pair writes its Cartesian block: overlap and
-// kinetic by 1-D Obara-Saika tables, nuclear attraction by Rys quadrature with
-// point charges (the ERI tables, n <= 5 for l <= 4 plus one derivative).
+// One thread per ordered shell pair writes its Cartesian block for overlap and
+// kinetic (1-D Obara-Saika tables); the nuclear-attraction kinds use one thread
+// per (shell pair, nucleus), Rys quadrature with point charges (the ERI tables,
+// n <= 5 for l <= 4 plus one derivative), kind 2 summing the nuclei atomically.
 //
 // kinds: 0 <i|j>  1 <i|T|j>  2 <i|V|j>  3 <d/dr i|j>  4 <d/dr i|T|j>
 //        5 <d/dr i| -Z_C/|r-C| |j> for every nucleus C (natm x 3 matrices)
@@ -65,9 +66,15 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                              const double* __restrict__ env, const double* __restrict__ coords,
                              const double* __restrict__ charges, int natm, int nbas, int nao)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nbas * nbas) return;
-    const int ish = t / nbas, jsh = t - ish * nbas;
+    // nuclear-attraction kinds: one thread per (shell pair, nucleus); others: per shell pair
+    const bool per_nucleus = kind == 2 || kind == 5;
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    const long long npair = (long long)nbas * nbas;
+    if (t >= (per_nucleus ? npair * natm : npair)) return;
+    const int pair = (int)(per_nucleus ? t / natm : t);
+    const int c_first = per_nucleus ? (int)(t - (long long)pair * natm) : 0;
+    const int c_last = per_nucleus ? c_first + 1 : natm;
+    const int ish = pair / nbas, jsh = pair - ish * nbas;
     const ShellInfo si = shells[ish], sj = shells[jsh];
     const int la = si.l, lb = sj.l;
     const int deriv = kind >= 3;
@@ -123,7 +130,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                             }
                     } else {
                         const int nroots = (lae + lb) / 2 + 1;
-                        for (int C = 0; C < natm; C++) {
+                        for (int C = c_first; C < c_last; C++) {
                             const double* Rc = coords + 3 * C;
                             double PC2 = 0.0;
                             for (int x = 0; x < 3; x++) PC2 += (P[x] - Rc[x]) * (P[x] - Rc[x]);
@@ -142,7 +149,8 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                         comp1(lb, fb, cb);
                                         const size_t e = (size_t)(i0 + fa) * nao + j0 + fb;
                                         if (!deriv) {
-                                            out[e] += I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]];
+                                            // the nuclei of one element are summed by different threads
+                                            atomicAdd(out + e, I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]]);
                                         } else {
                                             for (int c = 0; c < 3; c++) {
                                                 double f[3];
@@ -167,8 +175,10 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
 hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
                              const double* charges, int natm, int nbas, int nao, hipStream_t stream)
 {
-    const int n = nbas * nbas;
-    hipLaunchKernelGGL(int1e_kernel, dim3((n + 63) / 64), dim3(64), 0, stream, kind, out, shells, env, coords,
-                       charges, natm, nbas, nao);
+    const long long n = (long long)nbas * nbas * ((kind == 2 || kind == 5) ? natm : 1);
+    if (n <= 0) return hipSuccess;
+    if ((n + 63) / 64 > 2147483647LL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(int1e_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, kind, out, shells, env,
+                       coords, charges, natm, nbas, nao);
     return hipGetLastError();
 }
