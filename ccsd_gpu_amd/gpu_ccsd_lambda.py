@@ -362,7 +362,7 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
 
 
 def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle):
-    lp_tol = max(tol, getattr(mycc, "lambda_lp_tol", 1e-6 if precision == "fp32" else 1e-5))
+    lp_tol = max(tol, getattr(mycc, "lambda_lp_tol", 1e-6 if precision == "fp32" else 3e-6))
     _, l1, l2, normt, iters = fixed_point(l1, l2, precision, lp_tol, max_cycle)
     total, fp64_updates = iters, 0
     conv = False
@@ -383,10 +383,11 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
             break
         # correction: d = A d + r, to `lambda_corr_rtol` of the remaining residual in at most
         # `lambda_corr_max_cycle` sweeps: short rounds restart DIIS on a fresh FP64 residual
-        # while its convergence is still fast (measured on (H2O)8: 16 sweeps + 3 FP64
-        # updates with 0.03 / 4, against 27 + 2 with one long round to 1e-3)
-        rtol = getattr(mycc, "lambda_corr_rtol", 3e-2)
-        ncyc = getattr(mycc, "lambda_corr_max_cycle", 4)
+        # while its convergence is still fast (measured on (H2O)8 with the first phase to
+        # 3e-6: 15 sweeps + 2 FP64 updates with 0.01 / 5, against 27 + 2 with one long
+        # round to 1e-3)
+        rtol = getattr(mycc, "lambda_corr_rtol", 1e-2)
+        ncyc = getattr(mycc, "lambda_corr_max_cycle", 5)
         _, d1, d2, _, it = fixed_point(r1.clone(), r2.clone(), precision, max(normt * rtol, tol * 1e-2), ncyc,
                                        rhs=(r1, r2))
         total += it

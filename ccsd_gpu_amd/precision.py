@@ -216,14 +216,16 @@ def contract(eq, *ops, precision="fp64"):
 def vvvv_term(l2, vvvv, precision):
     """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update).
 
-    X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so the mixed path
-    evaluates only i <= j (half the o^2 v^4 work) and mirrors the rest."""
-    if precision == "fp64":
-        return contract("ijcd,acbd->ijab", l2, vvvv)
+    X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so only i <= j is
+    evaluated (half the o^2 v^4 work) and the rest mirrored, at every precision."""
     o, _, v, _ = l2.shape
     iu, ju = torch.triu_indices(o, o, device=l2.device)
     A = l2[iu, ju].reshape(1, -1, v * v)
-    if precision == "bf16x3":
+    if precision == "fp64":
+        # B[(a, b), (c, d)] = (ac|bd) is symmetric: X = A B (DGEMM on the cached matrix)
+        B = _cached(vvvv, ("vvvv-abcd-f64",), lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).contiguous())
+        X = A[0] @ B
+    elif precision == "bf16x3":
         B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
         X = mfma.gemm_nt(mfma.split(A), B)[0]
     else:
