@@ -213,11 +213,20 @@ class CCSD:
         return np.ones(self.nmo, dtype=bool)
 
     def ao2mo(self, mo_coeff=None):
-        if mo_coeff is None and self._eris is not None:
+        """MO integrals of the SCF orbitals, built once per orbital set: they are kept on
+        the SCF object (keyed by its mo_coeff object) so that every CCSD of the same
+        orbitals -- the T solve, the lambda solve, the gradient's Z-vector -- shares them."""
+        if mo_coeff is not None:
+            return ERIs(self._scf, mo_coeff)
+        if self._eris is not None:
             return self._eris
-        eris = ERIs(self._scf, mo_coeff)
-        if mo_coeff is None:
-            self._eris = eris
+        cache = getattr(self._scf, "_cc_eris", None)
+        if cache is not None and cache[0] is self._scf.mo_coeff:
+            self._eris = cache[1]
+            return self._eris
+        eris = ERIs(self._scf, None)
+        self._eris = eris
+        self._scf._cc_eris = (self._scf.mo_coeff, eris)
         return eris
 
     def amplitudes_to_vector(self, t1, t2):

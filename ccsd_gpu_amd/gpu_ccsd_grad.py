@@ -183,18 +183,23 @@ def _pack_halved(full):
 # Phase 4: Z-vector
 # ---------------------------------------------------------------------------
 def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
-    """Solve (e_a - e_i) x_ai + fvind(x)_ai = -Xvo_ai by preconditioned CG on the device."""
+    """Solve (e_a - e_i) x_ai + fvind(x)_ai = -Xvo_ai by preconditioned CG on the device.
+
+    fvind is the reference's 2 C_v^T (J - K/2)[D] C_o with D = C_v x C_o^T + transpose
+    (gpu_ccsd_grad.py:644-651). Its vo block needs only MO integrals the CCSD
+    already holds: 4 (ai|bj) x_bj - (ab|ij) x_bj - (aj|ib) x_bj, i.e. contractions
+    over ovov and oovv (o^2 v^2 each) instead of JK builds over the nao^4 AO tensor."""
     mf = mycc._scf
     nocc = mycc.nocc
     c = _dev(mycc.mo_coeff)
-    co, cv = c[:, :nocc], c[:, nocc:]
     e = _dev(mf.mo_energy)
     eai = e[nocc:, None] - e[None, :nocc]
+    eris = mycc.ao2mo()
+    ovov, oovv = eris.ovov, eris.oovv
 
     def fvind(x):
-        dm = cv @ x @ co.T
-        j, k = ints.get_jk(dm + dm.T)
-        return cv.T @ (j - 0.5 * k) @ co * 2
+        return (4.0 * contract("iajb,bj->ai", ovov, x) - contract("ijab,bj->ai", oovv, x)
+                - contract("jaib,bj->ai", ovov, x))
 
     def A(x):
         return eai * x + fvind(x)
