@@ -34,6 +34,7 @@ from .gpu_ccsd_lambda import _dev
 from .hip.grad2e import contract_int2e_ip1
 from .hip.int1e_gpu import int1e
 from .hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
+from . import mfma
 from .precision import contract
 from .scf import DEV
 
@@ -138,7 +139,7 @@ def _assemble_dm2_mo(d2):
     return dm2
 
 
-def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff):
+def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff, precision="fp64"):
     """Symmetrized AO 2-RDM, full (nao^4), in the AO basis of the columns of
     `mo_coeff`'s rows (Cartesian when mo_coeff = cart2sph @ C).
 
@@ -153,12 +154,35 @@ def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff):
     _lib_sym_pairs(dm2, 0.5)
     c = _dev(mo_coeff)
     nao = c.shape[0]
-    t = (c @ dm2.reshape(nmo, -1)).reshape(nao, nmo, nmo, nmo)                 # (i q r s)
-    del dm2
-    t = (t.reshape(-1, nmo) @ c.T).reshape(nao, nmo, nmo, nao)                 # (i q r l)
-    t = torch.matmul(c, t.reshape(nao, nmo, nmo * nao)).reshape(nao, nao, nmo, nao)   # (i j r l)
-    t = torch.matmul(c, t.reshape(nao * nao, nmo, nao))                        # (i j k l)
-    return t.reshape(nao, nao, nao, nao)
+    t = dm2                                               # (p q r s)
+    for _ in range(4):
+        # last index transformed and rotated to the front: (p q r s) -> (l p q r) -> ... -> (i j k l)
+        t = _rotate_transform(t, c, precision)
+    return t
+
+
+def _rotate_transform(X, c, precision):
+    """Y[l, r...] = sum_s X[r..., s] c[l, s] for a contiguous X: one GEMM with the
+    contracted index contiguous in both operands, the product written transposed
+    (the new index slowest), so four calls give the full C x C x C x C transform
+    with every operand contiguous and no permuted copy."""
+    K = X.shape[-1]
+    R = X.numel() // K
+    n = c.shape[0]
+    X2 = X.reshape(R, K)
+    if precision == "bf16x3":
+        out = torch.empty((n, R), dtype=torch.float64, device=X.device)
+        zero = torch.zeros(1, dtype=torch.int64, device=X.device)
+        rows = torch.arange(R, dtype=torch.int64, device=X.device)
+        cols = torch.arange(n, dtype=torch.int64, device=X.device) * R
+        mfma.gemm_nt_scatter(mfma.split(X2[None]), mfma.split(c[None]), out, zero, rows, cols, beta=0)
+    elif precision == "fp32":
+        out = mfma.bmm(X2[None], c.T[None], "fp32")[0].T.contiguous()
+    elif precision == "fp64":
+        out = c @ X2.T
+    else:
+        raise ValueError(f"unknown precision {precision!r}")
+    return out.reshape((n,) + tuple(X.shape[:-1]))
 
 
 def _lib_sym_pairs(t, scale):
@@ -247,7 +271,7 @@ def _separable_gamma(D, P):
 
 
 def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose=0, fused=True,
-                  precision="fp64", timings=None):
+                  precision="fp64", timings=None, mo2ao_precision="fp64"):
     """Electronic CCSD gradient dE/dR (natm, 3) as numpy, as ccsd_gpu/gpu_ccsd_grad.py:344."""
     tm = {} if timings is None else timings
 
@@ -278,7 +302,9 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     t0 = mark("rdm", t0)
 
     # Phase 2: Cartesian AO 2-RDM (symmetrized)
-    gam = _rdm2_mo2ao_gpu(mycc, d2, c2s @ mo)
+    # FP64 by default: a split-BF16 operand keeps 16 mantissa bits, which the
+    # nao^4 contraction with the derivative integrals turns into ~1e-7 Eh/Bohr
+    gam = _rdm2_mo2ao_gpu(mycc, d2, c2s @ mo, mo2ao_precision)
     del d2
     t0 = mark("mo2ao", t0)
 

@@ -191,6 +191,21 @@ def test_gradient_gpu_matches_oracle(gpu, h2o_pair, fused):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("precision,mo2ao,tol", [("bf16x3", "fp64", 1e-8), ("fp32", "fp64", 1e-7),
+                                                  ("bf16x3", "bf16x3", 1e-6), ("fp32", "fp32", 1e-6)])
+def test_gradient_mixed_precision_matches_oracle(gpu, h2o_pair, precision, mo2ao, tol):
+    """RDM contractions (and optionally the MO->AO transform) on the MFMA path."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import grad_elec_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    ref = chem.grad_elec(occ, occ.t1, occ.t2, l1, l2)
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    de = grad_elec_gpu(mycc, occ.t1, occ.t2, l1, l2, precision=precision, mo2ao_precision=mo2ao)
+    err = np.abs(de - ref).max()
+    print(precision, "mo2ao", mo2ao, "grad err", err, flush=True)
+    assert err < tol
+
+
+@pytest.mark.gpu
 def test_forces_entry_point_end_to_end(gpu):
     """RHF -> CCSD -> lambda -> compute_ml_ccsd_forces_gpu, all on the GPU, vs oracle."""
     from ccsd_gpu_amd import cc, scf

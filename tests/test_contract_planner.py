@@ -90,3 +90,20 @@ def test_k_splits_policy():
     assert k_splits(4096, 10 ** 6) == 1            # enough tiles already
     assert k_splits(1, 10 ** 6) == 976             # 1024 chunks wanted, limited by chunk >= 1024
     assert k_splits(4, 10 ** 7) == 256
+
+
+def test_rotated_one_index_transforms_give_full_mo_to_ao():
+    """Four rotate-transforms of a (p q r s) tensor equal C x C x C x C in (i j k l) order."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _rotate_transform
+    g = torch.Generator().manual_seed(3)
+    d = torch.randn(4, 4, 4, 4, dtype=torch.float64, generator=g)
+    c = torch.randn(6, 4, dtype=torch.float64, generator=g)
+    t = d
+    for _ in range(4):
+        t = _rotate_transform(t, c, "fp64")
+    ref = torch.einsum("ip,jq,kr,ls,pqrs->ijkl", c, c, c, c, d)
+    assert t.shape == (6, 6, 6, 6)
+    assert torch.allclose(t, ref, rtol=1e-12, atol=1e-12)
+    # one step: last index transformed, moved to the front
+    one = _rotate_transform(d, c, "fp64")
+    assert torch.allclose(one, torch.einsum("pqrs,ls->lpqr", d, c), rtol=1e-12, atol=1e-12)
