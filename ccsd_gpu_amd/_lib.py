@@ -44,6 +44,13 @@ _SIGNATURES = {
     "ccsd_grad_ip1_gamma_sep": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
                                         c_void_p]),
     "ccsd_sym_pairs": (c_int, [c_void_p, c_int, c_double, c_void_p]),
+    "ccsd_permute": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ccsd_gemv": (c_int, [c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, c_void_p,
+                          ctypes.c_longlong, c_void_p, c_double, c_int, c_void_p, c_int, c_void_p]),
+    "ccsd_l1_terms": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_double,
+                              c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+    "ccsd_pair_combine": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_double, c_double,
+                                  c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "ccsd_gemm_bf16x3_nt_scatter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                             ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                             c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),

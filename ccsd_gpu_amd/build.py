@@ -18,7 +18,7 @@ OUT = os.path.join(HERE, "libccsd_gpu_amd.so")
 # (source, object name, extra defines)
 UNITS = [(s, s.replace(".hip", ""), []) for s in
          ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip",
-          "rdm_kernels.hip", "capi.hip"]]
+          "rdm_kernels.hip", "lambda_kernels.hip", "tensor_kernels.hip", "capi.hip"]]
 UNITS += [("fused_ip1_reg_tu.hip", f"fused_ip1_reg_l{i}{j}{k}",
            [f"-DCCSD_REG_LI={i}", f"-DCCSD_REG_LJ={j}", f"-DCCSD_REG_LK={k}"])
           for i in range(3) for j in range(3) for k in range(3)]

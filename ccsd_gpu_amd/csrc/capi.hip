@@ -40,6 +40,17 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
                                   hipStream_t stream);
 hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
+size_t ccsd_l1_terms_lds(int v);
+hipError_t ccsd_launch_permute(const double* src, int ndim, const long long* sizes, const long long* sstrides,
+                               double* dst, void* hi, void* lo, hipStream_t stream);
+hipError_t ccsd_launch_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x,
+                            long long incx, double* y, double alpha, int beta, double* ws, int S, hipStream_t stream);
+hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, long long sy0, long long sy1,
+                                double c1, double c2, double c3, double c4, double* ws, double* out,
+                                hipStream_t stream);
+hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o, int v, double c1, double c2,
+                                    double c3, double c4, const double* D, double* out, const long long* os, int beta,
+                                    hipStream_t stream);
 hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
                                   long long lda, long long ldb, long long ldc, long long sA, long long sB,
                                   long long sC, int batch, hipStream_t stream);
@@ -1178,6 +1189,61 @@ int ccsd_sym_pairs(double* t, int n, double scale, void* stream)
     if (n < 0 || n > 46340) return fail(CCSD_ERR_SHAPE, "bad dimension");
     hipError_t e = ccsd_launch_sym_pairs(t, n, scale, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "sym_pairs_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_l1_terms(const double* x, int o, int v, const double* y, long long sy0, long long sy1, double c1, double c2,
+                  double c3, double c4, double* ws, double* out, void* stream)
+{
+    if (!x || !y || !ws || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0 || (long long)o * o > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    if (ccsd_l1_terms_lds(v) > 65536) return fail(CCSD_ERR_UNSUPPORTED, "nvir too large for l1_block_kernel (<= 682)");
+    hipError_t e = ccsd_launch_l1_terms(x, o, v, y, sy0, sy1, c1, c2, c3, c4, ws, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "l1_block_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_pair_combine(const double* x, const long long* xstrides, int o, int v, double c1, double c2, double c3,
+                      double c4, const double* d, double* out, const long long* ostrides, int beta, void* stream)
+{
+    if (!x || !out || !xstrides || !ostrides) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0 || (beta != 0 && beta != 1)) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    const long long nt = (v + 31) / 32;
+    if ((long long)o * o * nt * nt > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "too many tiles for one launch");
+    hipError_t e = ccsd_launch_pair_combine(x, xstrides, o, v, c1, c2, c3, c4, d, out, ostrides, beta,
+                                            (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "pair_combine_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_permute(const double* src, int ndim, const long long* sizes, const long long* strides, double* dst, void* hi,
+                 void* lo, void* stream)
+{
+    if (!src || !sizes || !strides || (!dst && (!hi || !lo))) return fail(CCSD_ERR_NULL, "null pointer");
+    if (ndim < 1 || ndim > 6) return fail(CCSD_ERR_SHAPE, "ndim must be 1..6");
+    for (int k = 0; k < ndim; k++)
+        if (sizes[k] < 0) return fail(CCSD_ERR_SHAPE, "negative size");
+    hipError_t e = ccsd_launch_permute(src, ndim, sizes, strides, dst, hi, lo, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "permute_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x, long long incx,
+              double* y, double alpha, int beta, double* ws, int nslices, void* stream)
+{
+    if (!A || !x || !y) return fail(CCSD_ERR_NULL, "null pointer");
+    if (M < 0 || K < 0 || (beta != 0 && beta != 1) || nslices < 1 || (nslices > 1 && !ws) ||
+        lda < (cols ? M : K))
+        return fail(CCSD_ERR_SHAPE, "bad GEMV arguments");
+    if (K == 0) nslices = 1;   // the kernel writes beta * y (no memset: callers may capture this in a graph)
+    if ((cols ? (M + 255) / 256 * nslices : (M * nslices + 3) / 4) > 0x7fffffffLL)
+        return fail(CCSD_ERR_SHAPE, "GEMV too large for one launch");
+    hipError_t e = ccsd_launch_gemv(cols, A, M, K, lda, x, incx, y, alpha, beta, ws, nslices, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gemv_kernel");
+    if (nslices > 1) {
+        e = ccsd_launch_sum_slices(ws, y, M, nslices, beta, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "sum_slices_kernel");
+    }
     return CCSD_OK;
 }
 

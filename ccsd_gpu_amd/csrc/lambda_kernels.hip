@@ -1,0 +1,186 @@
+// Lambda-update helpers over (o, o, v, v) tensors (HBM-bound, FP64).
+//
+// l1_block_kernel + l1_reduce_kernel: the singles terms that contract a doubles-
+// shaped tensor X[i][j][a][b] with a singles-shaped vector y[j][b],
+//   out[i][a] += sum_{j,b} (c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a]) y[j,b],
+// in one read of X. Workgroup (p, q) reads the v x v block B = X[p][q] once and
+// forms B y_q and B^T y_q (the part of out[p]) and B y_p, B^T y_p (the part of
+// out[q]); the per-block parts go to a workspace that a second kernel sums in a
+// fixed order (no atomics: graph replays and eager calls agree bit for bit).
+//
+// pair_combine_kernel: for every (i, j) and 32 x 32 tile of (a, b),
+//   out[i,j,a,b] = beta * out + (c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a]) * D[i,j,a,b]
+// with the transposed blocks staged through LDS so that every global read and
+// write runs along b (coalesced when X's and out's b strides are 1); X and out
+// are any strided 4-index views (a permuted layout such as [i][a][j][b]), D
+// optional and contiguous.
+#include <hip/hip_runtime.h>
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double x)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+__global__ void __launch_bounds__(256) l1_block_kernel(const double* __restrict__ X, int o, int v,
+                                                       const double* __restrict__ y, long long sy0, long long sy1,
+                                                       double c1, double c2, double c3, double c4,
+                                                       double* __restrict__ ws)
+{
+    extern __shared__ double sh[];
+    double* yq = sh;             // y[q][:]
+    double* yp = yq + v;         // y[p][:]
+    double* rq = yp + v;         // (B y_q)[r]
+    double* rp = rq + v;         // (B y_p)[r]
+    double* cq = rp + v;         // per wave: partial (B^T y_q)[c]
+    double* cp = cq + 4 * v;     // per wave: partial (B^T y_p)[c]
+    const int p = blockIdx.x / o, q = blockIdx.x - (blockIdx.x / o) * o;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    for (int a = tid; a < v; a += 256) {
+        yq[a] = y[q * sy0 + a * sy1];
+        yp[a] = y[p * sy0 + a * sy1];
+    }
+    __syncthreads();
+    const double* B = X + ((long long)p * o + q) * v * v;
+    const bool rows = (c1 != 0.0) || (c3 != 0.0);
+    const bool cols = (c2 != 0.0) || (c4 != 0.0);
+    for (int c0 = 0; c0 < v; c0 += 64) {
+        const int c = c0 + lane;
+        const bool ok = c < v;
+        const double yqc = ok ? yq[c] : 0.0, ypc = ok ? yp[c] : 0.0;
+        double colq = 0.0, colp = 0.0;
+        for (int r = wave; r < v; r += 4) {
+            const double b = ok ? B[(long long)r * v + c] : 0.0;
+            if (cols) {
+                colq += b * yq[r];
+                colp += b * yp[r];
+            }
+            if (rows) {
+                const double sq = wave_sum(b * yqc);
+                const double sp = wave_sum(b * ypc);
+                if (lane == 0) {
+                    // row r belongs to this wave only; chunks run in order
+                    rq[r] = (c0 == 0) ? sq : rq[r] + sq;
+                    rp[r] = (c0 == 0) ? sp : rp[r] + sp;
+                }
+            }
+        }
+        if (ok) {
+            cq[wave * v + c] = colq;
+            cp[wave * v + c] = colp;
+        }
+    }
+    __syncthreads();
+    const long long nblk = (long long)o * o * v;
+    double* Wp = ws + ((long long)p * o + q) * v;          // part of out[p]
+    double* Wq = ws + nblk + ((long long)p * o + q) * v;   // part of out[q]
+    for (int a = tid; a < v; a += 256) {
+        double wp = 0.0, wq = 0.0;
+        if (rows) {
+            wp += c1 * rq[a];
+            wq += c3 * rp[a];
+        }
+        if (cols) {
+            const double sq = (cq[a] + cq[v + a]) + (cq[2 * v + a] + cq[3 * v + a]);
+            const double sp = (cp[a] + cp[v + a]) + (cp[2 * v + a] + cp[3 * v + a]);
+            wp += c2 * sq;
+            wq += c4 * sp;
+        }
+        Wp[a] = wp;
+        Wq[a] = wq;
+    }
+}
+
+__global__ void l1_reduce_kernel(const double* __restrict__ ws, int o, int v, double* __restrict__ out)
+{
+    const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (t >= (long long)o * v) return;
+    const int i = (int)(t / v), a = (int)(t - (t / v) * v);
+    const long long nblk = (long long)o * o * v;
+    double s = 0.0;
+    for (int q = 0; q < o; q++) s += ws[((long long)i * o + q) * v + a];
+    for (int p = 0; p < o; p++) s += ws[nblk + ((long long)p * o + i) * v + a];
+    out[t] += s;
+}
+
+constexpr int PT = 32;   // pair_combine tile edge
+
+__global__ void __launch_bounds__(256) pair_combine_kernel(const double* __restrict__ X, long long x0, long long x1,
+                                                           long long x2, long long x3, int o, int v, double c1,
+                                                           double c2, double c3, double c4,
+                                                           const double* __restrict__ D, double* __restrict__ out,
+                                                           long long s0, long long s1, long long s2, long long s3,
+                                                           int beta)
+{
+    __shared__ double tij[PT][PT + 1];   // X[i][j][b][a] for the tile, indexed [b - b0][a - a0]
+    __shared__ double tji[PT][PT + 1];   // X[j][i][b][a]
+    const int nt = (v + PT - 1) / PT;
+    const long long blk = blockIdx.x;
+    const int tile = (int)(blk % ((long long)nt * nt));
+    const long long ij = blk / ((long long)nt * nt);
+    const int i = (int)(ij / o), j = (int)(ij - (ij / o) * o);
+    const int a0 = (tile / nt) * PT, b0 = (tile - (tile / nt) * nt) * PT;
+    const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;   // 32 x 8
+    const long long vv = (long long)v * v;
+    const double* Xij = X + (long long)i * x0 + (long long)j * x1;
+    const double* Xji = X + (long long)j * x0 + (long long)i * x1;
+    if (c2 != 0.0 || c4 != 0.0) {
+#pragma unroll
+        for (int k = 0; k < PT; k += 8) {
+            const int b = b0 + ty + k, a = a0 + tx;
+            const bool ok = (b < v) && (a < v);
+            const long long e = (long long)b * x2 + (long long)a * x3;
+            tij[ty + k][tx] = (ok && c2 != 0.0) ? Xij[e] : 0.0;
+            tji[ty + k][tx] = (ok && c4 != 0.0) ? Xji[e] : 0.0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = a0 + ty + k, b = b0 + tx;
+        if (a < v && b < v) {
+            const long long e = (long long)a * x2 + (long long)b * x3;
+            double val = 0.0;
+            if (c1 != 0.0) val += c1 * Xij[e];
+            if (c3 != 0.0) val += c3 * Xji[e];
+            if (c2 != 0.0) val += c2 * tij[tx][ty + k];
+            if (c4 != 0.0) val += c4 * tji[tx][ty + k];
+            if (D) val *= D[((long long)i * o + j) * vv + (long long)a * v + b];
+            double* dst = out + (long long)i * s0 + (long long)j * s1 + (long long)a * s2 + (long long)b * s3;
+            *dst = beta ? *dst + val : val;
+        }
+    }
+}
+
+}  // namespace
+
+size_t ccsd_l1_terms_lds(int v) { return (size_t)12 * v * sizeof(double); }
+
+hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, long long sy0, long long sy1,
+                                double c1, double c2, double c3, double c4, double* ws, double* out,
+                                hipStream_t stream)
+{
+    if (o <= 0 || v <= 0) return hipSuccess;
+    hipLaunchKernelGGL(l1_block_kernel, dim3((unsigned)((long long)o * o)), dim3(256), ccsd_l1_terms_lds(v), stream,
+                       X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const long long n = (long long)o * v;
+    hipLaunchKernelGGL(l1_reduce_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, ws, o, v, out);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o, int v, double c1, double c2,
+                                    double c3, double c4, const double* D, double* out, const long long* os, int beta,
+                                    hipStream_t stream)
+{
+    if (o <= 0 || v <= 0) return hipSuccess;
+    const long long nt = (v + PT - 1) / PT;
+    const long long nb = (long long)o * o * nt * nt;
+    hipLaunchKernelGGL(pair_combine_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, xs[0], xs[1], xs[2], xs[3], o,
+                       v, c1, c2, c3, c4, D, out, os[0], os[1], os[2], os[3], beta);
+    return hipGetLastError();
+}

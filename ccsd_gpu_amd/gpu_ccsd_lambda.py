@@ -24,6 +24,7 @@ import numpy as np
 import torch
 
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
+from .pairops import l1_terms, pair_combine
 from .precision import contract, contract_acc, static_operands, vvvv_term
 from .scf import DEV
 
@@ -155,7 +156,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     if imds is None:
         imds = make_intermediates_gpu(mycc, t1, t2, eris, precision)
-    t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
+    t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2).contiguous()
     nocc, nvir = t1.shape
     fov = eris.fock[:nocc, nocc:]
     e_o = eris.mo_energy[:nocc]
@@ -176,14 +177,14 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
         l2new = vvvv_term(l2, eris.vvvv, precision)
     else:
         l2new = torch.zeros_like(l2)
-    l1new = C("ijab,jb->ia", l2new, t1) * 2 - C("jiab,jb->ia", l2new, t1)
+    l1new = l1_terms(torch.zeros_like(t1), l2new, t1, c1=2.0, c3=-1.0)
     l2new.mul_(0.5)
 
     w1 = imds.w1 - torch.diag(e_v)
     w2 = imds.w2 - torch.diag(e_o)
     l1new = l1new + inh * fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
-    l1new = l1new + C("ijab,bj->ia", l2, imds.w3) * 2 - C("ijba,bj->ia", l2, imds.w3)
+    l1new = l1_terms(l1new.contiguous(), l2, imds.w3.T, c1=2.0, c2=-1.0)
     l2new.add_(torch.einsum("ia,jb->ijab", l1, imds.w4))
     acc(l2new, "jibc,ca->jiba", l2, w1)
     acc(l2new, "jk,kiba->jiba", w2, l2, -1.0)
@@ -200,10 +201,9 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     ovvv = eris.ovvv
     l1new = l1new + C("iabc,bc->ia", ovvv, mba1) * 2 - C("ibca,bc->ia", ovvv, mba1)
     acc(l2new, "jbac,ic->jiba", ovvv, l1)
-    m4 = C("ijkd,kadb->ijab", l2t1, ovvv)
+    m4 = C("ijkd,kadb->ijab", l2t1, ovvv).contiguous()
     l2new.sub_(m4)
-    l1new = l1new - C("ijab,jb->ia", m4, t1) * 2 - C("ijab,ia->jb", m4, t1) * 2
-    l1new = l1new + C("jiab,jb->ia", m4, t1) + C("jiab,ia->jb", m4, t1)
+    l1new = l1_terms(l1new.contiguous(), m4, t1, c1=-2.0, c2=1.0, c3=1.0, c4=-2.0)
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
@@ -212,25 +212,26 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     acc(l2new, "bjic,ca->jiba", voov, mba1, -1.0)
     acc(l2new, "bjka,ik->jiba", voov, mij1, -1.0)
     l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
-    m4 = C("ijkl,aklb->ijab", l2tau, voov)
+    m4 = C("ijkl,aklb->ijab", l2tau, voov).contiguous()
     l2new.add_(m4, alpha=0.5)
-    l1new = l1new + C("ijab,jb->ia", m4, t1) * 2 - C("ijba,jb->ia", m4, t1)
+    l1new = l1_terms(l1new.contiguous(), m4, t1, c1=2.0, c2=-1.0)
 
     l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
     tmp_voov = imds.tmp_voov
-    tmp = l2.permute(0, 2, 1, 3) - l2.permute(0, 3, 1, 2) * 0.5
+    # tmp[i,a,j,b] = l2[i,j,a,b] - l2[i,j,b,a] / 2, stored [i][a][j][b]
+    tmp = torch.empty((nocc, nvir, nocc, nvir), dtype=l2.dtype, device=l2.device)
+    pair_combine(l2, c1=1.0, c2=-0.5, out=tmp.permute(0, 2, 1, 3))
     acc(l2new, "iakc,bjkc->jiba", tmp, tmp_voov)
     tmp = C("jkca,bikc->jiba", l2, imds.wvOOv)
-    l2new.add_(tmp)
-    l2new.add_(tmp.permute(1, 0, 2, 3), alpha=0.5)
+    pair_combine(tmp, c1=1.0, c3=0.5, out=l2new, beta=1)         # l2new += tmp + tmp[j,i,a,b] / 2
 
-    m3 = C("ijkl,klab->ijab", imds.woooo, l2)
+    m3 = C("ijkl,klab->ijab", imds.woooo, l2).contiguous()
     l2new.add_(m3, alpha=0.5)
-    l1new = l1new + C("ijab,jb->ia", m3, t1) * 2 - C("ijba,jb->ia", m3, t1)
+    l1new = l1_terms(l1new.contiguous(), m3, t1, c1=2.0, c2=-1.0)
 
     l1new = l1new / imds.eia
     # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
-    l2new = (l2new + l2new.permute(1, 0, 3, 2)) * imds.inv_d2
+    l2new = pair_combine(l2new, c1=1.0, c4=1.0, d=imds.inv_d2)
     return l1new, l2new
 
 

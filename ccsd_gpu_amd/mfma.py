@@ -48,18 +48,62 @@ class Split:
         return self.hi.shape
 
 
+def _permute(x, dst, hi, lo):
+    """The library's permute kernel: x (float64 strided view, ndim <= 6) -> contiguous
+    dst, or split-BF16 planes hi / lo."""
+    nd = max(1, x.dim())
+    sizes = (ctypes.c_longlong * nd)(*(list(x.shape) or [1]))
+    strides = (ctypes.c_longlong * nd)(*(list(x.stride()) or [1]))
+    _lib.check(_lib.load().ccsd_permute(_lib.ptr(x), nd, sizes, strides,
+                                        _lib.ptr(dst) if dst is not None else None,
+                                        _lib.ptr(hi) if hi is not None else None,
+                                        _lib.ptr(lo) if lo is not None else None, _lib.current_stream_ptr()),
+               "ccsd_permute")
+
+
+def permute_copy(x):
+    """Contiguous copy of a float64 strided view (any permutation; transposing copies
+    through LDS tiles). CPU tensors and views of more than six dimensions use torch."""
+    if not x.is_cuda or x.dtype != torch.float64 or x.dim() > 6 or x.dim() == 0:
+        return x.contiguous()
+    out = torch.empty(x.shape, dtype=torch.float64, device=x.device)
+    if out.numel():
+        _permute(x, out, None, None)
+    return out
+
+
 def split(x):
-    """(nb, rows, cols) float64 (any strides; made contiguous) -> Split."""
-    x = x.to(torch.float64).contiguous()
+    """(nb, rows, cols) float64 (any strides) -> Split (contiguous planes)."""
+    x = x.to(torch.float64)
     hi = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device)
     lo = torch.empty_like(hi)
-    n = x.numel()
-    if n:
-        cols = x.shape[-1]
-        rows = n // cols
-        _lib.check(_lib.load().ccsd_split_bf16(_lib.ptr(x), _lib.ptr(hi), _lib.ptr(lo), rows, cols, cols,
-                                                _lib.current_stream_ptr()), "ccsd_split_bf16")
+    if x.numel():
+        _permute(x, None, hi, lo)
     return Split(hi, lo)
+
+
+def gemv(A, x, y, alpha=1.0, beta=0):
+    """y (+)= alpha * A @ x in FP64 on the library's GEMV kernels: A (m, k) any
+    strides with one of them 1, x (k,) any stride, y (m,) contiguous. Returns y."""
+    m, k = A.shape
+    if A.stride(1) == 1 and m > 0:
+        cols, lda = 0, A.stride(0) if m > 1 else k
+    elif A.stride(0) == 1:
+        cols, lda = 1, A.stride(1) if k > 1 else m
+    else:
+        A = A.contiguous()
+        cols, lda = 0, k
+    if m == 0:
+        return y
+    if cols == 0:
+        S = max(1, min(-(-4096 // m), k // 1024))
+    else:
+        S = max(1, min(-(-262144 // m), k // 64))
+    ws = torch.empty(S * m, dtype=torch.float64, device=y.device) if S > 1 else None
+    _lib.check(_lib.load().ccsd_gemv(cols, _lib.ptr(A), m, k, max(lda, 1), _lib.ptr(x), x.stride(0) if k else 1,
+                                     _lib.ptr(y), float(alpha), int(beta), _lib.ptr(ws) if ws is not None else None,
+                                     S, _lib.current_stream_ptr()), "ccsd_gemv")
+    return y
 
 
 def sum_slices(ws, out, beta=0):
@@ -102,7 +146,9 @@ def split_view(x, shape):
     lo = torch.empty_like(hi)
     if hi.numel() != n:
         raise ValueError("shape does not match the view's size")
-    if n:
+    if n and 1 <= x.dim() <= 6:
+        _permute(x, None, hi, lo)        # planes in the view's row-major order
+    elif n:
         nd = max(1, x.dim())
         sizes = (ctypes.c_longlong * nd)(*(list(x.shape) or [1]))
         strides = (ctypes.c_longlong * nd)(*(list(x.stride()) or [1]))

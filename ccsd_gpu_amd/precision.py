@@ -70,7 +70,7 @@ def _view(X, s, g1, g2, g3, size):
     if s == "".join(g1 + g3 + g2) and X.is_contiguous():
         return X.reshape(n1, n3, n2).transpose(1, 2), False
     perm = [s.index(c) for c in g1 + g2 + g3]
-    return X.permute(perm).reshape(n1, n2, n3), True
+    return mfma.permute_copy(X.permute(perm)).reshape(n1, n2, n3), True
 
 
 def _order_within(group, s):
@@ -83,6 +83,16 @@ def _dgemm(Av, Bv):
     call, then a sum), otherwise such a product runs on a handful of workgroups."""
     nb, m, k = Av.shape
     n = Bv.shape[2]
+    if Av.is_cuda and (m == 1 or n == 1) and k > 0:
+        # matrix-vector shapes: the library's GEMV kernels (coalesced along the matrix's
+        # contiguous index, K sliced for parallelism) instead of a 128-wide GEMM tile
+        out = torch.empty((nb, m, n), dtype=Av.dtype, device=Av.device)
+        for b in range(nb):
+            if n == 1:
+                mfma.gemv(Av[b], Bv[b, :, 0], out[b, :, 0])
+            else:
+                mfma.gemv(Bv[b].T, Av[b, 0, :], out[b, 0, :])
+        return out
     s = mfma.k_splits(nb * -(-m // 64) * -(-n // 64), k, want=512, min_chunk=2048) if nb <= 8 else 1
     if s <= 1:
         return torch.mm(Av[0], Bv[0]).unsqueeze(0) if nb == 1 else torch.bmm(Av, Bv)
@@ -138,6 +148,10 @@ def _prepare(eq, A, B):
     for idx in [c for c in b if c not in a and c not in out]:
         B = B.sum(dim=b.index(idx))
         b = b.replace(idx, "")
+    # the output's fastest index goes to the GEMM's column side, so that the product
+    # (and the scatter epilogue) is written along it
+    if out and out[-1] in a and out[-1] not in b and any(c in out for c in b if c not in a):
+        A, B, a, b = B, A, b, a
     size = {**dict(zip(a, A.shape)), **dict(zip(b, B.shape))}
     batch, left, contr, right = _groups(a, b, out)
     batch = _order_within(batch, a)

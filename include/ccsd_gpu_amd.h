@@ -158,6 +158,47 @@ int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int
  * two permuted copies, ccsd_gpu/gpu_ccsd_grad.py:324-326). */
 int ccsd_sym_pairs(double* t, int n, double scale, void* stream);
 
+/* Singles terms of the lambda update that contract a doubles-shaped tensor
+ * x (o, o, v, v, contiguous) with a singles-shaped y (y[j][b] at j*sy0 + b*sy1):
+ *   out[i][a] += sum_{j,b} (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) y[j,b]
+ * in one read of x; ws is a device workspace of 2 o^2 v doubles, out (o, v)
+ * contiguous. Replaces the einsum pairs of ccsd_gpu/gpu_ccsd_lambda.py:244-245,
+ * 256-257, 288-291, 302-303, 322-323 (each of which reads its tensor twice or
+ * through a permuted copy). nvir <= 682. Deterministic (fixed summation order). */
+int ccsd_l1_terms(const double* x, int o, int v, const double* y, long long sy0, long long sy1, double c1, double c2,
+                  double c3, double c4, double* ws, double* out, void* stream);
+
+/* out[i,j,a,b] = beta * out + (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) * d[i,j,a,b]
+ * for x and out any strided (o, o, v, v) views (host arrays of the four element
+ * strides of i, j, a, b; reads and writes are coalesced when the b strides are
+ * 1; out must not overlap x), d optional (NULL = 1, else a contiguous
+ * (o, o, v, v) tensor). Pair transposes go through LDS tiles. Replaces the permuted
+ * elementwise forms of ccsd_gpu/gpu_ccsd_lambda.py:312 (l2 transposes),
+ * :315-317 (tmp + tmp^T) and :327-334 (symmetrise and divide by D_ijab). */
+int ccsd_pair_combine(const double* x, const long long* xstrides, int o, int v, double c1, double c2, double c3,
+                      double c4, const double* d, double* out, const long long* ostrides, int beta, void* stream);
+
+/* Copy a strided view into a contiguous destination: ndim <= 6 dimensions in the
+ * destination's order (sizes), with the source's element stride of each
+ * (strides; any permutation). Writes float64 to dst, or, when dst is NULL,
+ * the split-BF16 planes hi = bf16(x), lo = bf16(x - hi) (the operand form of
+ * ccsd_gemm_bf16x3_nt). Transposing copies go through 32 x 32 LDS tiles. The
+ * permuted copies of the reference's einsum operands (CuPy's
+ * .transpose(...).copy() / cuTENSOR, e.g. ccsd_gpu/gpu_ccsd_lambda.py:120, 162, 312)
+ * and the planner's operand splits. */
+int ccsd_permute(const double* src, int ndim, const long long* sizes, const long long* strides, double* dst, void* hi,
+                 void* lo, void* stream);
+
+/* y (+)= alpha * A x in FP64: cols = 0 reads A as M rows of length K
+ * (y[m] = sum_k A[m*lda + k] x[k*incx]), cols = 1 as K rows of length M
+ * (y[m] = sum_k A[k*lda + m] x[k*incx]); beta 0 overwrites, 1 accumulates.
+ * With nslices > 1 the K range is cut into slices whose partial sums go to ws
+ * (nslices * M doubles) and are added in a fixed order. The matrix-vector
+ * einsums of the lambda code (ccsd_gpu/gpu_ccsd_lambda.py:264-265, 280,
+ * 283-284, 294, 298-299) and of the Z-vector's orbital Hessian. */
+int ccsd_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x, long long incx,
+              double* y, double alpha, int beta, double* ws, int nslices, void* stream);
+
 /* C[b] (+)= A[b] B[b]^T from split-BF16 operands (A planes [batch][M][K] with row
  * stride lda, B planes [batch][N][K] with row stride ldb, both K-contiguous;
  * strides in elements), three bf16 MFMA products per tile

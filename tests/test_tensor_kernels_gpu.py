@@ -1,0 +1,88 @@
+"""Permute-copy / split and GEMV kernels of the contraction planner against torch."""
+import ctypes
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda")
+
+
+PERMS = [((7, 5, 33, 70), (0, 1, 2, 3)), ((7, 5, 33, 70), (0, 2, 1, 3)), ((7, 5, 33, 70), (0, 1, 3, 2)),
+         ((7, 5, 33, 70), (3, 2, 1, 0)), ((6, 40, 3, 45), (1, 3, 0, 2)), ((3, 4, 5, 6, 7), (4, 0, 3, 1, 2)),
+         ((130, 67), (1, 0)), ((1, 1, 9, 1), (3, 2, 1, 0))]
+
+
+@pytest.mark.parametrize("shape,perm", PERMS)
+def test_permute_copy_matches_torch(dev, shape, perm):
+    from ccsd_gpu_amd import mfma
+    g = torch.Generator(device=dev).manual_seed(sum(shape))
+    x = torch.randn(shape, dtype=torch.float64, device=dev, generator=g)
+    v = x.permute(perm)
+    got = mfma.permute_copy(v)
+    assert got.is_contiguous() and torch.equal(got, v.contiguous())
+
+
+def _split_ref(lib, v):
+    """Planes from the elementwise strided split kernel (reference for the tiled one)."""
+    from ccsd_gpu_amd import _lib
+    hi = torch.empty(v.shape, dtype=torch.bfloat16, device=v.device)
+    lo = torch.empty_like(hi)
+    nd = v.dim()
+    sizes = (ctypes.c_longlong * nd)(*v.shape)
+    strides = (ctypes.c_longlong * nd)(*v.stride())
+    _lib.check(lib.ccsd_split_bf16_strided(_lib.ptr(v), _lib.ptr(hi), _lib.ptr(lo), nd, sizes, strides,
+                                           _lib.current_stream_ptr()), "split")
+    return hi, lo
+
+
+@pytest.mark.parametrize("shape,perm", PERMS[:6])
+def test_split_through_permute_kernel_is_bitwise_the_split(dev, shape, perm):
+    from ccsd_gpu_amd import _lib, mfma
+    g = torch.Generator(device=dev).manual_seed(len(shape) + sum(perm))
+    v = torch.randn(shape, dtype=torch.float64, device=dev, generator=g).permute(perm)
+    n = v.numel()
+    s = mfma.split_view(v, (1, n // v.shape[-1], v.shape[-1]))
+    hi, lo = _split_ref(_lib.load(), v)
+    assert torch.equal(s.hi.reshape(-1).view(torch.int16), hi.reshape(-1).view(torch.int16))
+    assert torch.equal(s.lo.reshape(-1).view(torch.int16), lo.reshape(-1).view(torch.int16))
+    # hi + lo carries ~16 bits of x
+    x = v.contiguous().reshape(-1)
+    rec = s.hi.reshape(-1).double() + s.lo.reshape(-1).double()
+    assert float(((rec - x).abs() / x.abs().clamp_min(1e-30)).max()) < 2 ** -15
+
+
+@pytest.mark.parametrize("m,k", [(6080, 300), (3, 23104), (200, 1), (1, 5000), (777, 4099)])
+@pytest.mark.parametrize("layout", ["rows", "cols"])
+def test_gemv_matches_torch(dev, m, k, layout):
+    from ccsd_gpu_amd import mfma
+    g = torch.Generator(device=dev).manual_seed(m + k)
+    if layout == "rows":
+        A = torch.randn(m, k, dtype=torch.float64, device=dev, generator=g)
+    else:
+        A = torch.randn(k, m, dtype=torch.float64, device=dev, generator=g).T
+    x = torch.randn(2 * k, dtype=torch.float64, device=dev, generator=g)[::2]      # strided vector
+    y0 = torch.randn(m, dtype=torch.float64, device=dev, generator=g)
+    ref = A @ x
+    got = mfma.gemv(A, x, torch.empty(m, dtype=torch.float64, device=dev))
+    assert torch.allclose(got, ref, rtol=1e-12, atol=1e-10)
+    got2 = mfma.gemv(A, x, y0.clone(), alpha=-0.5, beta=1)
+    assert torch.allclose(got2, y0 - 0.5 * ref, rtol=1e-12, atol=1e-10)
+    assert torch.equal(mfma.gemv(A, x, torch.empty(m, dtype=torch.float64, device=dev)), got)   # deterministic
+
+
+@pytest.mark.parametrize("eq,shapes", [("iajb,bj->ai", [(9, 31, 9, 31), (31, 9)]),
+                                       ("jb,aijb->ia", [(9, 31), (31, 9, 9, 31)]),
+                                       ("iabc,bc->ia", [(4, 30, 30, 30), (30, 30)]),
+                                       ("ibca,bc->ia", [(4, 30, 30, 30), (30, 30)])])
+def test_planner_matrix_vector_contractions(dev, eq, shapes):
+    from ccsd_gpu_amd.precision import contract
+    g = torch.Generator(device=dev).manual_seed(len(eq))
+    ops = [torch.randn(s, dtype=torch.float64, device=dev, generator=g) for s in shapes]
+    assert torch.allclose(contract(eq, *ops), torch.einsum(eq, *ops), rtol=1e-12, atol=1e-11)
