@@ -175,6 +175,7 @@ struct ccsd_plan {
     hipStream_t side[kSideStreams] = {};
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
     bool concurrent = true;
+    bool ket_shell_order = false;                    // unscreened kets in shell order inside a primitive group
     std::vector<int> eff;                            // surviving primitive pairs of same-molecule shell pairs
     std::vector<long long> mol_eff_off;
     std::vector<C2sTermHost> c2s;                    // per spherical AO
@@ -236,8 +237,11 @@ double pair_q(const ccsd_plan* p, const int2& pr) { return p->q[(size_t)pr.x * p
 
 // Classes by (l1, l2, several contraction columns or not); inside a class:
 // molecule, then primitive group (descending), then Schwarz factor (descending)
-// when `by_q`.
-std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>& pairs, bool by_q)
+// when `by_q`, else surviving primitive pairs (descending) or, with
+// `shell_order`, the input (shell) order, which keeps the packed kl index of
+// neighbouring lanes close.
+std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>& pairs, bool by_q,
+                                    bool shell_order = false)
 {
     std::vector<PairClass> cls;
     std::map<std::tuple<int, int, int>, int> where;
@@ -258,12 +262,13 @@ std::vector<PairClass> make_classes(const ccsd_plan* p, const std::vector<int2>&
         c.max_nc = std::max(c.max_nc, a.nctr * b.nctr);
     }
     for (PairClass& c : cls) {
-        std::stable_sort(c.pairs.begin(), c.pairs.end(), [p, by_q](const int2& x, const int2& y) {
+        std::stable_sort(c.pairs.begin(), c.pairs.end(), [p, by_q, shell_order](const int2& x, const int2& y) {
             const int mx = p->shells[x.x].mol, my = p->shells[y.x].mol;
             if (mx != my) return mx < my;
             const int gx = prim_group(p, x), gy = prim_group(p, y);
             if (gx != gy) return gx > gy;
             if (by_q) return pair_q(p, x) > pair_q(p, y);
+            if (shell_order) return false;
             return pair_eff(p, x.x, x.y) > pair_eff(p, y.x, y.y);
         });
         for (size_t t = 0; t < c.pairs.size(); t++) {
@@ -362,7 +367,7 @@ int build_ket(ccsd_plan* p)
     for (int m = 0; m < p->nmol; m++)
         for (int k = p->mol_shell0[m]; k < p->mol_shell0[m + 1]; k++)
             for (int l = p->mol_shell0[m]; l <= k; l++) kpairs.push_back(make_int2(k, l));
-    p->ket = make_classes(p, kpairs, p->thr > 0.0 && p->q_valid);
+    p->ket = make_classes(p, kpairs, p->thr > 0.0 && p->q_valid, p->ket_shell_order);
     int rc = upload_classes(p->ket);
     if (rc) return rc;
     return upload_ket_prims(p);
@@ -1337,6 +1342,12 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
     if (option == 2) {
         p->concurrent = value != 0;
         return CCSD_OK;
+    }
+    if (option == 3) {
+        p->ket_shell_order = value != 0;
+        clear_bra_cache(p);
+        p->choice.clear();
+        return build_ket(p);
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");
 }

@@ -25,6 +25,10 @@ __device__ __forceinline__ double wave_sum(double x)
     return x;
 }
 
+// NC = number of 64-column chunks the kernel is compiled for (>= ceil(v / 64)); each
+// wave takes 4 of its rows per step and all their chunks, so 4 NC loads are in
+// flight before the row sums are reduced (one wave reduction per row and vector).
+template <int NC>
 __global__ void __launch_bounds__(256) l1_block_kernel(const double* __restrict__ X, int o, int v,
                                                        const double* __restrict__ y, long long sy0, long long sy1,
                                                        double c1, double c2, double c3, double c4,
@@ -47,30 +51,57 @@ __global__ void __launch_bounds__(256) l1_block_kernel(const double* __restrict_
     const double* B = X + ((long long)p * o + q) * v * v;
     const bool rows = (c1 != 0.0) || (c3 != 0.0);
     const bool cols = (c2 != 0.0) || (c4 != 0.0);
-    for (int c0 = 0; c0 < v; c0 += 64) {
-        const int c = c0 + lane;
-        const bool ok = c < v;
-        const double yqc = ok ? yq[c] : 0.0, ypc = ok ? yp[c] : 0.0;
-        double colq = 0.0, colp = 0.0;
-        for (int r = wave; r < v; r += 4) {
-            const double b = ok ? B[(long long)r * v + c] : 0.0;
-            if (cols) {
-                colq += b * yq[r];
-                colp += b * yp[r];
+    double colq[NC], colp[NC], yqc[NC], ypc[NC];
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++) {
+        const int c = ch * 64 + lane;
+        colq[ch] = 0.0;
+        colp[ch] = 0.0;
+        yqc[ch] = c < v ? yq[c] : 0.0;
+        ypc[ch] = c < v ? yp[c] : 0.0;
+    }
+    for (int r0 = wave; r0 < v; r0 += 16) {
+        double pq[4], pp[4], yr_q[4], yr_p[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int r = r0 + 4 * u;
+            pq[u] = 0.0;
+            pp[u] = 0.0;
+            yr_q[u] = r < v ? yq[r] : 0.0;
+            yr_p[u] = r < v ? yp[r] : 0.0;
+        }
+#pragma unroll
+        for (int ch = 0; ch < NC; ch++) {
+            const int c = ch * 64 + lane;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = r0 + 4 * u;
+                const double b = (c < v && r < v) ? B[(long long)r * v + c] : 0.0;
+                pq[u] += b * yqc[ch];
+                pp[u] += b * ypc[ch];
+                colq[ch] += b * yr_q[u];
+                colp[ch] += b * yr_p[u];
             }
-            if (rows) {
-                const double sq = wave_sum(b * yqc);
-                const double sp = wave_sum(b * ypc);
-                if (lane == 0) {
-                    // row r belongs to this wave only; chunks run in order
-                    rq[r] = (c0 == 0) ? sq : rq[r] + sq;
-                    rp[r] = (c0 == 0) ? sp : rp[r] + sp;
+        }
+        if (rows) {
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int r = r0 + 4 * u;
+                const double sq = wave_sum(pq[u]);
+                const double sp = wave_sum(pp[u]);
+                if (lane == 0 && r < v) {
+                    rq[r] = sq;
+                    rp[r] = sp;
                 }
             }
         }
-        if (ok) {
-            cq[wave * v + c] = colq;
-            cp[wave * v + c] = colp;
+    }
+#pragma unroll
+    for (int ch = 0; ch < NC; ch++) {
+        const int c = ch * 64 + lane;
+        if (c < v) {
+            cq[wave * v + c] = colq[ch];
+            cp[wave * v + c] = colp[ch];
         }
     }
     __syncthreads();
@@ -164,8 +195,25 @@ hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, 
                                 hipStream_t stream)
 {
     if (o <= 0 || v <= 0) return hipSuccess;
-    hipLaunchKernelGGL(l1_block_kernel, dim3((unsigned)((long long)o * o)), dim3(256), ccsd_l1_terms_lds(v), stream,
-                       X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    const int nch = (v + 63) / 64;
+    const dim3 grid((unsigned)((long long)o * o)), block(256);
+    const size_t lds = ccsd_l1_terms_lds(v);
+    if (nch <= 1)
+        hipLaunchKernelGGL(l1_block_kernel<1>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 2)
+        hipLaunchKernelGGL(l1_block_kernel<2>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 3)
+        hipLaunchKernelGGL(l1_block_kernel<3>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 4)
+        hipLaunchKernelGGL(l1_block_kernel<4>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 6)
+        hipLaunchKernelGGL(l1_block_kernel<6>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 8)
+        hipLaunchKernelGGL(l1_block_kernel<8>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else if (nch <= 11)
+        hipLaunchKernelGGL(l1_block_kernel<11>, grid, block, lds, stream, X, o, v, y, sy0, sy1, c1, c2, c3, c4, ws);
+    else
+        return hipErrorInvalidValue;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const long long n = (long long)o * v;

@@ -20,6 +20,8 @@ Differences of behaviour, on purpose:
   * use_cpu_vvvv=True includes the l2.vvvv term (computed on the GPU);
     use_cpu_vvvv=False omits it, as the reference does (:244-245).
 """
+import time
+
 import numpy as np
 import torch
 
@@ -142,13 +144,14 @@ def _static_pieces(imds, t1, t2, eris, level_shift):
 
 
 def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=True, precision="fp64",
-                      homogeneous=False):
+                      homogeneous=False, inh=None):
     """One lambda update l -> F(l) (the reference's update_lambda_gpu).
 
     The update is affine in (l1, l2): F(l) = A l + b. With homogeneous=True the
     l-independent terms b (fov, the bare voov term and the bare t1 inside
     mia1) are left out, giving A l -- used by the mixed-precision solver for
-    its correction equations."""
+    its correction equations. `inh`, a 0-d device tensor, weights b instead
+    (1: F, 0: A), read when the kernels run: one captured graph serves both."""
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
 
     def acc(out, eq, a, b, alpha=1.0):
@@ -161,7 +164,8 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     fov = eris.fock[:nocc, nocc:]
     e_o = eris.mo_energy[:nocc]
     e_v = eris.mo_energy[nocc:] + getattr(mycc, "level_shift", 0.0)
-    inh = 0.0 if homogeneous else 1.0
+    if inh is None:
+        inh = 0.0 if homogeneous else 1.0
 
     if not hasattr(imds, "theta"):
         _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0))
@@ -170,7 +174,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     mij = C("ikcd,jkcd->ij", l2, theta)
     mba1 = C("jc,jb->bc", l1, t1) + mba
     mij1 = C("kb,jb->kj", l1, t1) + mij
-    mia1 = inh * t1 + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
+    mia1 = t1 * inh + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
     mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
 
     if use_cpu_vvvv:
@@ -182,7 +186,7 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     w1 = imds.w1 - torch.diag(e_v)
     w2 = imds.w2 - torch.diag(e_o)
-    l1new = l1new + inh * fov + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
+    l1new = l1new + fov * inh + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
     l1new = l1_terms(l1new.contiguous(), l2, imds.w3.T, c1=2.0, c2=-1.0)
     l2new.add_(torch.einsum("ia,jb->ijab", l1, imds.w4))
@@ -207,7 +211,9 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
-    if inh:
+    if torch.is_tensor(inh):
+        l2new.addcmul_(imds.l2_const, inh)
+    elif inh:
         l2new.add_(imds.l2_const)
     acc(l2new, "bjic,ca->jiba", voov, mba1, -1.0)
     acc(l2new, "bjka,ik->jiba", voov, mij1, -1.0)
@@ -293,26 +299,33 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     max_cycle = mycc.max_cycle if max_cycle is None else max_cycle
     tol = mycc.conv_tol_normt if tol is None else tol
     nocc, nvir = t1.shape
+    phases = {}
+    mycc._lambda_phases = phases
+    tick = _Ticker(phases)
     # intermediates once in FP64: they enter every FP64 residual
     imds = make_intermediates_gpu(mycc, t1, t2, eris, "fp64")
+    tick("intermediates")
     diis_space = getattr(mycc, "diis_space", 6)
     use_diis = bool(getattr(mycc, "diis", True))
     # the mixed-precision sweeps replay a captured HIP graph of the update
     if use_graph is None:
         use_graph = bool(getattr(mycc, "lambda_graph", True)) and precision != "fp64"
     graphs = {}
+    inh_t = torch.ones((), dtype=torch.float64, device=t1.device)     # 1: F(l), 0: A l (graph input)
     history = []
     mycc._lambda_history = history
 
     def update(l1, l2, prec, homogeneous):
         if not use_graph or prec == "fp64":
             return update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec, homogeneous=homogeneous)
-        key = (prec, homogeneous)
-        if key not in graphs:
-            graphs[key] = GraphedUpdate(
-                lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec,
-                                               homogeneous=homogeneous), l1, l2).bind_first(l1, l2)
-        return graphs[key](l1, l2)
+        inh_t.fill_(0.0 if homogeneous else 1.0)
+        if prec not in graphs:
+            tick(None)
+            graphs[prec] = GraphedUpdate(
+                lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec, inh=inh_t),
+                l1, l2).bind_first(l1, l2)
+            tick("graph capture (incl. first update)")
+        return graphs[prec](l1, l2)
 
     def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
         """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
@@ -350,21 +363,41 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
                          imds.theta, imds.tau, imds.tmp_voov):
         if precision == "fp64":
             conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
+            tick("fp64 updates")
             mycc._lambda_iterations = iters
             mycc._lambda_fp64_updates = iters
             mycc._lambda_normt = normt
             return conv, l1, l2
         # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
         out = _mixed_solve(mycc, fixed_point, update_lambda_gpu, t1, t2, l1, l2, eris, imds, use_cpu_vvvv,
-                           precision, tol, max_cycle)
+                           precision, tol, max_cycle, tick)
         mycc._lambda_graph_replays = sum(g.replays for g in graphs.values())
         graphs.clear()
         return out
 
 
-def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle):
+class _Ticker:
+    """Wall time per solve phase (seconds, accumulated by name; synchronises the
+    device at each mark, a handful of times per solve)."""
+
+    def __init__(self, phases):
+        self.phases = phases
+        torch.cuda.synchronize()
+        self.t = time.perf_counter()
+
+    def __call__(self, name):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        if name is not None:
+            self.phases[name] = self.phases.get(name, 0.0) + t - self.t
+        self.t = t
+
+
+def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle,
+                 tick=lambda name: None):
     lp_tol = max(tol, getattr(mycc, "lambda_lp_tol", 1e-6 if precision == "fp32" else 3e-6))
     _, l1, l2, normt, iters = fixed_point(l1, l2, precision, lp_tol, max_cycle)
+    tick("low-precision sweeps")
     total, fp64_updates = iters, 0
     conv = False
     if not getattr(mycc, "lambda_refine", True):
@@ -376,6 +409,7 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
     for _round in range(max(8, max_cycle // 4)):
         r1, r2 = update(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
         fp64_updates += 1
+        tick("fp64 residuals")
         r1, r2 = r1 - l1, r2 - l2
         normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))
         mycc._lambda_history.append(("fp64-residual", False, normt))
@@ -393,6 +427,7 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
                                        rhs=(r1, r2))
         total += it
         l1, l2 = l1 + d1, l2 + d2
+        tick("correction sweeps")
     mycc._lambda_iterations = total
     mycc._lambda_fp64_updates = fp64_updates
     mycc._lambda_normt = normt

@@ -64,5 +64,6 @@ def ccsd_gradient(mf, mycc, mode="mixed", precision="bf16x3", lambda_tol=1e-8):
             "lambda_fp64_updates": getattr(mycc, "_lambda_fp64_updates", None),
             "lambda_converged": bool(mycc.converged_lambda),
             "lambda_graph_replays": getattr(mycc, "_lambda_graph_replays", 0) if mode != "plain" else 0,
+            "lambda_phases_s": {k: round(v, 4) for k, v in getattr(mycc, "_lambda_phases", {}).items()},
             "phases_s": timings}
     return np.asarray(de), info

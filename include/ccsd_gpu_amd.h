@@ -69,7 +69,10 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * the kernel candidates of each class once per plan and keeps the fastest;
  * 0 takes the first candidate (register kernel, 4 waves per block). option 2:
  * 1 (default) runs the class launches of a fused pass on four side streams
- * (forked from and joined back to `stream` with events), 0 on `stream` alone. */
+ * (forked from and joined back to `stream` with events), 0 on `stream` alone.
+ * option 3: 1 orders the (unscreened) ket pairs of each primitive group in
+ * shell order, so neighbouring lanes gather neighbouring packed Gamma; 0
+ * (default) by surviving primitive pairs, descending. */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices

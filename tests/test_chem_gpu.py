@@ -280,6 +280,35 @@ def test_lambda_graph_replays_match_eager(gpu, h2o_pair, precision):
 
 
 @pytest.mark.gpu
+def test_one_graph_serves_full_and_homogeneous_updates(gpu, h2o_pair):
+    """A graph captured with the device weight `inh` replays F(l) (inh = 1) and A l
+    (inh = 0) alternately, each equal to the eager update of that kind."""
+    from ccsd_gpu_amd.gpu_ccsd_lambda import GraphedUpdate, make_intermediates_gpu, update_lambda_gpu
+    from ccsd_gpu_amd.precision import static_operands
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    eris = mycc.ao2mo()
+    imds = make_intermediates_gpu(mycc, mycc.t1, mycc.t2, eris)
+    L1 = torch.as_tensor(l1, device="cuda")
+    L2 = torch.as_tensor(l2, device="cuda")
+    inh = torch.ones((), dtype=torch.float64, device="cuda")
+    with static_operands(mycc.t1, mycc.t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo):
+        g = GraphedUpdate(lambda a, b: update_lambda_gpu(mycc, mycc.t1, mycc.t2, a, b, eris, imds, True, "bf16x3",
+                                                         inh=inh), L1, L2)
+        for rep, homogeneous in enumerate((True, False, True, False)):
+            a, b = L1 * (1 + 0.1 * rep), L2 * (1 - 0.05 * rep)
+            r1, r2 = update_lambda_gpu(mycc, mycc.t1, mycc.t2, a, b, eris, imds, True, "bf16x3",
+                                       homogeneous=homogeneous)
+            inh.fill_(0.0 if homogeneous else 1.0)
+            g1, g2 = g(a, b)
+            torch.cuda.synchronize()
+            e1 = float((g1 - r1).abs().max() / r1.abs().max())
+            e2 = float((g2 - r2).abs().max() / r2.abs().max())
+            print(homogeneous, rep, e1, e2, flush=True)
+            assert e1 < 1e-12 and e2 < 1e-12
+
+
+@pytest.mark.gpu
 def test_split_view_equals_split_of_permuted_copy(gpu):
     from ccsd_gpu_amd import mfma
     gen = torch.Generator(device="cuda").manual_seed(13)
