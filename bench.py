@@ -109,6 +109,15 @@ def bench_eri(args):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     kern_ms = timed_fused_pass(mol, gp, -8.0)
     roof, flops, quartets, refq = roofline(mol, kern_ms)
+    if args.copy_calibration:
+        # a plain 256 MiB device copy (256 MiB read + 256 MiB written) in the same run: the
+        # reference that scales the FETCH_SIZE / WRITE_SIZE counters of the fused pass
+        x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        del x, y
     cpu = None
     if not args.no_cpu:
         rate, dt, nth = cpu_eri_rate(mol)
@@ -362,6 +371,8 @@ def main():
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--waters", type=int, default=0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="eri config: end with a plain 256 MiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:

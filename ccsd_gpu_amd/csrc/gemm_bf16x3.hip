@@ -10,8 +10,12 @@
 // (beta = 0) or accumulated into (beta = 1).
 //
 // Block: 256 threads = 4 waves, 128x128 C tile, each wave 64x64 = 2x2 MFMA
-// 32x32 tiles; K staged by 32 through LDS, next tile prefetched into registers
-// while the current one is multiplied.
+// 32x32 tiles; K staged by 64 through LDS, next tile prefetched into registers
+// while the current one is multiplied. Tiles are handed out XCD-aware: the
+// hardware deals consecutive workgroups round-robin to the 8 XCDs, so the linear
+// block index is remapped to give each XCD a contiguous range of tiles, ordered
+// in groups of 8 M-tiles (N-tile by N-tile inside a group) so that the blocks
+// running together on one XCD share A and B tiles in that XCD's L2.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -30,9 +34,12 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-constexpr int BM = 128, BN = 128, BK = 32;
-constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (80 bytes)
+constexpr int BM = 128, BN = 128, BK = 64;
+constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (144 bytes)
 constexpr int TILE = BM * LDK;              // bf16 elements of one operand plane tile
+constexpr int CPR = BK / 8;                 // 16-byte chunks per tile row
+constexpr int HCH = BM * CPR / 256;         // chunks per thread per plane
+constexpr int NXCD = 8, GROUP_M = 8;
 
 __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
@@ -44,24 +51,33 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
-    // blockIdx.x runs over M tiles so that consecutive blocks share a B tile (L2 reuse)
-    const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+    int lin = blockIdx.x + gridDim.x * blockIdx.y;
+    {
+        const int nT = gridDim.x * gridDim.y, per = nT / NXCD;
+        if (lin < per * NXCD) lin = (lin % NXCD) * per + lin / NXCD;
+    }
+    const int Mt = gridDim.x;
+    const int gid = lin / (GROUP_M * gridDim.y), first = gid * GROUP_M;
+    const int gsz = (Mt - first < GROUP_M) ? Mt - first : GROUP_M;
+    const int tm = first + (lin % (GROUP_M * gridDim.y)) % gsz;
+    const int tn = (lin % (GROUP_M * gridDim.y)) / gsz;
+    const int m0 = tm * BM, n0 = tn * BN;
     const long long bz = blockIdx.z;
     const __bf16* src[4] = {Ahi + bz * sA, Alo + bz * sA, Bhi + bz * sB, Blo + bz * sB};
     const int rows[4] = {M, M, N, N};
     const int base[4] = {m0, m0, n0, n0};
     const long long ld[4] = {lda, lda, ldb, ldb};
 
-    // each thread moves 2 chunks of 8 bf16 (16 bytes) per plane per K tile:
-    // chunk c = tid + 256*h, row = c / 4, kchunk = c % 4
-    u32x4 stage[4][2];
+    // each thread moves HCH chunks of 8 bf16 (16 bytes) per plane per K tile:
+    // chunk c = tid + 256*h, row = c / CPR, kchunk = c % CPR
+    u32x4 stage[4][HCH];
     auto load_tile = [&](int k0) {
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < HCH; h++) {
                 const int c = tid + 256 * h;
-                const int r = c >> 2, kc = (c & 3) * 8;
+                const int r = c / CPR, kc = (c % CPR) * 8;
                 const int gr = base[p] + r, gk = k0 + kc;
                 u32x4 v = {0u, 0u, 0u, 0u};
                 if (gr < rows[p]) {
@@ -83,9 +99,9 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
 #pragma unroll
         for (int p = 0; p < 4; p++)
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
+            for (int h = 0; h < HCH; h++) {
                 const int c = tid + 256 * h;
-                const int r = c >> 2, kc = (c & 3) * 8;
+                const int r = c / CPR, kc = (c % CPR) * 8;
                 *reinterpret_cast<u32x4*>(lds + p * TILE + r * LDK + kc) = stage[p][h];
             }
     };

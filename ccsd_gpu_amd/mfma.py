@@ -173,7 +173,7 @@ def gemm_nt_scatter(A, B, C, off_b, off_r, off_c, alpha=1.0, beta=1, ksplit=None
     if ksplit is None:
         ksplit = k_splits(nb * -(-m // 128) * -(-n // 128), k) if nb <= 8 else 1
     if ksplit > 1 and k > 0:
-        kc = -(-(-(-k // ksplit)) // 32) * 32
+        kc = -(-(-(-k // ksplit)) // 64) * 64
         full, rest = divmod(k, kc)
         S = full + (rest > 0)
         ws = torch.empty((S, nb, m, n), dtype=torch.float64, device=C.device)
@@ -216,7 +216,7 @@ def gemm_nt(A, B, out=None, beta=0, ksplit=None):
     if ksplit is None:
         ksplit = k_splits(nb * -(-m // 128) * -(-n // 128), k) if nb <= 8 else 1
     if ksplit > 1:
-        kc = -(-(-(-k // ksplit)) // 32) * 32          # chunk length, a multiple of the K tile
+        kc = -(-(-(-k // ksplit)) // 64) * 64          # chunk length, a multiple of the K tile
         full, rest = divmod(k, kc)
         ws = torch.empty((full + (rest > 0), m, n), dtype=torch.float64, device=out.device)
         for b in range(nb):
