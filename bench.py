@@ -1,6 +1,6 @@
 """Benchmark of the CCSD-gradient hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad|eri] [--waters N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad|eri|batch|precision|gemm] [--waters N]
 
 --config grad (default): the full CCSD gradient of a seeded water octamer
 (H2O)8 / cc-pVDZ (192 AOs, nocc 40, nvir 152; rigid monomers, random
@@ -25,8 +25,14 @@ plan; one step = one Schwarz-screened (1e-12) fused int2e_ip1 x Gamma pass over
 the whole batch with a separable density Gamma(D) per molecule; value = surviving
 quartets/s.
 
+--config gemm: the split-BF16 NT GEMM of the lambda / RDM contractions on the
+(H2O)8 sweep shapes (random operands); value = achieved bf16 MFMA TFLOP/s of the
+largest product (three bf16 products per split pair counted).
+
 Every line carries `roofline` (the dominant kernel: FP64 vector for the ERI
-kernels, time from HIP events on the launch stream) and `cpu_baseline`.
+kernels, bf16 MFMA for the GEMM, time from HIP events on the launch stream) and
+`cpu_baseline`; the grad line also reports the MFMA roofline of the lambda
+sweep's largest GEMM in config.lambda_gemm.
 """
 import argparse
 import json
@@ -286,7 +292,6 @@ def bench_precision(args):
 def bench_grad(args):
     import torch
     from ccsd_gpu_amd import pipeline, systems
-    from ccsd_gpu_amd.gpu_ccsd_grad import _pack_halved  # noqa: F401
     nw = args.waters or 8
     mol = systems.water_cluster(nw, seed=1, cart=False)
     log(f"(H2O){nw}/cc-pVDZ nao={mol.nao} nbas={mol.nbas}")
@@ -318,6 +323,9 @@ def bench_grad(args):
     kern_ms = timed_fused_pass(mol, gp, -2.0, reps=2)
     del gp
     roof, flops, quartets, refq = roofline(mol, kern_ms)
+    # the lambda sweeps' largest product (l2 (ac|bd), i <= j) on the MFMA roofline
+    _, gm, gn, gk = LAMBDA_GEMMS[0]
+    gemm_ms = timed_gemm(gm, gn, gk, reps=3)
 
     cpu = None
     if not args.no_cpu:
@@ -357,8 +365,63 @@ def bench_grad(args):
                    "net_force": net_force, "net_torque": torque,
                    "eri_prime_quartets_per_s": refq / (kern_ms * 1e-3),
                    "eri_prime_pass_ms": kern_ms,
-                   "quartets_reference_equivalent": refq},
+                   "quartets_reference_equivalent": refq,
+                   "lambda_gemm": {"shape_mnk": [gm, gn, gk], "ms": gemm_ms,
+                                   "roofline": gemm_roofline(gm, gn, gk, gemm_ms)}},
         "roofline": roof, "cpu_baseline": cpu}
+
+
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
+
+# the split-BF16 products of one (H2O)8 lambda sweep (nocc 40, nvir 152): (name, M, N, K)
+LAMBDA_GEMMS = [("l2 (ac|bd), i<=j", 820, 23104, 23104), ("ring terms (ov x ov)", 6080, 6080, 6080),
+                ("l2t1 ovvv", 1600, 23104, 6080), ("l2tau voov / woooo l2", 1600, 23104, 1600)]
+
+
+def timed_gemm(m, n, k, reps=5, seed=0):
+    """HIP-event time (ms) of one split-BF16 NT GEMM (no K split) on random planes."""
+    import torch
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(seed)
+    A = mfma.split(torch.randn((1, m, k), dtype=torch.float64, device="cuda", generator=gen))
+    B = mfma.split(torch.randn((1, n, k), dtype=torch.float64, device="cuda", generator=gen))
+    out = torch.empty((1, m, n), dtype=torch.float64, device="cuda")
+    stream = torch.cuda.current_stream()
+    mfma.gemm_nt(A, B, out=out, ksplit=1)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    for _ in range(reps):
+        mfma.gemm_nt(A, B, out=out, ksplit=1)
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def gemm_roofline(m, n, k, ms):
+    """MFMA roofline of a split-BF16 GEMM: three bf16 products per element pair."""
+    achieved = 3 * 2.0 * m * n * k / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / BF16_PEAK_TFLOPS, "traffic": None}
+
+
+def bench_gemm(args):
+    """The split-BF16 GEMM of the lambda / RDM contractions on the (H2O)8 sweep shapes."""
+    rows = []
+    for name, m, n, k in LAMBDA_GEMMS:
+        ms = timed_gemm(m, n, k, reps=max(1, args.steps))
+        roof = gemm_roofline(m, n, k, ms)
+        rows.append({"product": name, "m": m, "n": n, "k": k, "ms": ms, "bf16_tflops": roof["achieved"],
+                     "fp64_equivalent_tflops": roof["achieved"] / 3})
+        log(f"{name}: {m}x{n}x{k} {ms:.3f} ms, {roof['achieved']:.0f} TF/s bf16 ({roof['frac']:.1%} of peak)")
+    head = rows[0]
+    return {
+        "metric": METRIC, "value": head["bf16_tflops"], "unit": "TFLOP/s (bf16 MFMA, 3 products per split pair)",
+        "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": head["ms"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3 (fp32 accumulate, fp64 out)", "data": "synthetic",
+        "config": {"workload": "split-BF16 NT GEMM on the (H2O)8/cc-pVDZ lambda-sweep shapes, random operands",
+                   "products": rows},
+        "roofline": gemm_roofline(head["m"], head["n"], head["k"], head["ms"]), "cpu_baseline": None}
 
 
 def main():
@@ -366,7 +429,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision"])
+    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision", "gemm"])
     ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--waters", type=int, default=0)
@@ -383,7 +446,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     _lib.load()
-    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision}[args.config](args)
+    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision,
+            "gemm": bench_gemm}[args.config](args)
     print(json.dumps(line), flush=True)
 
 

@@ -10,8 +10,8 @@
 // (beta = 0) or accumulated into (beta = 1).
 //
 // Block: 256 threads = 4 waves, 128x128 C tile, each wave 64x64 = 2x2 MFMA
-// 32x32 tiles; K staged by 64 through LDS, next tile prefetched into registers
-// while the current one is multiplied. Tiles are handed out XCD-aware: the
+// 32x32 tiles; K staged by 64 (32 when K <= 256) through LDS, next tile
+// prefetched into registers while the current one is multiplied. Tiles are handed out XCD-aware: the
 // hardware deals consecutive workgroups round-robin to the 8 XCDs, so the linear
 // block index is remapped to give each XCD a contiguous range of tiles, ordered
 // in groups of 8 M-tiles (N-tile by N-tile inside a group) so that the blocks
@@ -34,33 +34,39 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-constexpr int BM = 128, BN = 128, BK = 64;
-constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (144 bytes)
-constexpr int TILE = BM * LDK;              // bf16 elements of one operand plane tile
-constexpr int CPR = BK / 8;                 // 16-byte chunks per tile row
-constexpr int HCH = BM * CPR / 256;         // chunks per thread per plane
+constexpr int BM = 128, BN = 128;
 constexpr int NXCD = 8, GROUP_M = 8;
 
+// BK: K elements staged per LDS tile (64 for long K; 32 when K is short, where the
+// deeper tile only adds zero padding and LDS)
+template <int BK>
 __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
     const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long lda, long long ldb,
     long long sA, long long sB, long long sC, long long ldc, int beta, double alpha, const long long* __restrict__ off_b,
     const long long* __restrict__ off_r, const long long* __restrict__ off_c)
 {
+    constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (BK + 8: conflict-free b128 reads)
+    constexpr int TILE = BM * LDK;              // bf16 elements of one operand plane tile
+    constexpr int CPR = BK / 8;                 // 16-byte chunks per tile row
+    constexpr int HCH = BM * CPR / 256;         // chunks per thread per plane
     __shared__ __attribute__((aligned(16))) __bf16 lds[4 * TILE];   // Ahi, Alo, Bhi, Blo
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wm = wave >> 1, wn = wave & 1;
-    int lin = blockIdx.x + gridDim.x * blockIdx.y;
-    {
+    int tm = blockIdx.x, tn = blockIdx.y;
+    if (BK == 64) {
+        // long K: XCD-aware grouped order (short K keeps the plain M-fastest order,
+        // which measured faster when the output write dominates)
+        int lin = blockIdx.x + gridDim.x * blockIdx.y;
         const int nT = gridDim.x * gridDim.y, per = nT / NXCD;
         if (lin < per * NXCD) lin = (lin % NXCD) * per + lin / NXCD;
+        const int Mt = gridDim.x;
+        const int gid = lin / (GROUP_M * gridDim.y), first = gid * GROUP_M;
+        const int gsz = (Mt - first < GROUP_M) ? Mt - first : GROUP_M;
+        tm = first + (lin % (GROUP_M * gridDim.y)) % gsz;
+        tn = (lin % (GROUP_M * gridDim.y)) / gsz;
     }
-    const int Mt = gridDim.x;
-    const int gid = lin / (GROUP_M * gridDim.y), first = gid * GROUP_M;
-    const int gsz = (Mt - first < GROUP_M) ? Mt - first : GROUP_M;
-    const int tm = first + (lin % (GROUP_M * gridDim.y)) % gsz;
-    const int tn = (lin % (GROUP_M * gridDim.y)) / gsz;
     const int m0 = tm * BM, n0 = tn * BN;
     const long long bz = blockIdx.z;
     const __bf16* src[4] = {Ahi + bz * sA, Alo + bz * sA, Bhi + bz * sB, Blo + bz * sB};
@@ -288,9 +294,14 @@ hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const vo
 {
     if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
     dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-    hipLaunchKernelGGL(gemm_bf16x3_nt_kernel, grid, dim3(256), 0, stream, (const __bf16*)Ahi, (const __bf16*)Alo,
-                       (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta, alpha,
-                       off_b, off_r, off_c);
+    if (K > 256)
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<64>, grid, dim3(256), 0, stream, (const __bf16*)Ahi,
+                           (const __bf16*)Alo, (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB,
+                           sC, ldc, beta, alpha, off_b, off_r, off_c);
+    else
+        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<32>, grid, dim3(256), 0, stream, (const __bf16*)Ahi,
+                           (const __bf16*)Alo, (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB,
+                           sC, ldc, beta, alpha, off_b, off_r, off_c);
     return hipGetLastError();
 }
 

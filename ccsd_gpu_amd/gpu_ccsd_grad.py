@@ -161,6 +161,28 @@ def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff, precision="fp64"):
     return t
 
 
+def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
+    """The same AO 2-RDM, packed as the fused gradient kernel and Imat take it:
+    G[i][j][k(k+1)/2 + l] for k >= l, the kl diagonal halved (nao^2 npair).
+
+    The two kl transforms run on the full MO tensor; their product is symmetric in
+    (k, l), so only k >= l is kept before the i and j transforms, which then do
+    half the work and write the packed layout directly."""
+    dm2 = _assemble_dm2_mo(d2)
+    _lib_sym_pairs(dm2, 0.5)
+    c = _dev(mo_coeff)
+    n = c.shape[0]
+    t = _rotate_transform(dm2, c, precision)     # (l p q r)
+    del dm2
+    t = _rotate_transform(t, c, precision)       # (k l p q)
+    k, l = torch.tril_indices(n, n, device=t.device)
+    tp = t[k, l]                                 # (kl p q), k >= l
+    del t
+    tp[k == l] *= 0.5
+    tp = _rotate_transform(tp, c, precision)     # (j kl p)
+    return _rotate_transform(tp, c, precision)   # (i j kl)
+
+
 def _rotate_transform(X, c, precision):
     """Y[l, r...] = sum_s X[r..., s] c[l, s] for a contiguous X: one GEMM with the
     contracted index contiguous in both operands, the product written transposed
@@ -193,14 +215,6 @@ def _lib_sym_pairs(t, scale):
     _lib.check(_lib.load().ccsd_sym_pairs(_lib.ptr(t), n, float(scale), _lib.current_stream_ptr()),
                "ccsd_sym_pairs")
     return t
-
-
-def _pack_halved(full):
-    n = full.shape[-1]
-    k, l = torch.tril_indices(n, n, device=full.device)
-    out = full[..., k, l].contiguous()
-    out[..., k == l] *= 0.5
-    return out
 
 
 # ---------------------------------------------------------------------------
@@ -304,12 +318,13 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     # Phase 2: Cartesian AO 2-RDM (symmetrized)
     # FP64 by default: a split-BF16 operand keeps 16 mantissa bits, which the
     # nao^4 contraction with the derivative integrals turns into ~1e-7 Eh/Bohr
-    gam = _rdm2_mo2ao_gpu(mycc, d2, c2s @ mo, mo2ao_precision)
+    gp = _rdm2_mo2ao_packed_gpu(mycc, d2, c2s @ mo, mo2ao_precision)
     del d2
     t0 = mark("mo2ao", t0)
 
-    # Phase 3a: Imat (one DGEMM over the HBM-resident Cartesian ERIs)
-    imat_cc = 0.5 * (ints.eri_c.reshape(nc, -1) @ gam.reshape(nc, -1).T)
+    # Phase 3a: Imat (one DGEMM over the HBM-resident packed Cartesian ERIs:
+    # sum over all kl of (ij|kl) G_i'jkl = 2 sum_{k>=l} with the diagonal halved)
+    imat_cc = ints.eri_p.reshape(nc, -1) @ gp.reshape(nc, -1).T
     ctc = c2s.T @ c2s
     imat = c2s.T @ imat_cc @ c2s @ torch.linalg.inv(ctc)
     t0 = mark("imat", t0)
@@ -341,13 +356,9 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     Pc = c2s @ dm1p @ c2s.T
     if fused:
         # one pass: the CCSD density explicitly, the D/P separable part formed in the kernel
-        gp = _pack_halved(gam)
-        del gam
         contract_int2e_ip1(mol, gp, scale=-2.0, de=de2, dm=Dc.contiguous(), pm=Pc.contiguous())
         del gp
     else:
-        gp = _pack_halved(gam)
-        del gam
         _grad2e_unfused(mol, gp, Dc, Pc, de2)
         del gp
     t0 = mark("grad2e", t0)

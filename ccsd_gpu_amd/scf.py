@@ -38,7 +38,8 @@ class AOIntegrals:
         mol_c = mol.copy()
         mol_c.cart = True
         self.mol_cart = mol_c
-        self.eri_c = unpack_s2kl(compute_int2e_gpu(mol_c, 0, mol.nbas))     # (nc, nc, nc, nc)
+        self.eri_p = compute_int2e_gpu(mol_c, 0, mol.nbas)                  # (nc, nc, nc(nc+1)/2), s2kl
+        self.eri_c = unpack_s2kl(self.eri_p)                                 # (nc, nc, nc, nc)
         self.S = int1e(mol, "ovlp")
         self.T = int1e(mol, "kin")
         self.V = int1e(mol, "nuc")

@@ -107,3 +107,24 @@ def test_rotated_one_index_transforms_give_full_mo_to_ao():
     # one step: last index transformed, moved to the front
     one = _rotate_transform(d, c, "fp64")
     assert torch.allclose(one, torch.einsum("pqrs,ls->lpqr", d, c), rtol=1e-12, atol=1e-12)
+
+
+def test_packed_transform_equals_packed_full_transform():
+    """MO->AO with the kl pair packed after the first two transforms equals the full
+    transform packed afterwards (k >= l, diagonal halved), for a (pq)(rs)-symmetric D."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _rotate_transform
+    g = torch.Generator().manual_seed(5)
+    d = torch.randn(5, 5, 5, 5, dtype=torch.float64, generator=g)
+    d = d + d.permute(1, 0, 2, 3)
+    d = d + d.permute(0, 1, 3, 2)
+    c = torch.randn(7, 5, dtype=torch.float64, generator=g)
+    full = torch.einsum("ip,jq,kr,ls,pqrs->ijkl", c, c, c, c, d)
+    k, l = torch.tril_indices(7, 7)
+    ref = full[:, :, k, l].clone()
+    ref[:, :, k == l] *= 0.5
+    t = _rotate_transform(_rotate_transform(d, c, "fp64"), c, "fp64")
+    tp = t[k, l]
+    tp[k == l] *= 0.5
+    got = _rotate_transform(_rotate_transform(tp, c, "fp64"), c, "fp64")
+    assert got.shape == ref.shape
+    assert torch.allclose(got, ref, rtol=1e-12, atol=1e-12)
