@@ -45,6 +45,8 @@ _SIGNATURES = {
                                         c_void_p]),
     "ccsd_sym_pairs": (c_int, [c_void_p, c_int, c_double, c_void_p]),
     "ccsd_permute": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "ccsd_combine": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_int,
+                             c_void_p]),
     "ccsd_gemv": (c_int, [c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, c_void_p,
                           ctypes.c_longlong, c_void_p, c_double, c_int, c_void_p, c_int, c_void_p]),
     "ccsd_l1_terms": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_double,

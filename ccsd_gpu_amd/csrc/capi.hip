@@ -45,6 +45,9 @@ hipError_t ccsd_launch_permute(const double* src, int ndim, const long long* siz
                                double* dst, void* hi, void* lo, hipStream_t stream);
 hipError_t ccsd_launch_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x,
                             long long incx, double* y, double alpha, int beta, double* ws, int S, hipStream_t stream);
+hipError_t ccsd_launch_combine(int ndim, const long long* sizes, double* dst, const long long* dstrides, int nt,
+                               const double* const* srcs, const long long* sstrides, const double* coef, int beta,
+                               hipStream_t stream);
 hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, long long sy0, long long sy1,
                                 double c1, double c2, double c3, double c4, double* ws, double* out,
                                 hipStream_t stream);
@@ -1230,6 +1233,22 @@ int ccsd_permute(const double* src, int ndim, const long long* sizes, const long
         if (sizes[k] < 0) return fail(CCSD_ERR_SHAPE, "negative size");
     hipError_t e = ccsd_launch_permute(src, ndim, sizes, strides, dst, hi, lo, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "permute_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_combine(int ndim, const long long* sizes, double* dst, const long long* dstrides, int nterms,
+                 const double* const* srcs, const long long* sstrides, const double* coef, int beta, void* stream)
+{
+    if (!sizes || !dst || !dstrides || !srcs || !sstrides || !coef) return fail(CCSD_ERR_NULL, "null pointer");
+    if (ndim < 1 || ndim > 6 || nterms < 1 || nterms > 4 || (beta != 0 && beta != 1))
+        return fail(CCSD_ERR_SHAPE, "ndim must be 1..6 and nterms 1..4");
+    for (int t = 0; t < nterms; t++)
+        if (!srcs[t]) return fail(CCSD_ERR_NULL, "null source");
+    for (int k = 0; k < ndim; k++)
+        if (sizes[k] < 0) return fail(CCSD_ERR_SHAPE, "negative size");
+    hipError_t e = ccsd_launch_combine(ndim, sizes, dst, dstrides, nterms, srcs, sstrides, coef, beta,
+                                       (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "combine_kernel");
     return CCSD_OK;
 }
 

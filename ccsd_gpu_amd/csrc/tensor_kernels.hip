@@ -10,6 +10,10 @@
 //     are tiled 32 x 32 through LDS, so reads run along the first and writes
 //     along the second (the transposing copies of permuted GEMM operands).
 //
+// combine_rows_kernel / combine_tiled_kernel: dst (+)= sum_t coef_t src_t over up to
+// four strided views of one shape (the permuted sums of the density and lambda code),
+// transposed terms through the same LDS tiles.
+//
 // gemv_rows_kernel: y[m] (+)= alpha sum_k A[m*lda + k] x[k*incx], one wave per
 // (row, K slice), coalesced along k; gemv_cols_kernel: y[n] (+)= alpha
 // sum_k A[k*lda + n] x[k*incx], one thread per (column, K slice), coalesced
@@ -108,6 +112,107 @@ __global__ void __launch_bounds__(256) permute_tiled_kernel(const double* __rest
     for (int k = 0; k < TT; k += 8) {
         const long long i = i0 + ty + k, j = j0 + tx;
         if (i < Ni && j < Nj) store(tile[tx][ty + k], doff + i * dsi + j, dst, hi, lo);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// combine: dst (+)= sum_t coef[t] * src_t, each src_t a strided view of the
+// destination's shape (ndim <= 6), dst any strided view. Terms contiguous along the
+// destination's last dimension dj are read directly; terms contiguous along one
+// other dimension di are staged through 32 x 32 LDS tiles (the transposed terms of
+// the permuted sums in the density code); any other term is read elementwise.
+constexpr int MAXT = 4;
+
+struct CombDesc {
+    int ndim, nt, di;
+    long long size[MAXD];
+    long long dstride[MAXD];
+    long long sstride[MAXT][MAXD];
+    const double* src[MAXT];
+    double coef[MAXT];
+    int mode[MAXT];          // 0 direct (stride 1 along dj), 1 tiled (stride 1 along di), 2 gather
+};
+
+__global__ void __launch_bounds__(256) combine_rows_kernel(CombDesc d, long long nrows, double* __restrict__ dst,
+                                                           int beta)
+{
+    const int lane = threadIdx.x & 63;
+    const int dj = d.ndim - 1;
+    const long long L = d.size[dj];
+    const long long nw = (long long)gridDim.x * 4;
+    for (long long row = blockIdx.x * 4LL + (threadIdx.x >> 6); row < nrows; row += nw) {
+        long long rem = row, doff = 0;
+        long long soff[MAXT] = {0, 0, 0, 0};
+        for (int k = dj - 1; k >= 0; k--) {
+            const long long q = rem / d.size[k];
+            const long long x = rem - q * d.size[k];
+            doff += x * d.dstride[k];
+#pragma unroll
+            for (int t = 0; t < MAXT; t++)
+                if (t < d.nt) soff[t] += x * d.sstride[t][k];
+            rem = q;
+        }
+        for (long long c = lane; c < L; c += 64) {
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < MAXT; t++)
+                if (t < d.nt) v += d.coef[t] * d.src[t][soff[t] + c * d.sstride[t][dj]];
+            double* p = dst + doff + c * d.dstride[dj];
+            *p = beta ? *p + v : v;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) combine_tiled_kernel(CombDesc d, long long nouter, double* __restrict__ dst,
+                                                            int beta)
+{
+    __shared__ double tile[MAXT][TT][TT + 1];
+    const int dj = d.ndim - 1, di = d.di;
+    const long long Ni = d.size[di], Nj = d.size[dj];
+    const long long ti = (Ni + TT - 1) / TT, tj = (Nj + TT - 1) / TT;
+    const long long blk = blockIdx.x;
+    const long long tile_id = blk % (ti * tj);
+    const long long outer = blk / (ti * tj);
+    if (outer >= nouter) return;
+    const long long i0 = (tile_id / tj) * TT, j0 = (tile_id % tj) * TT;
+    long long rem = outer, doff = 0;
+    long long soff[MAXT] = {0, 0, 0, 0};
+    for (int k = d.ndim - 1; k >= 0; k--) {
+        if (k == di || k == dj) continue;
+        const long long q = rem / d.size[k];
+        const long long x = rem - q * d.size[k];
+        doff += x * d.dstride[k];
+#pragma unroll
+        for (int t = 0; t < MAXT; t++)
+            if (t < d.nt) soff[t] += x * d.sstride[t][k];
+        rem = q;
+    }
+    const int tx = threadIdx.x & (TT - 1), ty = threadIdx.x >> 5;
+#pragma unroll
+    for (int t = 0; t < MAXT; t++) {
+        if (t >= d.nt || d.mode[t] != 1) continue;
+#pragma unroll
+        for (int k = 0; k < TT; k += 8) {
+            const long long i = i0 + tx, j = j0 + ty + k;
+            tile[t][ty + k][tx] = (i < Ni && j < Nj) ? d.src[t][soff[t] + i + j * d.sstride[t][dj]] : 0.0;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < TT; k += 8) {
+        const long long i = i0 + ty + k, j = j0 + tx;
+        if (i < Ni && j < Nj) {
+            double v = 0.0;
+#pragma unroll
+            for (int t = 0; t < MAXT; t++) {
+                if (t >= d.nt) continue;
+                const double x = d.mode[t] == 1 ? tile[t][tx][ty + k]
+                                                : d.src[t][soff[t] + i * d.sstride[t][di] + j * d.sstride[t][dj]];
+                v += d.coef[t] * x;
+            }
+            double* p = dst + doff + i * d.dstride[di] + j * d.dstride[dj];
+            *p = beta ? *p + v : v;
+        }
     }
 }
 
@@ -210,6 +315,59 @@ hipError_t ccsd_launch_permute(const double* src, int ndim, const long long* siz
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(permute_tiled_kernel, dim3((unsigned)nb), dim3(256), 0, stream, src, d, di, nouter, dst,
                        (uint16_t*)hi, (uint16_t*)lo);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_combine(int ndim, const long long* sizes, double* dst, const long long* dstrides, int nt,
+                               const double* const* srcs, const long long* sstrides, const double* coef, int beta,
+                               hipStream_t stream)
+{
+    if (ndim < 1 || ndim > MAXD || nt < 1 || nt > MAXT) return hipErrorInvalidValue;
+    CombDesc d;
+    d.ndim = ndim;
+    d.nt = nt;
+    long long n = 1;
+    for (int k = 0; k < MAXD; k++) {
+        d.size[k] = k < ndim ? sizes[k] : 1;
+        d.dstride[k] = k < ndim ? dstrides[k] : 0;
+        if (k < ndim) n *= sizes[k];
+    }
+    if (n <= 0) return hipSuccess;
+    const int dj = ndim - 1;
+    // the tile dimension: the (largest) dimension that most non-direct terms walk with stride 1
+    int votes[MAXD] = {0, 0, 0, 0, 0, 0};
+    for (int t = 0; t < MAXT; t++) {
+        d.src[t] = t < nt ? srcs[t] : nullptr;
+        d.coef[t] = t < nt ? coef[t] : 0.0;
+        for (int k = 0; k < MAXD; k++) d.sstride[t][k] = (t < nt && k < ndim) ? sstrides[t * ndim + k] : 0;
+        d.mode[t] = 2;
+        if (t >= nt) continue;
+        if (d.sstride[t][dj] == 1 || d.size[dj] == 1) {
+            d.mode[t] = 0;
+            continue;
+        }
+        for (int k = 0; k < dj; k++)
+            if (d.sstride[t][k] == 1 && d.size[k] > 1) votes[k]++;
+    }
+    int di = -1;
+    for (int k = 0; k < dj; k++)
+        if (votes[k] > 0 && (di < 0 || votes[k] > votes[di] || (votes[k] == votes[di] && d.size[k] > d.size[di])))
+            di = k;
+    d.di = di;
+    if (di < 0 || d.size[dj] < 4) {
+        const long long nrows = n / d.size[dj];
+        long long nb = (nrows + 3) / 4;
+        if (nb > 65536LL * 4) nb = 65536LL * 4;
+        hipLaunchKernelGGL(combine_rows_kernel, dim3((unsigned)nb), dim3(256), 0, stream, d, nrows, dst, beta);
+        return hipGetLastError();
+    }
+    for (int t = 0; t < nt; t++)
+        if (d.mode[t] == 2 && d.sstride[t][di] == 1) d.mode[t] = 1;
+    const long long ti = (d.size[di] + TT - 1) / TT, tj = (d.size[dj] + TT - 1) / TT;
+    const long long nouter = n / (d.size[di] * d.size[dj]);
+    const long long nb = nouter * ti * tj;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(combine_tiled_kernel, dim3((unsigned)nb), dim3(256), 0, stream, d, nouter, dst, beta);
     return hipGetLastError();
 }
 

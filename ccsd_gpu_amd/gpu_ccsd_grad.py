@@ -77,10 +77,10 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     mij = C("kc,jc->jk", l1, t1) + moo * 0.5
     tau = E("ia,jb->ijab", t1, t1) + t2
     goooo = C("ijab,klab->ijkl", tau, l2) * 0.5
-    doooo = goooo.permute(0, 2, 1, 3) * 2 - goooo.permute(0, 3, 1, 2)
+    doooo = mfma.combine([(2, goooo.permute(0, 2, 1, 3)), (-1, goooo.permute(0, 3, 1, 2))])
     gooov = gooov + C("ji,ka->jkia", -0.5 * moo, t1) + C("la,jkil->jkia", 2 * t1, goooo)
     gooov = gooov - C("ib,jkba->jkia", l1, tau) - C("jkba,ib->jkia", l2, t1)
-    dooov = gooov.permute(0, 2, 1, 3) * 2 - gooov.permute(1, 2, 0, 3)
+    dooov = mfma.combine([(2, gooov.permute(0, 2, 1, 3)), (-1, gooov.permute(1, 2, 0, 3))])
 
     goovv = E("ia,jb->ijab", mia, t1)
     tmp = C("ijkl,klab->ijab", goooo, tau) - C("jk,ikab->ijab", mij, tau)
@@ -90,24 +90,24 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     gOvvO = C("kiac,jckb->iabj", l2, t1t1) + pvOOv.permute(1, 0, 3, 2)
     govVO = E("ia,jb->iabj", l1, t1) - C("ikac,jckb->iabj", l2, t1t1) + pvoOV.permute(1, 0, 3, 2)
     dovvo = 2 * govVO + gOvvO
-    doovv = (-2 * gOvvO - govVO).permute(3, 0, 1, 2)
+    doovv = mfma.combine([(-2, gOvvO.permute(3, 0, 1, 2)), (-1, govVO.permute(3, 0, 1, 2))])
     tau2 = tau - t2 * 0.5
     goovv = goovv + C("dlib,jlda->ijab", pvOOv, tau2) - C("dlia,jldb->ijab", pvoOV, tau2)
     goovv = goovv + C("dlia,jlbd->ijab", pvoOV + pvOOv * 0.5, t2)
-    dovov = goovv.permute(0, 2, 1, 3) * 2 - goovv.permute(1, 2, 0, 3)
+    dovov = mfma.combine([(2, goovv.permute(0, 2, 1, 3)), (-1, goovv.permute(1, 2, 0, 3))])
     del goovv
 
     gvvvv = C("ijab,ijcd->abcd", l2, t2)
     jabc = C("ijab,ic->jabc", l2, t1)
     gvvvv = gvvvv + C("jabc,jd->abcd", jabc, t1)
     vvv = gvvvv.permute(0, 2, 1, 3)
-    dvvvv = vvv - vvv.permute(0, 3, 2, 1) * 0.5
+    dvvvv = mfma.combine([(1, vvv), (-0.5, vvv.permute(0, 3, 2, 1))])
     gvovv = C("adbc,id->aibc", gvvvv, -t1)
     del gvvvv, vvv
     gvovv = gvovv + C("akic,kb->aibc", pvoOV, t1) - C("akib,kc->aibc", pvOOv, t1)
     gvovv = gvovv + C("ja,jibc->aibc", l1, t2) + C("ja,jbic->aibc", l1, E("jb,ic->jbic", t1, t1))
     gvovv = gvovv + C("ba,ic->aibc", mvv * 0.5, t1) + C("ja,jibc->aibc", t1, l2)
-    dovvv = gvovv.permute(1, 3, 0, 2) * 2 - gvovv.permute(1, 2, 0, 3)
+    dovvv = mfma.combine([(2, gvovv.permute(1, 3, 0, 2)), (-1, gvovv.permute(1, 2, 0, 3))])
     return (dovov, dvvvv, doooo, doovv, dovvo, None, dovvv, dooov)
 
 
@@ -115,27 +115,32 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
 # Phase 2: full 2-RDM and MO -> AO
 # ---------------------------------------------------------------------------
 def _assemble_dm2_mo(d2):
+    """Full MO 2-RDM (chemist's order) from the eight density blocks, as
+    ccsd_gpu/gpu_ccsd_grad.py:272-300; each of the 16 occupied/virtual blocks is
+    written once by the combine kernel (no zero fill, transposed blocks through
+    LDS tiles)."""
     dovov, dvvvv, doooo, doovv, dovvo, _, dovvv, dooov = d2
     nocc, nvir = dovov.shape[:2]
     nmo = nocc + nvir
     o, v = slice(0, nocc), slice(nocc, None)
-    dm2 = torch.zeros((nmo,) * 4, dtype=torch.float64, device=DEV)
-    dm2[o, v, o, v] = dovov + dovov.permute(2, 3, 0, 1)
-    dm2[v, o, v, o] = dm2[o, v, o, v].permute(1, 0, 3, 2)
-    dm2[o, o, v, v] = doovv + doovv.permute(1, 0, 3, 2)
-    dm2[v, v, o, o] = dm2[o, o, v, v].permute(2, 3, 0, 1)
-    dm2[o, v, v, o] = dovvo + dovvo.permute(3, 2, 1, 0)
-    dm2[v, o, o, v] = dm2[o, v, v, o].permute(1, 0, 3, 2)
-    dm2[v, v, v, v] = (dvvvv + dvvvv.permute(1, 0, 3, 2)) * 2
-    dm2[o, o, o, o] = (doooo + doooo.permute(1, 0, 3, 2)) * 2
-    dm2[o, v, v, v] = dovvv
-    dm2[v, v, o, v] = dovvv.permute(2, 3, 0, 1)
-    dm2[v, v, v, o] = dovvv.permute(3, 2, 1, 0)
-    dm2[v, o, v, v] = dovvv.permute(1, 0, 3, 2)
-    dm2[o, o, o, v] = dooov
-    dm2[o, v, o, o] = dooov.permute(2, 3, 0, 1)
-    dm2[o, o, v, o] = dooov.permute(1, 0, 3, 2)
-    dm2[v, o, o, o] = dooov.permute(3, 2, 1, 0)
+    dm2 = torch.empty((nmo,) * 4, dtype=torch.float64, device=DEV)
+    cb = mfma.combine
+    cb([(1, dovov), (1, dovov.permute(2, 3, 0, 1))], out=dm2[o, v, o, v])
+    cb([(1, dm2[o, v, o, v].permute(1, 0, 3, 2))], out=dm2[v, o, v, o])
+    cb([(1, doovv), (1, doovv.permute(1, 0, 3, 2))], out=dm2[o, o, v, v])
+    cb([(1, dm2[o, o, v, v].permute(2, 3, 0, 1))], out=dm2[v, v, o, o])
+    cb([(1, dovvo), (1, dovvo.permute(3, 2, 1, 0))], out=dm2[o, v, v, o])
+    cb([(1, dm2[o, v, v, o].permute(1, 0, 3, 2))], out=dm2[v, o, o, v])
+    cb([(2, dvvvv), (2, dvvvv.permute(1, 0, 3, 2))], out=dm2[v, v, v, v])
+    cb([(2, doooo), (2, doooo.permute(1, 0, 3, 2))], out=dm2[o, o, o, o])
+    cb([(1, dovvv)], out=dm2[o, v, v, v])
+    cb([(1, dovvv.permute(2, 3, 0, 1))], out=dm2[v, v, o, v])
+    cb([(1, dovvv.permute(3, 2, 1, 0))], out=dm2[v, v, v, o])
+    cb([(1, dovvv.permute(1, 0, 3, 2))], out=dm2[v, o, v, v])
+    cb([(1, dooov)], out=dm2[o, o, o, v])
+    cb([(1, dooov.permute(2, 3, 0, 1))], out=dm2[o, v, o, o])
+    cb([(1, dooov.permute(1, 0, 3, 2))], out=dm2[o, o, v, o])
+    cb([(1, dooov.permute(3, 2, 1, 0))], out=dm2[v, o, o, o])
     return dm2
 
 
@@ -165,22 +170,36 @@ def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
     """The same AO 2-RDM, packed as the fused gradient kernel and Imat take it:
     G[i][j][k(k+1)/2 + l] for k >= l, the kl diagonal halved (nao^2 npair).
 
-    The two kl transforms run on the full MO tensor; their product is symmetric in
-    (k, l), so only k >= l is kept before the i and j transforms, which then do
-    half the work and write the packed layout directly."""
+    The two kl transforms run on the full MO tensor; the (pq)(rs) symmetrisation
+    commutes with them and is applied while k >= l is packed out of their product,
+    so the i and j transforms do half the work and write the packed layout
+    directly."""
     dm2 = _assemble_dm2_mo(d2)
-    _lib_sym_pairs(dm2, 0.5)
     c = _dev(mo_coeff)
-    n = c.shape[0]
     t = _rotate_transform(dm2, c, precision)     # (l p q r)
     del dm2
     t = _rotate_transform(t, c, precision)       # (k l p q)
-    k, l = torch.tril_indices(n, n, device=t.device)
-    tp = t[k, l]                                 # (kl p q), k >= l
+    tp = _pack_pairs_sym(t)                      # (kl p q), k >= l, symmetrised, diagonal halved
     del t
-    tp[k == l] *= 0.5
     tp = _rotate_transform(tp, c, precision)     # (j kl p)
     return _rotate_transform(tp, c, precision)   # (i j kl)
+
+
+def _pack_pairs_sym(t):
+    """t (n, n, m, m) -> G (n(n+1)/2, m, m) with G[k(k+1)/2 + l, p, q] =
+    (t[k,l,p,q] + t[l,k,p,q] + t[k,l,q,p] + t[l,k,q,p]) / 2 for k >= l (the
+    (ij)(kl) symmetrisation with weight 1/2 per term) and the k == l rows halved:
+    one combine launch per k (its rows l <= k are contiguous in G)."""
+    n, _, m, _ = t.shape
+    tp = torch.empty((n * (n + 1) // 2, m, m), dtype=t.dtype, device=t.device)
+    for k in range(n):
+        r0 = k * (k + 1) // 2
+        a, b = t[k, :k + 1], t[:k + 1, k]
+        mfma.combine([(0.5, a), (0.5, b), (0.5, a.transpose(1, 2)), (0.5, b.transpose(1, 2))],
+                     out=tp[r0:r0 + k + 1])
+    kk = torch.arange(n, device=t.device)
+    tp[kk * (kk + 1) // 2 + kk] *= 0.5
+    return tp
 
 
 def _rotate_transform(X, c, precision):

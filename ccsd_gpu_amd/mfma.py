@@ -72,6 +72,33 @@ def permute_copy(x):
     return out
 
 
+def combine(terms, out=None, beta=0):
+    """out (+)= sum_t alpha_t * x_t for up to four (alpha, x_t) float64 views of one shape
+    (ndim <= 6) on the library's combine kernel (transposed terms through LDS tiles);
+    out (new contiguous when None, else any strided view not overlapping a source).
+    CPU tensors use torch."""
+    shape = tuple(terms[0][1].shape)
+    if out is None:
+        out = torch.empty(shape, dtype=torch.float64, device=terms[0][1].device)
+        beta = 0
+    if any(tuple(x.shape) != shape for _, x in terms) or tuple(out.shape) != shape:
+        raise ValueError("combine needs views of one shape")
+    nd, nt = len(shape), len(terms)
+    if not out.is_cuda or nt > 4 or nd > 6 or nd == 0 or any(x.dtype != torch.float64 for _, x in terms):
+        r = sum(float(a) * x for a, x in terms)
+        return out.add_(r) if beta else out.copy_(r)
+    if out.numel() == 0:
+        return out
+    sizes = (ctypes.c_longlong * nd)(*shape)
+    dstr = (ctypes.c_longlong * nd)(*out.stride())
+    srcs = (ctypes.c_void_p * nt)(*[x.data_ptr() for _, x in terms])
+    sstr = (ctypes.c_longlong * (nt * nd))(*[st for _, x in terms for st in x.stride()])
+    coef = (ctypes.c_double * nt)(*[float(a) for a, _ in terms])
+    _lib.check(_lib.load().ccsd_combine(nd, sizes, _lib.ptr(out), dstr, nt, srcs, sstr, coef, int(beta),
+                                        _lib.current_stream_ptr()), "ccsd_combine")
+    return out
+
+
 def split(x):
     """(nb, rows, cols) float64 (any strides) -> Split (contiguous planes)."""
     x = x.to(torch.float64)

@@ -192,6 +192,16 @@ int ccsd_pair_combine(const double* x, const long long* xstrides, int o, int v, 
 int ccsd_permute(const double* src, int ndim, const long long* sizes, const long long* strides, double* dst, void* hi,
                  void* lo, void* stream);
 
+/* dst (+)= sum_t coef[t] * src_t for nterms <= 4 strided views of one shape (ndim <= 6,
+ * sizes in dst's order; dstrides and sstrides[t * ndim + k] in elements; dst must
+ * not overlap a source; beta 0 overwrites, 1 accumulates). Terms contiguous along
+ * dst's last dimension are read directly, transposed ones through 32 x 32 LDS
+ * tiles. The permuted sums of the RDM code (ccsd_gpu/gpu_ccsd_grad.py:129-246,
+ * e.g. dovov = goovv.transpose(0,2,1,3)*2 - goovv.transpose(1,2,0,3)) and of the
+ * 2-RDM assembly (:272-300). */
+int ccsd_combine(int ndim, const long long* sizes, double* dst, const long long* dstrides, int nterms,
+                 const double* const* srcs, const long long* sstrides, const double* coef, int beta, void* stream);
+
 /* y (+)= alpha * A x in FP64: cols = 0 reads A as M rows of length K
  * (y[m] = sum_k A[m*lda + k] x[k*incx]), cols = 1 as K rows of length M
  * (y[m] = sum_k A[k*lda + m] x[k*incx]); beta 0 overwrites, 1 accumulates.

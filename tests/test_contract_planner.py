@@ -128,3 +128,15 @@ def test_packed_transform_equals_packed_full_transform():
     got = _rotate_transform(_rotate_transform(tp, c, "fp64"), c, "fp64")
     assert got.shape == ref.shape
     assert torch.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
+def test_pack_pairs_sym_symmetrises_and_packs():
+    """(ij)(kl) symmetrisation (weight 1/2 per term) folded into the k >= l packing."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _pack_pairs_sym
+    g = torch.Generator().manual_seed(6)
+    t = torch.randn(6, 6, 4, 4, dtype=torch.float64, generator=g)
+    sym = 0.5 * (t + t.permute(1, 0, 2, 3) + t.permute(0, 1, 3, 2) + t.permute(1, 0, 3, 2))
+    k, l = torch.tril_indices(6, 6)
+    ref = sym[k, l].clone()
+    ref[k == l] *= 0.5
+    assert torch.allclose(_pack_pairs_sym(t), ref, rtol=1e-14, atol=1e-14)

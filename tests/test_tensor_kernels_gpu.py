@@ -86,3 +86,45 @@ def test_planner_matrix_vector_contractions(dev, eq, shapes):
     g = torch.Generator(device=dev).manual_seed(len(eq))
     ops = [torch.randn(s, dtype=torch.float64, device=dev, generator=g) for s in shapes]
     assert torch.allclose(contract(eq, *ops), torch.einsum(eq, *ops), rtol=1e-12, atol=1e-11)
+
+
+COMBOS = [((6, 40, 5, 45), [(2.0, (0, 1, 2, 3)), (-1.0, (0, 1, 3, 2))]),
+          ((6, 40, 5, 45), [(1.0, (0, 2, 1, 3)), (-0.5, (0, 3, 2, 1)), (0.25, (1, 0, 2, 3)), (3.0, (0, 1, 2, 3))]),
+          ((33, 33, 33, 33), [(0.5, (0, 1, 2, 3)), (0.5, (1, 0, 2, 3)), (0.5, (0, 1, 3, 2)), (0.5, (1, 0, 3, 2))]),
+          ((4, 5, 6, 7, 8), [(1.0, (4, 3, 2, 1, 0)), (2.0, (0, 1, 2, 3, 4))]),
+          ((9, 70), [(1.0, (1, 0))])]
+
+
+@pytest.mark.parametrize("shape,terms", COMBOS)
+def test_combine_matches_torch(dev, shape, terms):
+    from ccsd_gpu_amd import mfma
+    g = torch.Generator(device=dev).manual_seed(len(shape) * 7 + len(terms))
+    base = torch.randn([shape[i] for i in range(len(shape))], dtype=torch.float64, device=dev, generator=g)
+    # every term is a permuted view of one square-compatible source of the output shape
+    srcs = []
+    for a, perm in terms:
+        inv = [perm.index(i) for i in range(len(perm))]
+        x = torch.randn([shape[p] for p in inv], dtype=torch.float64, device=dev, generator=g)
+        srcs.append((a, x.permute(perm)))
+    ref = sum(a * x for a, x in srcs)
+    got = mfma.combine(srcs)
+    assert torch.allclose(got, ref, rtol=1e-14, atol=1e-13)
+    # strided destination (a block of a larger tensor), accumulate
+    big = torch.randn([s + 2 for s in shape], dtype=torch.float64, device=dev, generator=g)
+    sl = tuple(slice(1, 1 + s) for s in shape)
+    before = big.clone()
+    mfma.combine(srcs, out=big[sl], beta=1)
+    assert torch.allclose(big[sl], before[sl] + ref, rtol=1e-14, atol=1e-13)
+    mask = torch.ones_like(big, dtype=torch.bool)
+    mask[sl] = False
+    assert torch.equal(big[mask], before[mask])          # nothing outside the view is touched
+    del base
+
+
+def test_pack_pairs_sym_gpu_matches_cpu(dev):
+    from ccsd_gpu_amd.gpu_ccsd_grad import _pack_pairs_sym
+    g = torch.Generator().manual_seed(8)
+    t = torch.randn(37, 37, 19, 19, dtype=torch.float64, generator=g)
+    ref = _pack_pairs_sym(t)                             # torch on the CPU
+    got = _pack_pairs_sym(t.to(dev))
+    assert torch.allclose(got.cpu(), ref, rtol=1e-14, atol=1e-14)
