@@ -118,13 +118,14 @@ struct ChunkList {
     int n = 0;
     long long quartets = 0;
     std::vector<long long> cumq;   // quartets in the first c chunks (autotuner samples)
+    int split = 1;                  // primitive split: each quartet listed `split` times
 };
 
 // Kernel choice of one class in the fused gradient.
 struct Choice {
     int kind = -1;   // 0 generic LDS kernel, 1 register kernel
     int wpb = 4;     // waves per block (register kernel)
-    int chunk = 64;  // ket pairs per chunk
+    int chunk = 64;  // ket pairs per chunk; bits 16+: primitive split of the register kernel (see get_chunks)
 };
 
 struct BraSet {
@@ -179,6 +180,7 @@ struct ccsd_plan {
     hipEvent_t fork_ev = nullptr, join_ev[kSideStreams] = {};
     bool concurrent = true;
     bool ket_shell_order = false;                    // unscreened kets in shell order inside a primitive group
+    int force_split = 0;                             // primitive split of the register kernels (0: by wave count)
     std::vector<int> eff;                            // surviving primitive pairs of same-molecule shell pairs
     std::vector<long long> mol_eff_off;
     std::vector<C2sTermHost> c2s;                    // per spherical AO
@@ -402,10 +404,16 @@ int get_bra(ccsd_plan* p, int ish0, int ish1, BraSet** out)
 // Work chunks of (bra class bc) x (ket class kc): per bra pair, the ket pairs of
 // its molecule, per segment the Schwarz survivors (a prefix), cut into runs of at
 // most `chunk`; most expensive chunks first so that the tail of the launch is short.
-int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screened, bool fused, ChunkList** out)
+// Work list of one (bra class, ket class). `code` = ket pairs per chunk in its low 16
+// bits; a primitive split S = code >> 16 (> 1, register kernel only) lists every chunk
+// S times with w = (s << 8) | S, and copy s loops over the ket primitive pairs
+// s, s + S, ...: small classes then fill the GPU with S times as many waves (the
+// partial gradients add up in dE/dR).
+int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int code, bool screened, bool fused, ChunkList** out)
 {
     // fused gradients also drop pairs without a primitive pair past the Gaussian-product cut
-    auto key = std::make_tuple(bc, kc, chunk, (screened ? 1 : 0) | (fused ? 2 : 0));
+    auto key = std::make_tuple(bc, kc, code, (screened ? 1 : 0) | (fused ? 2 : 0));
+    const int chunk = code & 0xffff, split = std::max(1, std::min(255, code >> 16));
     auto it = bs.chunks.find(key);
     if (it == bs.chunks.end()) {
         const PairClass& B = bs.cls[bc];
@@ -438,7 +446,13 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screene
                                      p->shells[K.pairs[k0].y].nctr;
                 for (int k = k0; k < k1; k += chunk) {
                     const int ke = std::min(k1, k + chunk);
-                    work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
+                    if (split == 1) {
+                        work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
+                    } else {
+                        for (int sidx = 0; sidx < split; sidx++)
+                            work.push_back(std::make_pair(cb * ck * (ke - k) / split,
+                                                          make_int4(bq, k, ke, (sidx << 8) | split)));
+                    }
                     quartets += ke - k;
                 }
             }
@@ -455,7 +469,9 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int chunk, bool screene
         cl.quartets = quartets;
         cl.cumq.resize(h.size() + 1);
         cl.cumq[0] = 0;
+        // quartet-splits (S per quartet with a primitive split): per-quartet times divide by S
         for (size_t t = 0; t < h.size(); t++) cl.cumq[t + 1] = cl.cumq[t] + (h[t].z - h[t].y);
+        cl.split = split;
         if (cl.n) {
             HIP_TRY(hipMalloc(&cl.d, sizeof(int4) * h.size()));
             HIP_TRY(hipMemcpy(cl.d, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
@@ -562,11 +578,25 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
     std::vector<Choice> cand;
     const bool reg = p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
     if (reg) {
+        // primitive split for classes with too few waves to fill the GPU (2048 waves wanted)
+        int split = 1;
+        ChunkList* cl64 = nullptr;
+        int rc = get_chunks(p, bs, bc, kc, 64, screened, true, &cl64);
+        if (rc) return rc;
+        const long long waves =
+            (long long)cl64->n * ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl);
+        if (waves > 0 && waves < 2048)
+            split = (int)std::min<long long>({8LL, (2048 + waves - 1) / waves, (long long)a.max_nkl});
+        if (p->force_split > 0) split = std::min(255, p->force_split);
         for (int w : {4, 2, 1}) {
             if (ccsd_fused_reg_lds(a.max_nij, w) > 65536) continue;
             Choice c;
             c.kind = 1; c.wpb = w; c.chunk = 64;
-            cand.push_back(c);
+            Choice cs = c;
+            cs.chunk = 64 | (split << 16);
+            if (split > 1 && p->force_split > 0) cand.push_back(cs);   // first: taken when tuning is off
+            if (p->force_split <= 1) cand.push_back(c);
+            if (split > 1 && p->force_split == 0) cand.push_back(cs);
         }
     }
     // generic kernel: runs of 1, 4 or 16 quartets per wave; work lists kept below
@@ -603,7 +633,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             HIP_TRY(hipEventSynchronize(e1));
             float ms = 0.0f;
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            const double per_q = ms / (double)cl->cumq[nsample];
+            const double per_q = ms * cl->split / (double)cl->cumq[nsample];
             if (per_q < best_t) {
                 best_t = per_q;
                 best = c;
@@ -1367,6 +1397,13 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         clear_bra_cache(p);
         p->choice.clear();
         return build_ket(p);
+    }
+    if (option == 4) {
+        if (value < 0 || value > 255) return fail(CCSD_ERR_SHAPE, "primitive split must be 0..255");
+        p->force_split = value;
+        clear_bra_cache(p);
+        p->choice.clear();
+        return CCSD_OK;
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");
 }

@@ -288,9 +288,11 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
                 gam[cij][ckl][t] = g;
             }
 
-    // ket primitive pairs past the cut, precomputed per ket pair (no divergent skips)
+    // ket primitive pairs past the cut, precomputed per ket pair (no divergent skips);
+    // a chunk with a primitive split (w = s << 8 | S) takes pairs s, s + S, ...
     const int t1 = a.ket_pp_off[kq + 1];
-    for (int t = a.ket_pp_off[kq]; t < t1; t++) {
+    const int split = ch.w & 255, sstep = split > 1 ? split : 1;
+    for (int t = a.ket_pp_off[kq] + (split > 1 ? (ch.w >> 8) : 0); t < t1; t += sstep) {
             const double* kpd = a.ket_pp + (size_t)t * 10;
             const double akl = kpd[0];
             const double Qc[3] = {kpd[1], kpd[2], kpd[3]};

@@ -72,7 +72,11 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * (forked from and joined back to `stream` with events), 0 on `stream` alone.
  * option 3: 1 orders the (unscreened) ket pairs of each primitive group in
  * shell order, so neighbouring lanes gather neighbouring packed Gamma; 0
- * (default) by surviving primitive pairs, descending. */
+ * (default) by surviving primitive pairs, descending. option 4: primitive split
+ * of the register kernels: 0 (default) splits the ket primitive-pair loop of a
+ * class over S <= 8 waves when the class has fewer than 2048 waves of work (S
+ * then competes in the autotuner), S >= 1 fixes it (with option 1 = 0 the split
+ * register kernel is then the one taken). */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
@@ -97,7 +101,8 @@ int ccsd_plan_quartets(const ccsd_plan* plan, long long* count);
 
 /* The launches of the last integral or fused-gradient call, 10 numbers each, in
  * launch order: li, lj, lk, ll, general-contraction variant (0/1), kernel (0 LDS,
- * 1 register), waves per block, quartets per chunk, waves per quartet (register
+ * 1 register), waves per block, quartets per chunk (low 16 bits; bits 16+: the
+ * register kernel's primitive split, 0 = none), waves per quartet (register
  * kernel), quartets. *nrows receives the count; up to max_rows rows are written
  * to `rows` (may be NULL). No counterpart in the reference (profiling aid). */
 int ccsd_plan_last_launches(const ccsd_plan* plan, long long* rows, int max_rows, int* nrows);

@@ -119,6 +119,39 @@ def test_register_kernels_equal_lds_kernel(gpu, make):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("make", [molecules.h2o_ccpvdz, molecules.ch4_ccpvdz])
+@pytest.mark.parametrize("split", [2, 3, 8])
+def test_primitive_split_register_kernels_equal_lds_kernel(gpu, make, split):
+    """Register kernels with the ket primitive-pair loop split over `split` waves (forced,
+    autotuning off so the split kernel is the one run) give the LDS kernel's contraction."""
+    from ccsd_gpu_amd import _lib
+    mol = make()
+    g_t = torch.as_tensor(_random_gamma(mol, 23), device="cuda")
+    plan = _lib.get_plan(mol)
+    lib = _lib.load()
+    try:
+        _lib.check(lib.ccsd_plan_set_option(plan, 0, 0), "set_option")
+        ref = contract_int2e_ip1(mol, g_t).cpu().numpy()
+        _lib.check(lib.ccsd_plan_set_option(plan, 0, 1), "set_option")
+        _lib.check(lib.ccsd_plan_set_option(plan, 1, 0), "set_option")
+        _lib.check(lib.ccsd_plan_set_option(plan, 4, split), "set_option")
+        got = contract_int2e_ip1(mol, g_t).cpu().numpy()
+        launches = _lib_launches(mol)
+    finally:
+        _lib.check(lib.ccsd_plan_set_option(plan, 4, 0), "set_option")
+        _lib.check(lib.ccsd_plan_set_option(plan, 1, 1), "set_option")
+    scale = np.abs(ref).max()
+    print("split", split, "reg vs lds", np.abs(got - ref).max() / scale, flush=True)
+    assert np.abs(got - ref).max() <= 1e-11 * scale
+    assert any(rec["kernel"] == 1 and rec["chunk"] >> 16 == split for rec in launches)
+
+
+def _lib_launches(mol):
+    from ccsd_gpu_amd.hip.grad2e import last_launches
+    return last_launches(mol)
+
+
+@pytest.mark.gpu
 def test_fused_explicit_plus_separable_in_one_pass(gpu):
     """gamma + separable(D, P) in one kernel pass equals the explicit sum of both."""
     from ccsd_gpu_amd.gpu_ccsd_grad import _separable_gamma
