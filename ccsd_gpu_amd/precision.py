@@ -14,7 +14,9 @@ bf16x3  contractions of at least MIXED_MIN_FLOPS run on the split-BF16 MFMA
         float64 out); smaller ones stay FP64. Operands registered with
         `static_operands` (integrals and intermediates fixed during a solve)
         are split once and reused.
-fp32    as bf16x3 with the FP32-input MFMA kernel.
+fp32    contractions of at least MIXED_MIN_FLOPS in FP32 (operands rounded once,
+        static ones cached) on the BLAS library's SGEMM, FP32 accumulation,
+        float64 out; smaller ones stay FP64.
 Three-operand expressions are contracted left to right.
 """
 from contextlib import contextmanager
@@ -182,7 +184,12 @@ def _gemm_contract(eq, A, B, precision):
                      lambda: _view(A, a, batch, left, contr, size)[0])
         Bv = _cached(B, (b, "kn", tuple(batch), tuple(contr), tuple(right)),
                      lambda: _view(B, b, batch, contr, right, size)[0])
-        if precision == "fp32" and flops >= MIXED_MIN_FLOPS:
+        if precision == "fp32" and flops >= MIXED_MIN_FLOPS and Av.is_cuda:
+            # FP32 operands (static ones converted once) on the library's SGEMM
+            Af = _cached(A, (a, "mk32", tuple(batch), tuple(left), tuple(contr)), lambda: Av.float())
+            Bf = _cached(B, (b, "kn32", tuple(batch), tuple(contr), tuple(right)), lambda: Bv.float())
+            Cp = torch.bmm(Af, Bf).double()
+        elif precision == "fp32" and flops >= MIXED_MIN_FLOPS:
             Cp = mfma.bmm(Av, Bv, "fp32")
         else:
             Cp = _dgemm(Av, Bv)
@@ -243,7 +250,9 @@ def vvvv_term(l2, vvvv, precision):
         B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
         X = mfma.gemm_nt(mfma.split(A), B)[0]
     else:
-        X = mfma.bmm(A, vvvv.permute(1, 3, 0, 2).reshape(1, v * v, v * v), precision)[0]
+        B = _cached(vvvv, ("vvvv-abcd-f32",),
+                    lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).float().contiguous())
+        X = (A[0].float() @ B).double()
     X = X.reshape(-1, v, v)
     out = torch.empty_like(l2)
     out[iu, ju] = X
