@@ -117,8 +117,7 @@ struct ChunkList {
     int4* d = nullptr;
     int n = 0;
     long long quartets = 0;
-    std::vector<long long> cumq;   // quartets in the first c chunks (autotuner samples)
-    int split = 1;                  // primitive split: each quartet listed `split` times
+    std::vector<double> cumq;      // quartet equivalents in the first c entries (autotuner samples)
 };
 
 // Kernel choice of one class in the fused gradient.
@@ -405,15 +404,19 @@ int get_bra(ccsd_plan* p, int ish0, int ish1, BraSet** out)
 // its molecule, per segment the Schwarz survivors (a prefix), cut into runs of at
 // most `chunk`; most expensive chunks first so that the tail of the launch is short.
 // Work list of one (bra class, ket class). `code` = ket pairs per chunk in its low 16
-// bits; a primitive split S = code >> 16 (> 1, register kernel only) lists every chunk
-// S times with w = (s << 8) | S, and copy s loops over the ket primitive pairs
-// s, s + S, ...: small classes then fill the GPU with S times as many waves (the
-// partial gradients add up in dE/dR).
+// bits; bits 16+ select a primitive split of the register kernel: S = 2..254 lists
+// every chunk S times, 255 (kAdaptiveSplit) splits each chunk into S_c = ceil(cost /
+// (class cost / 2048)) <= 16 pieces, so that no chunk runs much longer than a
+// 2048th of the class (the expensive, heavily contracted chunks dominate a class's
+// time otherwise). Piece s of S has w = (s << 8) | S and loops over the ket
+// primitive pairs s, s + S, ...; the partial gradients add up in dE/dR.
+constexpr int kAdaptiveSplit = 255;
+
 int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int code, bool screened, bool fused, ChunkList** out)
 {
     // fused gradients also drop pairs without a primitive pair past the Gaussian-product cut
     auto key = std::make_tuple(bc, kc, code, (screened ? 1 : 0) | (fused ? 2 : 0));
-    const int chunk = code & 0xffff, split = std::max(1, std::min(255, code >> 16));
+    const int chunk = code & 0xffff, scode = std::min(255, code >> 16);
     auto it = bs.chunks.find(key);
     if (it == bs.chunks.end()) {
         const PairClass& B = bs.cls[bc];
@@ -446,16 +449,33 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int code, bool screened
                                      p->shells[K.pairs[k0].y].nctr;
                 for (int k = k0; k < k1; k += chunk) {
                     const int ke = std::min(k1, k + chunk);
-                    if (split == 1) {
-                        work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
-                    } else {
-                        for (int sidx = 0; sidx < split; sidx++)
-                            work.push_back(std::make_pair(cb * ck * (ke - k) / split,
-                                                          make_int4(bq, k, ke, (sidx << 8) | split)));
-                    }
+                    work.push_back(std::make_pair(cb * ck * (ke - k), make_int4(bq, k, ke, 0)));
                     quartets += ke - k;
                 }
             }
+        }
+        if (scode > 1) {
+            long long total = 0;
+            for (const auto& w : work) total += w.first;
+            const long long target = std::max(1LL, total / 2048);
+            const int smax = std::max(1, std::min(16, K.max_np));
+            std::vector<std::pair<long long, int4>> split_work;
+            split_work.reserve(work.size());
+            for (const auto& w : work) {
+                int S = scode;
+                if (scode == kAdaptiveSplit)
+                    S = (int)std::min<long long>(smax, std::max(1LL, (w.first + target - 1) / target));
+                if (S <= 1) {
+                    split_work.push_back(w);
+                    continue;
+                }
+                for (int sidx = 0; sidx < S; sidx++) {
+                    int4 c = w.second;
+                    c.w = (sidx << 8) | S;
+                    split_work.push_back(std::make_pair(w.first / S, c));
+                }
+            }
+            work.swap(split_work);
         }
         if (work.size() <= 4000000)
             std::stable_sort(work.begin(), work.end(),
@@ -468,10 +488,10 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int code, bool screened
         cl.n = (int)h.size();
         cl.quartets = quartets;
         cl.cumq.resize(h.size() + 1);
-        cl.cumq[0] = 0;
-        // quartet-splits (S per quartet with a primitive split): per-quartet times divide by S
-        for (size_t t = 0; t < h.size(); t++) cl.cumq[t + 1] = cl.cumq[t] + (h[t].z - h[t].y);
-        cl.split = split;
+        cl.cumq[0] = 0.0;
+        // quartet equivalents: a piece of an S-way split counts 1/S of its quartets
+        for (size_t t = 0; t < h.size(); t++)
+            cl.cumq[t + 1] = cl.cumq[t] + (double)(h[t].z - h[t].y) / std::max(1, h[t].w & 255);
         if (cl.n) {
             HIP_TRY(hipMalloc(&cl.d, sizeof(int4) * h.size()));
             HIP_TRY(hipMemcpy(cl.d, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
@@ -578,25 +598,19 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
     std::vector<Choice> cand;
     const bool reg = p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
     if (reg) {
-        // primitive split for classes with too few waves to fill the GPU (2048 waves wanted)
-        int split = 1;
-        ChunkList* cl64 = nullptr;
-        int rc = get_chunks(p, bs, bc, kc, 64, screened, true, &cl64);
-        if (rc) return rc;
-        const long long waves =
-            (long long)cl64->n * ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl);
-        if (waves > 0 && waves < 2048)
-            split = (int)std::min<long long>({8LL, (2048 + waves - 1) / waves, (long long)a.max_nkl});
-        if (p->force_split > 0) split = std::min(255, p->force_split);
+        // unsplit and adaptively split work lists (a fixed split replaces both when forced)
         for (int w : {4, 2, 1}) {
             if (ccsd_fused_reg_lds(a.max_nij, w) > 65536) continue;
             Choice c;
             c.kind = 1; c.wpb = w; c.chunk = 64;
             Choice cs = c;
-            cs.chunk = 64 | (split << 16);
-            if (split > 1 && p->force_split > 0) cand.push_back(cs);   // first: taken when tuning is off
-            if (p->force_split <= 1) cand.push_back(c);
-            if (split > 1 && p->force_split == 0) cand.push_back(cs);
+            cs.chunk = 64 | ((p->force_split > 1 ? p->force_split : kAdaptiveSplit) << 16);
+            if (p->force_split > 1) {
+                cand.push_back(cs);
+            } else {
+                cand.push_back(c);
+                if (p->force_split == 0) cand.push_back(cs);
+            }
         }
     }
     // generic kernel: runs of 1, 4 or 16 quartets per wave; work lists kept below
@@ -620,7 +634,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             int rc = get_chunks(p, bs, bc, kc, c.chunk, screened, true, &cl);
             if (rc) return rc;
             if (cl->n == 0) break;   // nothing to run in this class
-            const int ns = (int)(std::lower_bound(cl->cumq.begin() + 1, cl->cumq.end(), 262144LL) - cl->cumq.begin());
+            const int ns = (int)(std::lower_bound(cl->cumq.begin() + 1, cl->cumq.end(), 262144.0) - cl->cumq.begin());
             const int nsample = std::min(cl->n, std::max(1, ns));
             EriClassArgs t = a;
             t.chunks = cl->d;
@@ -633,7 +647,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             HIP_TRY(hipEventSynchronize(e1));
             float ms = 0.0f;
             HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-            const double per_q = ms * cl->split / (double)cl->cumq[nsample];
+            const double per_q = ms / cl->cumq[nsample];
             if (per_q < best_t) {
                 best_t = per_q;
                 best = c;
@@ -1399,7 +1413,7 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         return build_ket(p);
     }
     if (option == 4) {
-        if (value < 0 || value > 255) return fail(CCSD_ERR_SHAPE, "primitive split must be 0..255");
+        if (value < 0 || value > 254) return fail(CCSD_ERR_SHAPE, "primitive split must be 0..254");
         p->force_split = value;
         clear_bra_cache(p);
         p->choice.clear();

@@ -73,10 +73,11 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * option 3: 1 orders the (unscreened) ket pairs of each primitive group in
  * shell order, so neighbouring lanes gather neighbouring packed Gamma; 0
  * (default) by surviving primitive pairs, descending. option 4: primitive split
- * of the register kernels: 0 (default) splits the ket primitive-pair loop of a
- * class over S <= 8 waves when the class has fewer than 2048 waves of work (S
- * then competes in the autotuner), S >= 1 fixes it (with option 1 = 0 the split
- * register kernel is then the one taken). */
+ * of the register kernels: 0 (default) adds a candidate whose expensive chunks
+ * are cut into up to 16 pieces along the ket primitive-pair loop (each piece at
+ * most ~1/2048 of the class's work; the autotuner decides per class), 1 turns
+ * the split off, S = 2..254 splits every chunk S ways (with option 1 = 0 that
+ * split register kernel is the one taken). */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
