@@ -51,6 +51,8 @@ _SIGNATURES = {
                           ctypes.c_longlong, c_void_p, c_double, c_int, c_void_p, c_int, c_void_p]),
     "ccsd_l1_terms": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_double,
                               c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
+    "ccsd_unpack_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.c_longlong, c_double,
+                                  c_void_p, c_void_p]),
     "ccsd_pair_combine": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_double, c_double,
                                   c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "ccsd_gemm_bf16x3_nt_scatter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,

@@ -41,6 +41,8 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
 hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
 size_t ccsd_l1_terms_lds(int v);
+hipError_t ccsd_launch_unpack_pairs(const double* X, const long long* iu, const long long* ju, int o, int v,
+                                    long long npair, double s, double* out, hipStream_t stream);
 hipError_t ccsd_launch_permute(const double* src, int ndim, const long long* sizes, const long long* sstrides,
                                double* dst, void* hi, void* lo, hipStream_t stream);
 hipError_t ccsd_launch_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x,
@@ -1252,6 +1254,16 @@ int ccsd_l1_terms(const double* x, int o, int v, const double* y, long long sy0,
     if (ccsd_l1_terms_lds(v) > 65536) return fail(CCSD_ERR_UNSUPPORTED, "nvir too large for l1_block_kernel (<= 682)");
     hipError_t e = ccsd_launch_l1_terms(x, o, v, y, sy0, sy1, c1, c2, c3, c4, ws, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "l1_block_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_unpack_pairs(const double* x, const long long* iu, const long long* ju, int o, int v, long long npair,
+                      double scale, double* out, void* stream)
+{
+    if (!x || !iu || !ju || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0 || npair < 0 || npair > (long long)o * (o + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_unpack_pairs(x, iu, ju, o, v, npair, scale, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "unpack_pairs_kernel");
     return CCSD_OK;
 }
 

@@ -8,6 +8,10 @@
 // out[q]); the per-block parts go to a workspace that a second kernel sums in a
 // fixed order (no atomics: graph replays and eager calls agree bit for bit).
 //
+// unpack_pairs_kernel: the (i <= j) pair blocks of a doubles-shaped product (the
+// symmetric half of the l2 (ac|bd) term) written to both (i, j) and, transposed,
+// (j, i) of a dense (o, o, v, v) tensor.
+//
 // pair_combine_kernel: for every (i, j) and 32 x 32 tile of (a, b),
 //   out[i,j,a,b] = beta * out + (c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a]) * D[i,j,a,b]
 // with the transposed blocks staged through LDS so that every global read and
@@ -186,7 +190,60 @@ __global__ void __launch_bounds__(256) pair_combine_kernel(const double* __restr
     }
 }
 
+// out[i,j,a,b] = s X[p,a,b] and out[j,i,b,a] = s X[p,a,b] for the pair p = (iu[p], ju[p]),
+// iu <= ju; on the diagonal (i == j) out[i,i,a,b] = s (X[p,a,b] + X[p,b,a]) / 2. One
+// block per (pair, 32 x 32 tile); the transposed block is written from an LDS tile.
+__global__ void __launch_bounds__(256) unpack_pairs_kernel(const double* __restrict__ X,
+                                                           const long long* __restrict__ iu,
+                                                           const long long* __restrict__ ju, int o, int v,
+                                                           long long npair, double s, double* __restrict__ out)
+{
+    __shared__ double t1[PT][PT + 1];   // X[p][a0 + r][b0 + c]
+    __shared__ double t2[PT][PT + 1];   // X[p][b0 + r][a0 + c] (diagonal pairs)
+    const int nt = (v + PT - 1) / PT;
+    const long long blk = blockIdx.x;
+    const int tile = (int)(blk % ((long long)nt * nt));
+    const long long p = blk / ((long long)nt * nt);
+    if (p >= npair) return;
+    const int i = (int)iu[p], j = (int)ju[p];
+    const int a0 = (tile / nt) * PT, b0 = (tile - (tile / nt) * nt) * PT;
+    const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;
+    const long long vv = (long long)v * v;
+    const double* Xp = X + p * vv;
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int r = ty + k;
+        t1[r][tx] = (a0 + r < v && b0 + tx < v) ? Xp[(long long)(a0 + r) * v + b0 + tx] : 0.0;
+        if (i == j) t2[r][tx] = (b0 + r < v && a0 + tx < v) ? Xp[(long long)(b0 + r) * v + a0 + tx] : 0.0;
+    }
+    __syncthreads();
+    double* Oij = out + ((long long)i * o + j) * vv;
+    double* Oji = out + ((long long)j * o + i) * vv;
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int r = ty + k;
+        if (i == j) {
+            if (a0 + r < v && b0 + tx < v) Oij[(long long)(a0 + r) * v + b0 + tx] = 0.5 * s * (t1[r][tx] + t2[tx][r]);
+        } else {
+            if (a0 + r < v && b0 + tx < v) Oij[(long long)(a0 + r) * v + b0 + tx] = s * t1[r][tx];
+            // out[j,i][b0 + r][a0 + tx] = X[p][a0 + tx][b0 + r]
+            if (b0 + r < v && a0 + tx < v) Oji[(long long)(b0 + r) * v + a0 + tx] = s * t1[tx][r];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_unpack_pairs(const double* X, const long long* iu, const long long* ju, int o, int v,
+                                    long long npair, double s, double* out, hipStream_t stream)
+{
+    if (o <= 0 || v <= 0 || npair <= 0) return hipSuccess;
+    const long long nt = (v + PT - 1) / PT;
+    const long long nb = npair * nt * nt;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(unpack_pairs_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, iu, ju, o, v, npair, s, out);
+    return hipGetLastError();
+}
 
 size_t ccsd_l1_terms_lds(int v) { return (size_t)12 * v * sizeof(double); }
 

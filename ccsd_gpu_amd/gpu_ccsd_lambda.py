@@ -27,7 +27,7 @@ import torch
 
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
 from .pairops import l1_terms, pair_combine
-from .precision import contract, contract_acc, static_operands, vvvv_term
+from .precision import contract, contract_acc, split_memo, static_operands, vvvv_term
 from .scf import DEV
 
 
@@ -152,6 +152,11 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     mia1) are left out, giving A l -- used by the mixed-precision solver for
     its correction equations. `inh`, a 0-d device tensor, weights b instead
     (1: F, 0: A), read when the kernels run: one captured graph serves both."""
+    with split_memo():
+        return _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, homogeneous, inh)
+
+
+def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, homogeneous, inh):
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
 
     def acc(out, eq, a, b, alpha=1.0):
@@ -177,12 +182,12 @@ def update_lambda_gpu(mycc, t1, t2, l1, l2, eris=None, imds=None, use_cpu_vvvv=T
     mia1 = t1 * inh + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
     mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
 
+    # l2new starts as half the l2 (ac|bd) term; its singles part is 2 X_ijab - X_jiab
     if use_cpu_vvvv:
-        l2new = vvvv_term(l2, eris.vvvv, precision)
+        l2new = vvvv_term(l2, eris.vvvv, precision, scale=0.5)
     else:
         l2new = torch.zeros_like(l2)
-    l1new = l1_terms(torch.zeros_like(t1), l2new, t1, c1=2.0, c3=-1.0)
-    l2new.mul_(0.5)
+    l1new = l1_terms(torch.zeros_like(t1), l2new, t1, c1=4.0, c3=-2.0)
 
     w1 = imds.w1 - torch.diag(e_v)
     w2 = imds.w2 - torch.diag(e_o)

@@ -123,7 +123,7 @@ def gemv(A, x, y, alpha=1.0, beta=0):
     if m == 0:
         return y
     if cols == 0:
-        S = max(1, min(-(-4096 // m), k // 1024))
+        S = max(1, min(-(-16384 // m), k // 1024))
     else:
         S = max(1, min(-(-262144 // m), k // 64))
     ws = torch.empty(S * m, dtype=torch.float64, device=y.device) if S > 1 else None

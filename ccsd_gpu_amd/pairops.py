@@ -69,3 +69,17 @@ def pair_combine(X, c1=0.0, c2=0.0, c3=0.0, c4=0.0, d=None, out=None, beta=0):
                                              _strides(out), int(beta), _lib.current_stream_ptr()),
                "ccsd_pair_combine")
     return out
+
+
+def unpack_pairs(X, iu, ju, o, scale=1.0):
+    """X (npair, v, v) blocks of the pairs (iu[p], ju[p]), iu <= ju -> dense (o, o, v, v):
+    out[i,j] = scale X[p], out[j,i] = scale X[p]^T, diagonal blocks symmetrised."""
+    npair, v, v2 = X.shape
+    if v != v2 or not X.is_contiguous() or X.dtype != torch.float64 or not X.is_cuda:
+        raise ValueError("X must be a contiguous float64 (npair, v, v) device tensor")
+    iu = iu.to(torch.int64).contiguous()
+    ju = ju.to(torch.int64).contiguous()
+    out = torch.empty((o, o, v, v), dtype=torch.float64, device=X.device)
+    _lib.check(_lib.load().ccsd_unpack_pairs(_lib.ptr(X), _lib.ptr(iu), _lib.ptr(ju), o, v, npair, float(scale),
+                                             _lib.ptr(out), _lib.current_stream_ptr()), "ccsd_unpack_pairs")
+    return out

@@ -114,13 +114,45 @@ def _dgemm(Av, Bv):
     return out
 
 
+_SPLITS = None
+
+
+@contextmanager
+def split_memo():
+    """Within the block, the split-BF16 planes of a non-static operand are made once
+    per (storage, version, permuted layout) and reused by every contraction that
+    reads the same layout (the planes depend only on the permuted view, not on how
+    its indices are grouped)."""
+    global _SPLITS
+    prev = _SPLITS
+    _SPLITS = {}
+    try:
+        yield
+    finally:
+        _SPLITS = prev
+
+
 def _split_operand(X, s, g1, g2, g3, size, tag):
     """bf16 planes of X (index string s) as an (n1, n2, n3) matrix batch in group order
-    g1 g2 g3, permuted and split in one pass; cached for static operands."""
+    g1 g2 g3, permuted and split in one pass; cached for static operands (and, inside
+    `split_memo`, per layout for the others)."""
     n1, n2, n3 = prod(size[c] for c in g1), prod(size[c] for c in g2), prod(size[c] for c in g3)
     perm = [s.index(c) for c in g1 + g2 + g3]
-    return _cached(X, (s, tag, tuple(g1), tuple(g2), tuple(g3)),
-                   lambda: mfma.split_view(X.permute(perm), (n1, n2, n3)))
+
+    def make():
+        if _SPLITS is None or not X.is_cuda:
+            return mfma.split_view(X.permute(perm), (n1, n2, n3))
+        v = X.permute(perm)
+        key = (X.data_ptr(), X._version, tuple(v.shape), tuple(v.stride()))
+        hit = _SPLITS.get(key)
+        if hit is None:
+            # the entry keeps X alive, so its storage (the key) cannot be reused meanwhile
+            hit = (X, mfma.split_view(v, (v.numel(),)))
+            _SPLITS[key] = hit
+        planes = hit[1]
+        return mfma.Split(planes.hi.view(n1, n2, n3), planes.lo.view(n1, n2, n3))
+
+    return _cached(X, (s, tag, tuple(g1), tuple(g2), tuple(g3)), make)
 
 
 _OFFSETS = {}
@@ -234,11 +266,12 @@ def contract(eq, *ops, precision="fp64"):
     return cur
 
 
-def vvvv_term(l2, vvvv, precision):
+def vvvv_term(l2, vvvv, precision, scale=1.0):
     """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update).
 
     X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so only i <= j is
-    evaluated (half the o^2 v^4 work) and the rest mirrored, at every precision."""
+    evaluated (half the o^2 v^4 work) and the rest mirrored (times `scale`), at
+    every precision."""
     o, _, v, _ = l2.shape
     iu, ju = torch.triu_indices(o, o, device=l2.device)
     A = l2[iu, ju].reshape(1, -1, v * v)
@@ -254,7 +287,10 @@ def vvvv_term(l2, vvvv, precision):
                     lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).float().contiguous())
         X = (A[0].float() @ B).double()
     X = X.reshape(-1, v, v)
+    if X.is_cuda:
+        from .pairops import unpack_pairs
+        return unpack_pairs(X.contiguous(), iu, ju, o, scale)
     out = torch.empty_like(l2)
-    out[iu, ju] = X
-    out[ju, iu] = X.transpose(1, 2)
+    out[iu, ju] = X * scale
+    out[ju, iu] = X.transpose(1, 2) * scale
     return out

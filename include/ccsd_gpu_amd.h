@@ -177,6 +177,14 @@ int ccsd_sym_pairs(double* t, int n, double scale, void* stream);
 int ccsd_l1_terms(const double* x, int o, int v, const double* y, long long sy0, long long sy1, double c1, double c2,
                   double c3, double c4, double* ws, double* out, void* stream);
 
+/* Unpack pair blocks: for p < npair with (iu[p], ju[p]) = (i, j), i <= j (device int64
+ * arrays), out[i,j,a,b] = scale x[p,a,b] and out[j,i,b,a] = scale x[p,a,b]; for
+ * i == j, out[i,i,a,b] = scale (x[p,a,b] + x[p,b,a]) / 2. x (npair, v, v) and out
+ * (o, o, v, v) contiguous. The mirror step of the i <= j half of the l2 (ac|bd)
+ * term (the reference evaluates all (i, j), ccsd_gpu/gpu_ccsd_lambda.py:238-243). */
+int ccsd_unpack_pairs(const double* x, const long long* iu, const long long* ju, int o, int v, long long npair,
+                      double scale, double* out, void* stream);
+
 /* out[i,j,a,b] = beta * out + (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) * d[i,j,a,b]
  * for x and out any strided (o, o, v, v) views (host arrays of the four element
  * strides of i, j, a, b; reads and writes are coalesced when the b strides are

@@ -67,3 +67,18 @@ def test_pair_combine_matches_permuted_sums(dev, o, v):
     # the lambda update's final step: symmetrise and divide
     got3 = pair_combine(X, c1=1.0, c4=1.0, d=d)
     assert torch.allclose(got3, (X + X.permute(1, 0, 3, 2)) * d, rtol=1e-14, atol=1e-13)
+
+
+@pytest.mark.parametrize("o,v", [(5, 45), (3, 64), (1, 7)])
+def test_unpack_pairs_matches_scatter(dev, o, v):
+    from ccsd_gpu_amd.pairops import unpack_pairs
+    g = torch.Generator(device=dev).manual_seed(o * v)
+    iu, ju = torch.triu_indices(o, o, device=dev)
+    X = torch.randn(len(iu), v, v, dtype=torch.float64, device=dev, generator=g)
+    ref = torch.empty(o, o, v, v, dtype=torch.float64, device=dev)
+    ref[iu, ju] = X * 0.5
+    ref[ju, iu] = X.transpose(1, 2) * 0.5
+    diag = iu == ju
+    ref[iu[diag], iu[diag]] = 0.25 * (X[diag] + X[diag].transpose(1, 2))
+    got = unpack_pairs(X, iu, ju, o, scale=0.5)
+    assert torch.allclose(got, ref, rtol=1e-15, atol=1e-15)
