@@ -60,7 +60,7 @@ hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, do
                                   long long lda, long long ldb, long long ldc, long long sA, long long sB,
                                   long long sC, int batch, hipStream_t stream);
 hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
-                             const double* charges, int natm, int nbas, int nao, hipStream_t stream);
+                             const double* charges, int natm, int nbas, int nao, int maxprim, hipStream_t stream);
 
 namespace {
 
@@ -1215,9 +1215,12 @@ int ccsd_int1e(ccsd_plan* p, int kind, double* out, void* stream)
     const long long nn = (long long)p->nao_cart * p->nao_cart;
     const long long nmat = kind <= 2 ? 1 : (kind <= 4 ? 3 : 3LL * p->natm);
     hipError_t e = ccsd_launch_fill(out, nmat * nn, 0.0, (hipStream_t)stream);
-    if (e == hipSuccess)
+    if (e == hipSuccess) {
+        int maxprim = 1;
+        for (const ShellInfo& sh : p->shells) maxprim = std::max(maxprim, sh.nprim);
         e = ccsd_launch_int1e(kind, out, p->d_shells, p->d_env, p->d_coords, p->d_charges, p->natm, p->nbas,
-                              p->nao_cart, (hipStream_t)stream);
+                              p->nao_cart, maxprim, (hipStream_t)stream);
+    }
     if (e != hipSuccess) return hip_fail(e, "int1e_kernel");
     return CCSD_OK;
 }
