@@ -40,6 +40,8 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
                                   hipStream_t stream);
 hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
+hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
+                                      long long npair, double* out, hipStream_t stream);
 size_t ccsd_l1_terms_lds(int v);
 hipError_t ccsd_launch_unpack_pairs(const double* X, const long long* iu, const long long* ju, int o, int v,
                                     long long npair, double s, double* out, hipStream_t stream);
@@ -1327,6 +1329,16 @@ int ccsd_gemv(int cols, const double* A, long long M, long long K, long long lda
         e = ccsd_launch_sum_slices(ws, y, M, nslices, beta, (hipStream_t)stream);
         if (e != hipSuccess) return hip_fail(e, "sum_slices_kernel");
     }
+    return CCSD_OK;
+}
+
+int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m, long long npair,
+                        double* out, void* stream)
+{
+    if (!t || !kk || !ll || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || m < 0 || npair < 0 || npair > (long long)n * (n + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_pack_pairs_sym(t, kk, ll, n, m, npair, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "pack_pairs_sym_kernel");
     return CCSD_OK;
 }
 

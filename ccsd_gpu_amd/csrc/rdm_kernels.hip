@@ -1,5 +1,9 @@
 // Two-particle density helpers (HBM-bound).
 //
+// pack_pairs_sym_kernel: the (ij)(kl) symmetrisation of a half-transformed 2-RDM
+// t[k][l][p][q] folded into packing k >= l, the kl diagonal halved (the layout
+// the fused gradient kernel reads after the remaining two index transforms).
+//
 // sym_pairs_kernel: in place, for a dense (n, n, n, n) tensor t,
 //   t[p][q][r][s] <- scale * (t[p][q][r][s] + t[q][p][r][s] + t[p][q][s][r] + t[q][p][s][r])
 // for every index. One thread per orbit of the two swaps (representative
@@ -41,7 +45,62 @@ __global__ void sym_pairs_kernel(double* __restrict__ t, int n, double scale)
     }
 }
 
+constexpr int PT = 32;
+
+// out[r][a][b] = w (t[k,l,a,b] + t[l,k,a,b] + t[k,l,b,a] + t[l,k,b,a]) / 2 for the pair
+// r = (k, l) = (kk[r], ll[r]), k >= l, w = 1/2 on the diagonal (k == l), else 1;
+// t (n, n, m, m) contiguous. One block per (pair, 32 x 32 tile of (a, b)); the
+// transposed blocks come through LDS.
+__global__ void __launch_bounds__(256) pack_pairs_sym_kernel(const double* __restrict__ t,
+                                                             const long long* __restrict__ kk,
+                                                             const long long* __restrict__ ll, int n, int m,
+                                                             long long npair, double* __restrict__ out)
+{
+    __shared__ double tkl[PT][PT + 1];   // t[k][l][b0 + r][a0 + c]
+    __shared__ double tlk[PT][PT + 1];   // t[l][k][b0 + r][a0 + c]
+    const int nt = (m + PT - 1) / PT;
+    const long long blk = blockIdx.x;
+    const int tile = (int)(blk % ((long long)nt * nt));
+    const long long r = blk / ((long long)nt * nt);
+    if (r >= npair) return;
+    const int k = (int)kk[r], l = (int)ll[r];
+    const int a0 = (tile / nt) * PT, b0 = (tile - (tile / nt) * nt) * PT;
+    const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;
+    const long long mm = (long long)m * m;
+    const double* Tkl = t + ((long long)k * n + l) * mm;
+    const double* Tlk = t + ((long long)l * n + k) * mm;
+#pragma unroll
+    for (int q = 0; q < PT; q += 8) {
+        const int b = b0 + ty + q, a = a0 + tx;
+        const bool ok = b < m && a < m;
+        tkl[ty + q][tx] = ok ? Tkl[(long long)b * m + a] : 0.0;
+        tlk[ty + q][tx] = ok ? Tlk[(long long)b * m + a] : 0.0;
+    }
+    __syncthreads();
+    const double w = (k == l) ? 0.25 : 0.5;
+    double* O = out + r * mm;
+#pragma unroll
+    for (int q = 0; q < PT; q += 8) {
+        const int a = a0 + ty + q, b = b0 + tx;
+        if (a < m && b < m) {
+            const long long e = (long long)a * m + b;
+            O[e] = w * ((Tkl[e] + Tlk[e]) + (tkl[tx][ty + q] + tlk[tx][ty + q]));
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
+                                      long long npair, double* out, hipStream_t stream)
+{
+    if (n <= 0 || m <= 0 || npair <= 0) return hipSuccess;
+    const long long nt = (m + PT - 1) / PT;
+    const long long nb = npair * nt * nt;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(pack_pairs_sym_kernel, dim3((unsigned)nb), dim3(256), 0, stream, t, kk, ll, n, m, npair, out);
+    return hipGetLastError();
+}
 
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream)
 {

@@ -188,10 +188,16 @@ def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
 def _pack_pairs_sym(t):
     """t (n, n, m, m) -> G (n(n+1)/2, m, m) with G[k(k+1)/2 + l, p, q] =
     (t[k,l,p,q] + t[l,k,p,q] + t[k,l,q,p] + t[l,k,q,p]) / 2 for k >= l (the
-    (ij)(kl) symmetrisation with weight 1/2 per term) and the k == l rows halved:
-    one combine launch per k (its rows l <= k are contiguous in G)."""
+    (ij)(kl) symmetrisation with weight 1/2 per term) and the k == l rows halved, in
+    one kernel launch on the device (torch on the CPU)."""
     n, _, m, _ = t.shape
     tp = torch.empty((n * (n + 1) // 2, m, m), dtype=t.dtype, device=t.device)
+    if t.is_cuda and t.is_contiguous() and t.dtype == torch.float64:
+        from . import _lib
+        kk, ll = torch.tril_indices(n, n, device=t.device)
+        _lib.check(_lib.load().ccsd_pack_pairs_sym(_lib.ptr(t), _lib.ptr(kk), _lib.ptr(ll), n, m, kk.numel(),
+                                                   _lib.ptr(tp), _lib.current_stream_ptr()), "ccsd_pack_pairs_sym")
+        return tp
     for k in range(n):
         r0 = k * (k + 1) // 2
         a, b = t[k, :k + 1], t[:k + 1, k]

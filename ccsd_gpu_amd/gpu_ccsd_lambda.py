@@ -25,6 +25,7 @@ import time
 import numpy as np
 import torch
 
+from . import mfma
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
 from .pairops import l1_terms, pair_combine
 from .precision import contract, contract_acc, split_memo, static_operands, vvvv_term
@@ -69,16 +70,16 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     w3 = w3 + C("jkcd,kdcb->bj", theta, ovvv)
     wVOov = C("jbcd,kd->bjkc", ovvv, t1)
     wvOOv = C("cbjd,kd->cjkb", vvov, -t1)
-    g2vovv = vvov.permute(0, 2, 1, 3) * 2 - vvov.permute(0, 2, 3, 1)
+    g2vovv = mfma.combine([(2, vvov.permute(0, 2, 1, 3)), (-1, vvov.permute(0, 2, 3, 1))])
     wvooo = C("cibd,jkbd->ckij", g2vovv, tau)
     del g2vovv
     wvvov = C("jabd,jkcd->abkc", ovvv, t2) * -1.5
-    wvvov = wvvov + vvov.permute(0, 3, 2, 1) * 2 - vvov
-    g2vvov = vvov * 2 - ovvv.permute(1, 2, 0, 3)
+    mfma.combine([(2, vvov.permute(0, 3, 2, 1)), (-1, vvov)], out=wvvov, beta=1)
+    g2vvov = mfma.combine([(2, vvov), (-1, ovvv.permute(1, 2, 0, 3))])
     theta = t2 * 2 - t2.permute(0, 1, 3, 2)
     vackb = C("acjd,kjbd->ackb", g2vvov, theta)
     del g2vvov
-    wvvov = wvvov + vackb.permute(0, 3, 2, 1) - vackb * 0.5
+    mfma.combine([(1, vackb.permute(0, 3, 2, 1)), (-0.5, vackb)], out=wvvov, beta=1)
     del vackb
 
     ovoo = eris.ovoo
@@ -98,7 +99,7 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     woooo = woooo + C("cijd,klcd->ijkl", voov, tau)
-    g2voov = voov * 2 - voov.permute(0, 2, 1, 3)
+    g2voov = mfma.combine([(2, voov), (-1, voov.permute(0, 2, 1, 3))])
     tmpw4 = C("ckld,ld->kc", g2voov, t1)
     w1 = w1 - C("ckja,kjcb->ba", g2voov, t2) - C("ja,jb->ba", tmpw4, t1)
     w2 = w2 + C("jkbc,bikc->ij", t2, g2voov) + C("ib,jb->ij", tmpw4, t1)

@@ -167,6 +167,15 @@ int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int
  * two permuted copies, ccsd_gpu/gpu_ccsd_grad.py:324-326). */
 int ccsd_sym_pairs(double* t, int n, double scale, void* stream);
 
+/* out[r][a][b] = w (t[k,l,a,b] + t[l,k,a,b] + t[k,l,b,a] + t[l,k,b,a]) / 2 for the
+ * pairs r < npair with (kk[r], ll[r]) = (k, l), k >= l (device int64 arrays), w =
+ * 1/2 on the diagonal k == l, else 1; t (n, n, m, m) and out (npair, m, m)
+ * contiguous. The (ij)(kl) symmetrisation of the AO 2-RDM
+ * (ccsd_gpu/gpu_ccsd_grad.py:324-326) and its kl packing with the diagonal
+ * halved (:430-431), done once on the half-transformed density. */
+int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m, long long npair,
+                        double* out, void* stream);
+
 /* Singles terms of the lambda update that contract a doubles-shaped tensor
  * x (o, o, v, v, contiguous) with a singles-shaped y (y[j][b] at j*sy0 + b*sy1):
  *   out[i][a] += sum_{j,b} (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) y[j,b]
