@@ -35,7 +35,7 @@ from .hip.grad2e import contract_int2e_ip1
 from .hip.int1e_gpu import int1e
 from .hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
 from . import mfma
-from .precision import contract
+from .precision import contract, split_memo
 from .scf import DEV
 
 E = torch.einsum
@@ -335,9 +335,10 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     t0 = time.perf_counter()
 
     # Phase 1
-    d1 = _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
+    with split_memo():       # operands split once per layout across both RDM builders
+        d1 = _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
+        d2 = _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
     doo, dov, dvo, dvv = d1
-    d2 = _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision)
     t0 = mark("rdm", t0)
 
     # Phase 2: Cartesian AO 2-RDM (symmetrized)
