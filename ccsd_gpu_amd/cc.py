@@ -144,18 +144,47 @@ class DIIS:
         self.count = 0
         self.prev = None
 
+    def _alloc(self, n, like):
+        if self.X is None:
+            # one row more than the history: the first vector, kept as the first error's base
+            self.X = torch.empty((self.space + 1, n), dtype=like.dtype, device=like.device)
+            self.E = torch.empty((self.space, n), dtype=like.dtype, device=like.device)
+
     def update(self, x):
         if self.prev is None:
             self.prev = x
             return x
-        if self.X is None:
-            self.X = torch.empty((self.space, x.numel()), dtype=x.dtype, device=x.device)
-            self.E = torch.empty_like(self.X)
+        self._alloc(x.numel(), x)
+        self.X[self.count % self.space].copy_(x)
+        return self._extrapolate()[0]
+
+    def push(self, parts, rhs=None, start=None):
+        """update() of x = concat(flattened parts) (+ rhs), written straight into the
+        history buffer (no separate concatenated copy). Returns (extrapolated vector,
+        ||x - previous vector||): the previous vector is the last one returned, or
+        `start` on the first call (None: no norm)."""
+        n = sum(p.numel() for p in parts)
+        self._alloc(n, parts[0])
+        first = self.prev is None
+        dst = self.X[self.space] if first else self.X[self.count % self.space]
+        o = 0
+        for p in parts:
+            seg = dst[o:o + p.numel()]
+            if rhs is None:
+                seg.copy_(p.reshape(-1))
+            else:
+                torch.add(p.reshape(-1), rhs[o:o + p.numel()], out=seg)
+            o += p.numel()
+        if first:
+            self.prev = dst
+            return dst, (None if start is None else float(torch.linalg.norm(dst - start)))
+        return self._extrapolate()
+
+    def _extrapolate(self):
         slot = self.count % self.space
         self.count += 1
         n = min(self.count, self.space)
-        self.X[slot].copy_(x)
-        torch.sub(x, self.prev, out=self.E[slot])
+        torch.sub(self.X[slot], self.prev, out=self.E[slot])
         row = (self.E[:n] @ self.E[slot]).cpu().numpy()
         self.B[slot, :n] = row
         self.B[:n, slot] = row
@@ -164,10 +193,15 @@ class DIIS:
         A[n, n] = 0.0
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
-        c = np.linalg.lstsq(A, rhs, rcond=None)[0][:n]
-        out = torch.as_tensor(c, dtype=x.dtype, device=x.device) @ self.X[:n]
+        c = torch.as_tensor(np.linalg.lstsq(A, rhs, rcond=None)[0][:n], dtype=self.X.dtype, device=self.X.device)
+        if self.X.is_cuda:
+            # sum_s c_s X[s]: the column GEMV (one coalesced pass over the n stored vectors)
+            from . import mfma
+            out = mfma.gemv(self.X[:n].T, c, torch.empty(self.X.shape[1], dtype=self.X.dtype, device=self.X.device))
+        else:
+            out = c @ self.X[:n]
         self.prev = out
-        return out
+        return out, float(np.sqrt(max(row[slot], 0.0)))
 
 
 def amplitudes_to_vector(t1, t2):

@@ -38,7 +38,7 @@ hipError_t ccsd_launch_scatter_sum(const double* ws, int S, int nb, int M, int N
                                    hipStream_t stream);
 hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld,
                                   hipStream_t stream);
-hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
+hipError_t ccsd_launch_sum_slices(double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream);
@@ -1342,7 +1342,7 @@ int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* l
     return CCSD_OK;
 }
 
-int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int beta, void* stream)
+int ccsd_sum_slices(double* ws, double* out, long long n, int nslices, int beta, void* stream)
 {
     if (!ws || !out) return fail(CCSD_ERR_NULL, "null pointer");
     if (n < 0 || nslices < 0 || (beta != 0 && beta != 1)) return fail(CCSD_ERR_SHAPE, "bad slice-sum arguments");

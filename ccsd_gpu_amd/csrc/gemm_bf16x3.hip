@@ -230,26 +230,57 @@ __global__ void scatter_sum_kernel(const double* __restrict__ ws, int S, int nb,
     }
 }
 
-// out[i] = beta * out[i] + sum_s ws[s * n + i]: the partial products of a K-split
-// GEMM summed in a fixed order (one pass, no atomics, no memset).
+// out[i] = beta * out[i] + sum_q ws[q * rstride + i], q < S: the partial products of a
+// K-split GEMM summed in a fixed order (one pass, no atomics, no memset).
 __global__ void sum_slices_kernel(const double* __restrict__ ws, double* __restrict__ out, long long n, int S,
-                                  int beta)
+                                  long long rstride, int beta)
 {
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
         double s = beta ? out[t] : 0.0;
-        for (int q = 0; q < S; q++) s += ws[(long long)q * n + t];
+        for (int q = 0; q < S; q++) s += ws[(long long)q * rstride + t];
         out[t] = s;
+    }
+}
+
+// First level of a long slice sum: group g (blockIdx.y) adds its rows [g L, g L + L) of
+// ws in order and leaves the partial in place of its first row. Many slices over few
+// outputs then run on (outputs x groups) threads instead of one thread per output.
+__global__ void slice_partial_kernel(double* __restrict__ ws, long long n, int rows, long long rstride, int L)
+{
+    const int q0 = blockIdx.y * L;
+    const int q1 = q0 + L < rows ? q0 + L : rows;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int q = q0; q < q1; q++) s += ws[(long long)q * rstride + t];
+        ws[(long long)q0 * rstride + t] = s;
     }
 }
 
 }  // namespace
 
-hipError_t ccsd_launch_sum_slices(const double* ws, double* out, long long n, int S, int beta, hipStream_t stream)
+hipError_t ccsd_launch_sum_slices(double* ws, double* out, long long n, int S, int beta, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
     long long nb = (n + 255) / 256;
     if (nb > 65536LL * 8) nb = 65536LL * 8;
-    hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)nb), dim3(256), 0, stream, ws, out, n, S, beta);
+    // while the slices outnumber what one thread per output should add, sum them in
+    // groups of >= 8 first (partials in place), enough groups to give ~1024 workgroups
+    int rows = S;
+    long long rstride = n;
+    while (rows > 16 && nb < 1024) {
+        long long G = (1024 + nb - 1) / nb;
+        const long long gmax = (rows + 7) / 8;
+        if (G > gmax) G = gmax;
+        if (G < 2) break;
+        const int L = (int)((rows + G - 1) / G);
+        G = (rows + L - 1) / L;
+        hipLaunchKernelGGL(slice_partial_kernel, dim3((unsigned)nb, (unsigned)G), dim3(256), 0, stream, ws, n, rows,
+                           rstride, L);
+        rows = (int)G;
+        rstride *= L;
+    }
+    hipLaunchKernelGGL(sum_slices_kernel, dim3((unsigned)nb), dim3(256), 0, stream, (const double*)ws, out, n, rows,
+                       rstride, beta);
     return hipGetLastError();
 }
 

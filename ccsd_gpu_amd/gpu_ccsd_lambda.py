@@ -345,13 +345,15 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
                 n1, n2 = update(l1, l2, prec, False)
             else:
                 n1, n2 = update(l1, l2, prec, True)
-            vec = amplitudes_to_vector(n1, n2)
-            if rhs is not None:
-                vec += rhs_vec
-            normt = float(torch.linalg.norm(vec - prev))
-            history.append((prec, rhs is not None, normt))
             if adiis is not None:
-                vec = adiis.update(vec)
+                # the change norm is the newest DIIS error (x_new - previous vector)
+                vec, normt = adiis.push((n1, n2), rhs_vec, start=prev if it == 0 else None)
+            else:
+                vec = amplitudes_to_vector(n1, n2)
+                if rhs is not None:
+                    vec += rhs_vec
+                normt = float(torch.linalg.norm(vec - prev))
+            history.append((prec, rhs is not None, normt))
             prev = vec
             l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
             if normt < tol_:

@@ -135,7 +135,7 @@ def gemv(A, x, y, alpha=1.0, beta=0):
 
 def sum_slices(ws, out, beta=0):
     """out (+)= ws.sum(0) for a contiguous (S, ...) workspace and a contiguous out of
-    ws.shape[1:], by the library's fixed-order slice-sum kernel."""
+    ws.shape[1:], by the library's fixed-order slice-sum kernels (ws is overwritten)."""
     if not (ws.is_contiguous() and out.is_contiguous()) or tuple(ws.shape[1:]) != tuple(out.shape):
         raise ValueError("sum_slices needs contiguous ws (S, ...) and out of shape ws.shape[1:]")
     _lib.check(_lib.load().ccsd_sum_slices(_lib.ptr(ws), _lib.ptr(out), out.numel(), ws.shape[0], int(beta),

@@ -155,10 +155,12 @@ int ccsd_gemm_mixed(int mode, const double* A, const double* B, double* C, int M
 int ccsd_split_bf16(const double* x, void* hi, void* lo, long long rows, long long cols, long long ld, void* stream);
 
 /* out[i] = beta * out[i] + sum_{s < nslices} ws[s * n + i] (device arrays), in a
- * fixed order: the reduction of a K-split GEMM's partial products. Launches one
- * plain kernel (no memset), so it may sit inside a captured HIP graph. No
- * counterpart in the reference (its einsums are single cuBLAS/cuTENSOR calls). */
-int ccsd_sum_slices(const double* ws, double* out, long long n, int nslices, int beta, void* stream);
+ * fixed order: the reduction of a K-split GEMM's partial products. ws is scratch:
+ * many slices over few outputs are summed in groups first, with the group
+ * partials left in ws. Plain kernels only (no memset), so the call may sit inside
+ * a captured HIP graph. No counterpart in the reference (its einsums are single
+ * cuBLAS/cuTENSOR calls). */
+int ccsd_sum_slices(double* ws, double* out, long long n, int nslices, int beta, void* stream);
 
 /* In place on a dense (n, n, n, n) device tensor:
  *   t[p][q][r][s] <- scale * (t[pqrs] + t[qprs] + t[pqsr] + t[qpsr]).

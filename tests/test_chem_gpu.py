@@ -53,7 +53,7 @@ def test_bf16x3_nt_gemm_matches_fp64(gpu):
     from ccsd_gpu_amd import mfma
     gen = torch.Generator(device="cuda").manual_seed(7)
     for (nb, m, k, n) in [(1, 128, 32, 128), (2, 100, 77, 130), (1, 1, 300, 7), (3, 257, 1000, 65),
-                          (1, 300, 64, 1000)]:
+                          (1, 300, 64, 1000), (2, 256, 640, 300)]:
         A = torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen)
         B = torch.randn((nb, n, k), dtype=torch.float64, device="cuda", generator=gen)
         ref = A @ B.transpose(1, 2)

@@ -77,6 +77,22 @@ def test_gemv_matches_torch(dev, m, k, layout):
     assert torch.equal(mfma.gemv(A, x, torch.empty(m, dtype=torch.float64, device=dev)), got)   # deterministic
 
 
+@pytest.mark.parametrize("S,n", [(1, 5), (16, 3), (17, 1), (16384, 1), (497, 6080), (451, 1792), (3, 100000),
+                                 (223, 23104)])
+def test_sum_slices_matches_torch(dev, S, n):
+    """Fixed-order slice sum (grouped in place for many slices over few outputs)."""
+    from ccsd_gpu_amd import mfma
+    g = torch.Generator(device=dev).manual_seed(S * 7 + n)
+    ws = torch.randn(S, n, dtype=torch.float64, device=dev, generator=g)
+    y0 = torch.randn(n, dtype=torch.float64, device=dev, generator=g)
+    ref = ws.sum(0)
+    got = mfma.sum_slices(ws.clone(), torch.empty(n, dtype=torch.float64, device=dev))
+    assert torch.allclose(got, ref, rtol=1e-12, atol=1e-11 * S ** 0.5)
+    got2 = mfma.sum_slices(ws.clone(), y0.clone(), beta=1)
+    assert torch.allclose(got2, y0 + ref, rtol=1e-12, atol=1e-11 * S ** 0.5)
+    assert torch.equal(mfma.sum_slices(ws.clone(), torch.empty(n, dtype=torch.float64, device=dev)), got)
+
+
 @pytest.mark.parametrize("eq,shapes", [("iajb,bj->ai", [(9, 31, 9, 31), (31, 9)]),
                                        ("jb,aijb->ia", [(9, 31), (31, 9, 9, 31)]),
                                        ("iabc,bc->ia", [(4, 30, 30, 30), (30, 30)]),
