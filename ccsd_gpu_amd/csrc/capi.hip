@@ -40,6 +40,7 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
                                   hipStream_t stream);
 hipError_t ccsd_launch_sum_slices(double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
+hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream);
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream);
 size_t ccsd_l1_terms_lds(int v);
@@ -1339,6 +1340,15 @@ int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* l
     if (n < 0 || m < 0 || npair < 0 || npair > (long long)n * (n + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
     hipError_t e = ccsd_launch_pack_pairs_sym(t, kk, ll, n, m, npair, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "pack_pairs_sym_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream)
+{
+    if (!z || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || m < 0 || (long long)n * (n + 1) / 2 > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_repack_pairs(z, n, m, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "repack_pairs_kernel");
     return CCSD_OK;
 }
 

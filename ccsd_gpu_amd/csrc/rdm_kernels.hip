@@ -4,6 +4,10 @@
 // t[k][l][p][q] folded into packing k >= l, the kl diagonal halved (the layout
 // the fused gradient kernel reads after the remaining two index transforms).
 //
+// repack_pairs_kernel: z[k][l][p] (n, n, P) with p = (i, j) packed i >= j (P = m(m+1)/2)
+// -> out[r][i][j] (n(n+1)/2, m, m) for the pairs r = (k, l), k >= l: the kl pair packed,
+// the ij pair unpacked, with the packing weights moved from ij to kl.
+//
 // sym_pairs_kernel: in place, for a dense (n, n, n, n) tensor t,
 //   t[p][q][r][s] <- scale * (t[p][q][r][s] + t[q][p][r][s] + t[p][q][s][r] + t[q][p][s][r])
 // for every index. One thread per orbit of the two swaps (representative
@@ -89,7 +93,41 @@ __global__ void __launch_bounds__(256) pack_pairs_sym_kernel(const double* __res
     }
 }
 
+// out[r][i][j] = (wkl / wij) z[k][l][p(i, j)] for r = k (k + 1) / 2 + l, k >= l, with
+// p(i, j) = a (a + 1) / 2 + b for a = max(i, j), b = min(i, j), wkl = 1 (k == l) else 2
+// and wij = 1 (i == j) else 2. One block per pair r; wave w writes rows i = w, w + 4, ...
+// (coalesced along j), the gathered row of z is read from cache.
+__global__ void __launch_bounds__(256) repack_pairs_kernel(const double* __restrict__ z, int n, int m,
+                                                          double* __restrict__ out)
+{
+    const long long r = blockIdx.x;
+    int k, l;
+    pair_of(r, k, l);
+    const long long P = (long long)m * (m + 1) / 2;
+    const double* Z = z + ((long long)k * n + l) * P;
+    double* O = out + r * m * m;
+    const double wkl = (k == l) ? 1.0 : 2.0;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = wave; i < m; i += 4) {
+        const long long ri = (long long)i * (i + 1) / 2;
+        for (int j = lane; j < m; j += 64) {
+            const long long p = (j <= i) ? ri + j : (long long)j * (j + 1) / 2 + i;
+            const double f = (i == j) ? wkl : 0.5 * wkl;
+            O[(long long)i * m + j] = f * Z[p];
+        }
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream)
+{
+    if (n <= 0 || m <= 0) return hipSuccess;
+    const long long npair = (long long)n * (n + 1) / 2;
+    if (npair > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(repack_pairs_kernel, dim3((unsigned)npair), dim3(256), 0, stream, z, n, m, out);
+    return hipGetLastError();
+}
 
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream)

@@ -170,19 +170,41 @@ def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
     """The same AO 2-RDM, packed as the fused gradient kernel and Imat take it:
     G[i][j][k(k+1)/2 + l] for k >= l, the kl diagonal halved (nao^2 npair).
 
-    The two kl transforms run on the full MO tensor; the (pq)(rs) symmetrisation
-    commutes with them and is applied while k >= l is packed out of their product,
-    so the i and j transforms do half the work and write the packed layout
-    directly."""
+    The (pq)(rs) symmetrisation commutes with the index transforms, so it is applied
+    to the MO tensor while p >= q is packed out of it: the r and s transforms then
+    run on half the rows. Their product is repacked (rs pair packed, pq unpacked)
+    and the p and q transforms write the packed layout directly."""
     dm2 = _assemble_dm2_mo(d2)
     c = _dev(mo_coeff)
-    t = _rotate_transform(dm2, c, precision)     # (l p q r)
+    nmo = c.shape[1]
+    tp = _pack_pairs_sym(dm2)                    # (pq r s), p >= q, symmetrised
     del dm2
-    t = _rotate_transform(t, c, precision)       # (k l p q)
-    tp = _pack_pairs_sym(t)                      # (kl p q), k >= l, symmetrised, diagonal halved
-    del t
-    tp = _rotate_transform(tp, c, precision)     # (j kl p)
-    return _rotate_transform(tp, c, precision)   # (i j kl)
+    t = _rotate_transform(tp, c, precision)      # (s pq r)
+    del tp
+    t = _rotate_transform(t, c, precision)       # (r s pq)
+    t = _repack_pairs(t, nmo)                    # (rs p q), r >= s, the rs diagonal halved
+    t = _rotate_transform(t, c, precision)       # (q rs p)
+    return _rotate_transform(t, c, precision)    # (p q rs)
+
+
+def _repack_pairs(z, m):
+    """z (n, n, m(m+1)/2): the second pair packed with weight 2 off its diagonal (the
+    _pack_pairs_sym layout) -> (n(n+1)/2, m, m): the first pair packed k >= l with
+    that weight, the second unpacked (one kernel on the device, torch on the CPU)."""
+    n = z.shape[0]
+    out = torch.empty((n * (n + 1) // 2, m, m), dtype=z.dtype, device=z.device)
+    if z.is_cuda and z.is_contiguous() and z.dtype == torch.float64:
+        from . import _lib
+        _lib.check(_lib.load().ccsd_repack_pairs(_lib.ptr(z), n, m, _lib.ptr(out), _lib.current_stream_ptr()),
+                   "ccsd_repack_pairs")
+        return out
+    kk, ll = torch.tril_indices(n, n, device=z.device)
+    i = torch.arange(m, device=z.device)
+    a, b = torch.maximum(i[:, None], i[None, :]), torch.minimum(i[:, None], i[None, :])
+    p = a * (a + 1) // 2 + b
+    wkl = torch.where(kk == ll, 1.0, 2.0).to(z.dtype)
+    wij = torch.where(i[:, None] == i[None, :], 1.0, 2.0).to(z.dtype)
+    return torch.mul(z[kk, ll][:, p], wkl[:, None, None] / wij[None], out=out)
 
 
 def _pack_pairs_sym(t):

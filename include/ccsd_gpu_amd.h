@@ -178,6 +178,15 @@ int ccsd_sym_pairs(double* t, int n, double scale, void* stream);
 int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m, long long npair,
                         double* out, void* stream);
 
+/* out[r][i][j] = (w_kl / w_ij) z[k][l][p(i,j)] for r = k(k+1)/2 + l, k >= l < n, with
+ * p(i,j) = a(a+1)/2 + b (a = max(i,j), b = min(i,j)), w = 1 on a pair's diagonal
+ * else 2; z (n, n, m(m+1)/2) and out (n(n+1)/2, m, m) contiguous device arrays.
+ * With the 2-RDM symmetrised and packed over its first (MO) pair before the AO
+ * transform of the second pair, this moves the packing to the second (AO) pair,
+ * the layout of ccsd_gpu/gpu_ccsd_grad.py:330-333 and :430-431 for the first
+ * pair's remaining transforms. */
+int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream);
+
 /* Singles terms of the lambda update that contract a doubles-shaped tensor
  * x (o, o, v, v, contiguous) with a singles-shaped y (y[j][b] at j*sy0 + b*sy1):
  *   out[i][a] += sum_{j,b} (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) y[j,b]

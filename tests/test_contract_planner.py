@@ -130,6 +130,27 @@ def test_packed_transform_equals_packed_full_transform():
     assert torch.allclose(got, ref, rtol=1e-12, atol=1e-12)
 
 
+def test_mo_packed_then_repacked_transform_equals_packed_full_transform():
+    """MO->AO with the first pair symmetrised and packed in the MO basis, the second
+    pair transformed, the packing moved to the second (AO) pair, then the first pair
+    transformed: equals the full transform of the (pq)(rs)-symmetrised D packed over
+    k >= l with weight 2 off the diagonal (1 on it), for a general D."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _pack_pairs_sym, _repack_pairs, _rotate_transform
+    g = torch.Generator().manual_seed(8)
+    d = torch.randn(5, 5, 5, 5, dtype=torch.float64, generator=g)
+    c = torch.randn(7, 5, dtype=torch.float64, generator=g)
+    full = torch.einsum("ip,jq,kr,ls,pqrs->ijkl", c, c, c, c, d)
+    sym = 0.25 * (full + full.permute(1, 0, 2, 3) + full.permute(0, 1, 3, 2) + full.permute(1, 0, 3, 2))
+    k, l = torch.tril_indices(7, 7)
+    ref = 2.0 * sym[:, :, k, l]
+    ref[:, :, k == l] *= 0.5
+    t = _rotate_transform(_rotate_transform(_pack_pairs_sym(d), c, "fp64"), c, "fp64")
+    t = _repack_pairs(t, 5)
+    got = _rotate_transform(_rotate_transform(t, c, "fp64"), c, "fp64")
+    assert got.shape == ref.shape
+    assert torch.allclose(got, ref, rtol=1e-12, atol=1e-12)
+
+
 def test_pack_pairs_sym_symmetrises_and_packs():
     """(ij)(kl) symmetrisation (weight 1/2 per term) folded into the k >= l packing."""
     from ccsd_gpu_amd.gpu_ccsd_grad import _pack_pairs_sym

@@ -144,3 +144,13 @@ def test_pack_pairs_sym_gpu_matches_cpu(dev):
     ref = _pack_pairs_sym(t)                             # torch on the CPU
     got = _pack_pairs_sym(t.to(dev))
     assert torch.allclose(got.cpu(), ref, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("n,m", [(37, 19), (5, 70), (1, 1)])
+def test_repack_pairs_gpu_matches_cpu(dev, n, m):
+    from ccsd_gpu_amd.gpu_ccsd_grad import _repack_pairs
+    g = torch.Generator().manual_seed(n * 100 + m)
+    z = torch.randn(n, n, m * (m + 1) // 2, dtype=torch.float64, generator=g)
+    ref = _repack_pairs(z, m)                             # torch on the CPU
+    got = _repack_pairs(z.to(dev), m)
+    assert torch.equal(got.cpu(), ref)
