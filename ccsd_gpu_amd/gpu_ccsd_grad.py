@@ -273,7 +273,9 @@ def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
     fvind is the reference's 2 C_v^T (J - K/2)[D] C_o with D = C_v x C_o^T + transpose
     (gpu_ccsd_grad.py:644-651). Its vo block needs only MO integrals the CCSD
     already holds: 4 (ai|bj) x_bj - (ab|ij) x_bj - (aj|ib) x_bj, i.e. contractions
-    over ovov and oovv (o^2 v^2 each) instead of JK builds over the nao^4 AO tensor."""
+    over ovov and oovv (o^2 v^2 each) instead of JK builds over the nao^4 AO tensor.
+    The three pieces are summed once into one (vo, vo) matrix, so each CG step is
+    one matrix-vector product."""
     mf = mycc._scf
     nocc = mycc.nocc
     c = _dev(mycc.mo_coeff)
@@ -281,10 +283,13 @@ def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
     eai = e[nocc:, None] - e[None, :nocc]
     eris = mycc.ao2mo()
     ovov, oovv = eris.ovov, eris.oovv
+    nvir = eai.shape[0]
+    # H[a,i,b,j] = 4 (ia|jb) - (ij|ab) - (ja|ib)
+    H = mfma.combine([(4.0, ovov.permute(1, 0, 3, 2)), (-1.0, oovv.permute(2, 0, 3, 1)),
+                      (-1.0, ovov.permute(1, 2, 3, 0))]).reshape(nvir * nocc, nvir * nocc)
 
     def fvind(x):
-        return (4.0 * contract("iajb,bj->ai", ovov, x) - contract("ijab,bj->ai", oovv, x)
-                - contract("jaib,bj->ai", ovov, x))
+        return (H @ x.reshape(-1)).reshape(nvir, nocc)
 
     def A(x):
         return eai * x + fvind(x)

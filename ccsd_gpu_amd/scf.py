@@ -50,15 +50,44 @@ class AOIntegrals:
     def to_native(self, m):
         return self.c2s.T @ m @ self.c2s
 
-    def get_jk(self, dm):
-        """J, K of a (nao, nao) density in the native basis (two GEMVs over HBM-resident ERIs)."""
+    def get_jk(self, dm, hermi=1):
+        """J, K of a (nao, nao) density in the native basis (two GEMVs over HBM-resident
+        ERIs). hermi=1 (PySCF's convention: dm symmetric) reads the pair-packed forms,
+        half the bytes of each full one twice over:
+            J_(ij) = sum_{k>=l} (ij|kl) w_kl D_kl,
+            K_(ik) = sum_{j>=l} [(ij|kl) + (il|kj)] (w_jl / 2) D_jl,   w = 2 off the diagonal, 1 on it,
+        both for i >= j (i >= k) and mirrored."""
         nc = self.eri_c.shape[0]
+        dc = self.to_cart(dm)
+        if hermi == 1:
+            jpp, kpp, (a, b) = self._jk_packed()
+            dk = dc[a, b] * torch.where(a == b, 0.5, 1.0).to(dc.dtype)
+            jp, kp = jpp @ (2.0 * dk), kpp @ dk
+            j = torch.empty((nc, nc), dtype=dc.dtype, device=dc.device)
+            k = torch.empty_like(j)
+            j[a, b] = jp
+            j[b, a] = jp
+            k[a, b] = kp
+            k[b, a] = kp
+            return self.to_native(j), self.to_native(k)
         if getattr(self, "_eri_k", None) is None:
             self._eri_k = self.eri_c.permute(0, 2, 1, 3).contiguous()    # (ik|jl) as [i, j, k, l]
-        dc = self.to_cart(dm).reshape(-1)
-        j = (self.eri_c.reshape(nc * nc, nc * nc) @ dc).reshape(nc, nc)
-        k = (self._eri_k.reshape(nc * nc, nc * nc) @ dc).reshape(nc, nc)
+        j = (self.eri_c.reshape(nc * nc, nc * nc) @ dc.reshape(-1)).reshape(nc, nc)
+        k = (self._eri_k.reshape(nc * nc, nc * nc) @ dc.reshape(-1)).reshape(nc, nc)
         return self.to_native(j), self.to_native(k)
+
+    def _jk_packed(self):
+        """(npair, npair) matrices of the symmetric-density J and K (built once)."""
+        if getattr(self, "_jkpp", None) is None:
+            nc = self.eri_c.shape[0]
+            a, b = torch.tril_indices(nc, nc, device=self.eri_c.device)
+            jpp = self.eri_p.reshape(nc * nc, -1)[a * nc + b].contiguous()     # (ij|kl), i >= j, k >= l
+            t = self.eri_c.permute(0, 2, 1, 3)[a, b]                          # [(ik)][j][l] = (ij|kl)
+            t += t.transpose(1, 2).clone()                                      # + (il|kj)
+            kpp = t[:, a, b].contiguous()
+            del t
+            self._jkpp = (jpp, kpp, (a, b))
+        return self._jkpp
 
 
 class RHF:
