@@ -35,7 +35,7 @@ from .hip.grad2e import contract_int2e_ip1
 from .hip.int1e_gpu import int1e
 from .hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
 from . import mfma
-from .precision import contract, split_memo
+from .precision import contract, contract_acc, split_memo
 from .scf import DEV
 
 E = torch.einsum
@@ -61,14 +61,26 @@ def _gamma1_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
 
 
 def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
+    """2-RDM blocks (dovov, gvvvv, doooo, doovv, dovvo, None, dovvv, dooov) as
+    ccsd_gpu/gpu_ccsd_grad.py:129-246, except that the vvvv block is returned as
+    gvvvv[a,b,c,d] = sum_ij l2[i,j,a,b] tau[i,j,c,d] itself (the reference's dvvvv =
+    g[a,c,b,d] - g[a,c,d,b] / 2 is formed by the 2-RDM assembly, which reads g once).
+    l2 (t2 + t1 t1) is one product where the reference adds the t1 t1 part through
+    an o v^3 intermediate; terms are accumulated into their target by the GEMM
+    epilogue where the split-BF16 path allows it."""
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+
+    def acc(out, eq, a, b, alpha=1.0):
+        return contract_acc(out, eq, a, b, alpha=alpha, precision=precision)
+
     t1, t2, l1, l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
     pvOOv = C("ikca,jkcb->aijb", l2, t2)
     moo = E("dljd->jl", pvOOv) * 2
     mvv = E("blld->db", pvOOv) * 2
     gooov = C("kc,cija->jkia", t1, pvOOv)
     theta = t2 * 2 - t2.permute(0, 1, 3, 2)
-    pvoOV = -C("ikca,jkbc->aijb", l2, t2) + C("ikac,jkbc->aijb", l2, theta)
+    pvoOV = C("ikac,jkbc->aijb", l2, theta)
+    acc(pvoOV, "ikca,jkbc->aijb", l2, t2, -1.0)
     moo = moo + E("dljd->jl", pvoOV)
     mvv = mvv + E("blld->db", pvoOV)
     gooov = gooov - C("jc,cika->jkia", t1, pvoOV)
@@ -82,44 +94,45 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     gooov = gooov - C("ib,jkba->jkia", l1, tau) - C("jkba,ib->jkia", l2, t1)
     dooov = mfma.combine([(2, gooov.permute(0, 2, 1, 3)), (-1, gooov.permute(1, 2, 0, 3))])
 
-    goovv = E("ia,jb->ijab", mia, t1)
-    tmp = C("ijkl,klab->ijab", goooo, tau) - C("jk,ikab->ijab", mij, tau)
-    tmp = tmp - C("cb,ijac->ijab", mab, t2) - C("bd,ijad->ijab", mvv * 0.5, tau)
-    goovv = goovv + tmp + 0.5 * tau + 0.5 * l2
+    goovv = C("ijkl,klab->ijab", goooo, tau)
+    acc(goovv, "jk,ikab->ijab", mij, tau, -1.0)
+    acc(goovv, "cb,ijac->ijab", mab, t2, -1.0)
+    acc(goovv, "bd,ijad->ijab", mvv * 0.5, tau, -1.0)
+    goovv.addcmul_(mia[:, None, :, None], t1[None, :, None, :])
+    mfma.combine([(0.5, tau), (0.5, l2)], out=goovv, beta=1)
     t1t1 = E("jc,kb->jckb", t1, t1)
     gOvvO = C("kiac,jckb->iabj", l2, t1t1) + pvOOv.permute(1, 0, 3, 2)
     govVO = E("ia,jb->iabj", l1, t1) - C("ikac,jckb->iabj", l2, t1t1) + pvoOV.permute(1, 0, 3, 2)
     dovvo = 2 * govVO + gOvvO
     doovv = mfma.combine([(-2, gOvvO.permute(3, 0, 1, 2)), (-1, govVO.permute(3, 0, 1, 2))])
     tau2 = tau - t2 * 0.5
-    goovv = goovv + C("dlib,jlda->ijab", pvOOv, tau2) - C("dlia,jldb->ijab", pvoOV, tau2)
-    goovv = goovv + C("dlia,jlbd->ijab", pvoOV + pvOOv * 0.5, t2)
+    acc(goovv, "dlib,jlda->ijab", pvOOv, tau2)
+    acc(goovv, "dlia,jldb->ijab", pvoOV, tau2, -1.0)
+    acc(goovv, "dlia,jlbd->ijab", pvoOV + pvOOv * 0.5, t2)
     dovov = mfma.combine([(2, goovv.permute(0, 2, 1, 3)), (-1, goovv.permute(1, 2, 0, 3))])
     del goovv
 
-    gvvvv = C("ijab,ijcd->abcd", l2, t2)
-    jabc = C("ijab,ic->jabc", l2, t1)
-    gvvvv = gvvvv + C("jabc,jd->abcd", jabc, t1)
-    vvv = gvvvv.permute(0, 2, 1, 3)
-    dvvvv = mfma.combine([(1, vvv), (-0.5, vvv.permute(0, 3, 2, 1))])
+    gvvvv = C("ijab,ijcd->abcd", l2, tau)
     gvovv = C("adbc,id->aibc", gvvvv, -t1)
-    del gvvvv, vvv
-    gvovv = gvovv + C("akic,kb->aibc", pvoOV, t1) - C("akib,kc->aibc", pvOOv, t1)
-    gvovv = gvovv + C("ja,jibc->aibc", l1, t2) + C("ja,jbic->aibc", l1, E("jb,ic->jbic", t1, t1))
-    gvovv = gvovv + C("ba,ic->aibc", mvv * 0.5, t1) + C("ja,jibc->aibc", t1, l2)
+    acc(gvovv, "akic,kb->aibc", pvoOV, t1)
+    acc(gvovv, "akib,kc->aibc", pvOOv, t1, -1.0)
+    acc(gvovv, "ja,jibc->aibc", l1, t2)
+    acc(gvovv, "ja,jbic->aibc", l1, E("jb,ic->jbic", t1, t1))
+    acc(gvovv, "ba,ic->aibc", mvv * 0.5, t1)
+    acc(gvovv, "ja,jibc->aibc", t1, l2)
     dovvv = mfma.combine([(2, gvovv.permute(1, 3, 0, 2)), (-1, gvovv.permute(1, 2, 0, 3))])
-    return (dovov, dvvvv, doooo, doovv, dovvo, None, dovvv, dooov)
+    return (dovov, gvvvv, doooo, doovv, dovvo, None, dovvv, dooov)
 
 
 # ---------------------------------------------------------------------------
 # Phase 2: full 2-RDM and MO -> AO
 # ---------------------------------------------------------------------------
 def _assemble_dm2_mo(d2):
-    """Full MO 2-RDM (chemist's order) from the eight density blocks, as
+    """Full MO 2-RDM (chemist's order) from the density blocks (vvvv as gvvvv), as
     ccsd_gpu/gpu_ccsd_grad.py:272-300; each of the 16 occupied/virtual blocks is
     written once by the combine kernel (no zero fill, transposed blocks through
     LDS tiles)."""
-    dovov, dvvvv, doooo, doovv, dovvo, _, dovvv, dooov = d2
+    dovov, gvvvv, doooo, doovv, dovvo, _, dovvv, dooov = d2
     nocc, nvir = dovov.shape[:2]
     nmo = nocc + nvir
     o, v = slice(0, nocc), slice(nocc, None)
@@ -131,7 +144,12 @@ def _assemble_dm2_mo(d2):
     cb([(1, dm2[o, o, v, v].permute(2, 3, 0, 1))], out=dm2[v, v, o, o])
     cb([(1, dovvo), (1, dovvo.permute(3, 2, 1, 0))], out=dm2[o, v, v, o])
     cb([(1, dm2[o, v, v, o].permute(1, 0, 3, 2))], out=dm2[v, o, o, v])
-    cb([(2, dvvvv), (2, dvvvv.permute(1, 0, 3, 2))], out=dm2[v, v, v, v])
+    # 2 dvvvv + 2 dvvvv^(ba dc) with dvvvv[a,b,c,d] = g[a,c,b,d] - g[a,c,d,b] / 2, read from g
+    # through its (ab)(cd) symmetry g[a,b,c,d] = g[b,a,d,c] so that every view is contiguous
+    # along d or along c (one LDS-tiled dimension)
+    g = gvvvv
+    cb([(2, g.permute(0, 2, 1, 3)), (-1, g.permute(1, 2, 0, 3)), (2, g.permute(2, 0, 3, 1)),
+        (-1, g.permute(2, 1, 3, 0))], out=dm2[v, v, v, v])
     cb([(2, doooo), (2, doooo.permute(1, 0, 3, 2))], out=dm2[o, o, o, o])
     cb([(1, dovvv)], out=dm2[o, v, v, v])
     cb([(1, dovvv.permute(2, 3, 0, 1))], out=dm2[v, v, o, v])
