@@ -247,6 +247,15 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     return l1new, l2new
 
 
+# Largest doubles amplitude (o^2 v^2 elements) for which the mixed solve captures its update
+# as a HIP graph by default. Measured on MI355X (one capture, ~15 replays per solve): the
+# capture costs about one update more than the replays save from (H2O)4 (o^2 v^2 = 2.3e6)
+# up, where the eager update is kernel-bound (0.073 vs 0.078 s per gradient; (H2O)6 0.300
+# vs 0.315; (H2O)8 0.695 vs 0.721); smaller updates are launch-bound ((H2O)2, 1.4e5:
+# 0.056 s with the graph, 0.061 s eager).
+GRAPH_MAX_DOUBLES = 1e6
+
+
 class GraphedUpdate:
     """l -> F(l) (or A l when homogeneous) captured once as a HIP graph and replayed.
 
@@ -313,9 +322,12 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     tick("intermediates")
     diis_space = getattr(mycc, "diis_space", 6)
     use_diis = bool(getattr(mycc, "diis", True))
-    # the mixed-precision sweeps replay a captured HIP graph of the update
+    # the mixed-precision sweeps replay a captured HIP graph of the update when its
+    # launches, not its kernels, would set the pace (mycc.lambda_graph: None = automatic)
     if use_graph is None:
-        use_graph = bool(getattr(mycc, "lambda_graph", True)) and precision != "fp64"
+        forced = getattr(mycc, "lambda_graph", None)
+        use_graph = precision != "fp64" and (nocc * nocc * nvir * nvir <= GRAPH_MAX_DOUBLES if forced is None
+                                             else bool(forced))
     graphs = {}
     inh_t = torch.ones((), dtype=torch.float64, device=t1.device)     # 1: F(l), 0: A l (graph input)
     history = []

@@ -178,6 +178,22 @@ def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("graph", [True, False])
+def test_lambda_mixed_solve_with_and_without_graph(gpu, h2o_pair, graph):
+    """The mixed solve reaches the FP64 lambda with the update graph-captured or eager."""
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    mycc.conv_tol_normt = 1e-10
+    mycc.lambda_graph = graph
+    g1, g2 = solve_lambda_gpu(mycc, precision="bf16x3")
+    assert mycc.converged_lambda
+    assert (mycc._lambda_graph_replays > 0) == graph
+    assert float(np.abs(g1.cpu().numpy() - l1).max()) < 1e-8
+    assert float(np.abs(g2.cpu().numpy() - l2).max()) < 1e-8
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("fused", [True, False])
 def test_gradient_gpu_matches_oracle(gpu, h2o_pair, fused):
     from ccsd_gpu_amd.gpu_ccsd_grad import grad_elec_gpu
