@@ -195,7 +195,7 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     l1new = l1new + fov * inh + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
     l1new = l1_terms(l1new.contiguous(), l2, imds.w3.T, c1=2.0, c2=-1.0)
-    l2new.add_(torch.einsum("ia,jb->ijab", l1, imds.w4))
+    l2new.addcmul_(l1[:, None, :, None], imds.w4[None, :, None, :])       # + l1[i,a] w4[j,b]
     acc(l2new, "jibc,ca->jiba", l2, w1)
     acc(l2new, "jk,kiba->jiba", w2, l2, -1.0)
 
