@@ -42,14 +42,28 @@ def static_operands(*tensors):
         _STATIC = prev
 
 
+def _static_key(X):
+    """Cache key of a registered tensor, or of a view of one (made anew by each caller,
+    e.g. a permuted integral block): the base tensor plus the view's geometry."""
+    ids = _STATIC["ids"]
+    if id(X) in ids:
+        return (id(X),)
+    base = getattr(X, "_base", None)
+    if base is not None and id(base) in ids:
+        return (id(base), X.storage_offset(), tuple(X.shape), tuple(X.stride()))
+    return None
+
+
 def _cached(X, key, make):
-    if _STATIC is not None and id(X) in _STATIC["ids"]:
-        k = (id(X),) + key
-        v = _STATIC["cache"].get(k)
-        if v is None:
-            v = make()
-            _STATIC["cache"][k] = v
-        return v
+    if _STATIC is not None:
+        sk = _static_key(X)
+        if sk is not None:
+            k = sk + key
+            v = _STATIC["cache"].get(k)
+            if v is None:
+                v = make()
+                _STATIC["cache"][k] = v
+            return v
     return make()
 
 
