@@ -244,21 +244,59 @@ def _gemm_contract(eq, A, B, precision):
     return C.permute([order.index(c) for c in out])
 
 
+def _plain_layout(X, s, g1, g2, g3):
+    """True when _view gives X as a matrix batch without a copy."""
+    return X.is_contiguous() and s in ("".join(g1 + g2 + g3), "".join(g1 + g3 + g2))
+
+
+def _acc_fp64_direct(out, oidx, A, B, a, b, size, batch, left, contr, right, alpha):
+    """out += alpha A B as one FP64 GEMM accumulating into a matrix view of `out` (no
+    product tensor, no separate add). The free indices are taken in the planner's order
+    or in the order they lie in `out`; an operand then needing a permuted copy must be
+    static (copied once per solve) or small. Returns False when no view fits."""
+    nb, m, n = (prod(size[c] for c in g) for g in (batch, left, right))
+    if m <= 1 or n <= 1 or not contr:
+        return False
+    by_out = lambda g: sorted(g, key=lambda c: -out.stride(oidx.index(c)))  # noqa: E731
+    cheap = lambda X, s, g1, g2, g3: (_plain_layout(X, s, g1, g2, g3) or X.numel() <= 1 << 20  # noqa: E731
+                                      or (_STATIC is not None and _static_key(X) is not None))
+    for lft, rgt in ((left, right), (by_out(left), by_out(right))):
+        try:
+            outv = out.permute([oidx.index(c) for c in batch + lft + rgt]).view(nb, m, n)
+        except RuntimeError:
+            continue
+        if not (cheap(A, a, batch, lft, contr) and cheap(B, b, batch, contr, rgt)):
+            continue
+        Av = _cached(A, (a, "mk", tuple(batch), tuple(lft), tuple(contr)),
+                     lambda: _view(A, a, batch, lft, contr, size)[0])
+        Bv = _cached(B, (b, "kn", tuple(batch), tuple(contr), tuple(rgt)),
+                     lambda: _view(B, b, batch, contr, rgt, size)[0])
+        if nb == 1:
+            outv[0].addmm_(Av[0], Bv[0], alpha=alpha)
+        else:
+            outv.baddbmm_(Av, Bv, alpha=alpha)
+        return True
+    return False
+
+
 def contract_acc(out, eq, A, B, alpha=1.0, precision="fp64"):
     """out += alpha * einsum(eq, A, B), in place; eq's output labels out's indices.
 
     On the split-BF16 path the GEMM epilogue adds the term straight into `out`
-    through offset tables (no product tensor, no permuted copy); otherwise the
-    product is formed and added."""
+    through offset tables (no product tensor, no permuted copy); FP64 products
+    accumulate into a matrix view of `out` where one exists; otherwise the product
+    is formed and added."""
     lhs, oidx = eq.replace(" ", "").split("->")
-    if precision == "bf16x3" and out.is_cuda:
-        A2, B2, a, b, o, size, batch, left, contr, right, flops = _prepare(f"{lhs}->{oidx}", A, B)
-        if flops >= MIXED_MIN_FLOPS:
-            As = _split_operand(A2, a, batch, left, contr, size, "nt-a")
-            Bs = _split_operand(B2, b, batch, right, contr, size, "nt-b")
-            mfma.gemm_nt_scatter(As, Bs, out, _offsets(out, oidx, batch, size), _offsets(out, oidx, left, size),
-                                 _offsets(out, oidx, right, size), alpha=alpha, beta=1)
-            return out
+    A2, B2, a, b, o, size, batch, left, contr, right, flops = _prepare(f"{lhs}->{oidx}", A, B)
+    if precision == "bf16x3" and out.is_cuda and flops >= MIXED_MIN_FLOPS:
+        As = _split_operand(A2, a, batch, left, contr, size, "nt-a")
+        Bs = _split_operand(B2, b, batch, right, contr, size, "nt-b")
+        mfma.gemm_nt_scatter(As, Bs, out, _offsets(out, oidx, batch, size), _offsets(out, oidx, left, size),
+                             _offsets(out, oidx, right, size), alpha=alpha, beta=1)
+        return out
+    if (precision == "fp64" or flops < MIXED_MIN_FLOPS) and out.dtype == torch.float64 and \
+            _acc_fp64_direct(out, oidx, A2, B2, a, b, size, batch, left, contr, right, alpha):
+        return out
     return out.add_(contract(eq, A, B, precision=precision), alpha=alpha)
 
 

@@ -6,7 +6,7 @@ import re
 import pytest
 import torch
 
-from ccsd_gpu_amd.precision import contract, static_operands
+from ccsd_gpu_amd.precision import contract, contract_acc, static_operands
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FILES = ["ccsd_gpu_amd/gpu_ccsd_lambda.py", "ccsd_gpu_amd/gpu_ccsd_grad.py", "ccsd_gpu_amd/cc.py"]
@@ -35,6 +35,30 @@ def test_planner_matches_einsum(eq):
     with static_operands(*ops):
         for _ in range(2):
             assert torch.allclose(contract(eq, *ops), ref, rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("eq", [e for e in _equations() if e.count(",") == 1])
+def test_contract_acc_matches_einsum(eq):
+    """out += alpha einsum into contiguous and permuted destinations (the FP64 path
+    accumulates into a matrix view of out where one fits), plain and static operands."""
+    lhs, out_idx = eq.split("->")
+    if not out_idx:
+        pytest.skip("scalar output")
+    g = torch.Generator().manual_seed(len(eq) + 7)
+    ops = [torch.randn([SIZES[c] for c in t], dtype=torch.float64, generator=g) for t in lhs.split(",")]
+    ref = torch.einsum(eq, *ops)
+    for perm in (list(range(len(out_idx))), list(reversed(range(len(out_idx))))):
+        base = torch.randn([ref.shape[p] for p in perm], dtype=torch.float64, generator=g)
+        inv = [perm.index(i) for i in range(len(perm))]
+        for static in (False, True):
+            out = base.clone().permute(inv)          # a view labelled by out_idx
+            want = out + (-0.5) * ref
+            if static:
+                with static_operands(*ops):
+                    contract_acc(out, eq, ops[0], ops[1], alpha=-0.5)
+            else:
+                contract_acc(out, eq, ops[0], ops[1], alpha=-0.5)
+            assert torch.allclose(out, want, rtol=1e-12, atol=1e-12), (eq, perm, static)
 
 
 def test_planner_handles_transposed_and_permuted_views():
