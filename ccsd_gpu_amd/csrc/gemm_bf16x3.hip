@@ -153,22 +153,36 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     // lane holds C[8*(e/4) + 4*half + e%4][l32] of each 32x32 tile; the element (row, col)
     // of batch b goes to C[b*sC + row*ldc + col], or to C[off_b[b] + off_r[row] + off_c[col]]
     // when offset tables are given (a permuted destination, e.g. a term of a 4-index tensor)
+    // The destination offsets of the lane's 32 rows and 2 columns are loaded first, and
+    // a 32 x 32 tile's 16 previous values before any of its stores, so that the
+    // epilogue's memory round trips overlap instead of running one element at a time
+    // (all four tiles at once would need more than 256 registers).
     double* Cb = off_r ? C + off_b[bz] : C + bz * sC;
+    long long orow[2][16];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+            orow[i][e] = gr < M ? (off_r ? off_r[gr] : (long long)gr * ldc) : -1;
+        }
+    long long ocol[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int gc = n0 + wn * 64 + j * 32 + l32;
+        ocol[j] = gc < N ? (off_r ? off_c[gc] : (long long)gc) : -1;
+    }
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
-            const int gc = n0 + wn * 64 + j * 32 + l32;
-            const long long oc = (off_r && gc < N) ? off_c[gc] : (long long)gc;
+            if (ocol[j] < 0) continue;
+            double prev[16];
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
-                if (gr < M && gc < N) {
-                    double* p = Cb + (off_r ? off_r[gr] : (long long)gr * ldc) + oc;
-                    const double v = alpha * (double)acc[i][j][e];
-                    *p = beta ? *p + v : v;
-                }
-            }
+            for (int e = 0; e < 16; e++) prev[e] = (beta && orow[i][e] >= 0) ? Cb[orow[i][e] + ocol[j]] : 0.0;
+#pragma unroll
+            for (int e = 0; e < 16; e++)
+                if (orow[i][e] >= 0) Cb[orow[i][e] + ocol[j]] = prev[e] + alpha * (double)acc[i][j][e];
         }
 }
 
