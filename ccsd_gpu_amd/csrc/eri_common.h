@@ -85,20 +85,57 @@ struct EriClassArgs {
 };
 
 #ifdef __HIPCC__
+// Where the two-particle density of one molecule lives: resolved once per quartet, so
+// that the element reads below carry no dependent metadata loads of their own and can
+// all be in flight together.
+struct GammaSrc {
+    const double* gamma;
+    long long nao, npair;
+    int mode;
+    int o, n;            // molecule's first Cartesian AO and AO count (separable part)
+    const double* D;
+    const double* P;
+};
+
+__device__ __forceinline__ GammaSrc gamma_src(const EriClassArgs& a, int mol)
+{
+    GammaSrc s;
+    s.gamma = a.gamma;
+    s.nao = a.nao;
+    s.npair = a.npair;
+    s.mode = a.gamma_mode;
+    s.o = 0;
+    s.n = 0;
+    s.D = nullptr;
+    s.P = nullptr;
+    if (s.mode != 0) {
+        s.o = a.mol_ao0[mol];
+        s.n = a.mol_nao[mol];
+        s.D = a.dm + a.mol_dm_off[mol];
+        s.P = a.pm + a.mol_dm_off[mol];
+    }
+    return s;
+}
+
 // Two-particle density element for Cartesian AOs (global indices) i j k l, k >= l,
 // of the packed (diagonal-halved) form, in either representation.
-__device__ __forceinline__ double gamma_packed(const EriClassArgs& a, int mol, int i, int j, int k, int l)
+__device__ __forceinline__ double gamma_at(const GammaSrc& s, int i, int j, int k, int l)
 {
-    const double ge = a.gamma_mode != 1 ? a.gamma[((long long)i * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l]
-                                        : 0.0;
-    if (a.gamma_mode == 0) return ge;
-    const int o = a.mol_ao0[mol], n = a.mol_nao[mol];
-    const double* D = a.dm + a.mol_dm_off[mol];
-    const double* P = a.pm + a.mol_dm_off[mol];
-    i -= o; j -= o; k -= o; l -= o;
+    const double ge = s.mode != 1 ? s.gamma[((long long)i * s.nao + j) * s.npair + (long long)k * (k + 1) / 2 + l]
+                                  : 0.0;
+    if (s.mode == 0) return ge;
+    const int n = s.n;
+    const double* D = s.D;
+    const double* P = s.P;
+    i -= s.o; j -= s.o; k -= s.o; l -= s.o;
     const double g = D[i * n + j] * P[k * n + l] + P[i * n + j] * D[k * n + l] -
                      0.25 * (D[i * n + l] * P[k * n + j] + D[i * n + k] * P[l * n + j] + P[i * n + l] * D[j * n + k] +
                              P[i * n + k] * D[j * n + l]);
     return ge + (k == l ? 0.5 * g : g);
+}
+
+__device__ __forceinline__ double gamma_packed(const EriClassArgs& a, int mol, int i, int j, int k, int l)
+{
+    return gamma_at(gamma_src(a, mol), i, j, k, l);
 }
 #endif

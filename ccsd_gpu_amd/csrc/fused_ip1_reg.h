@@ -264,6 +264,7 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
     const bool same_kl = kp.x == kp.y;
 
     // Gamma of this part for every contraction-column combination, loaded once
+    const GammaSrc gs = gamma_src(a, si.mol);
     double gam[MIJ][MKL][NFP];
 #pragma unroll
     for (int cij = 0; cij < MIJ; cij++)
@@ -283,7 +284,7 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
                     const int j = sj.ao_cart + c_j * NFJ + fj;
                     const int k = sk.ao_cart + c_k * NFK + fk;
                     const int l = sl.ao_cart + c_l * NFL + fl;
-                    if (!(same_kl && l > k)) g = gamma_packed(a, si.mol, i, j, k, l);
+                    if (!(same_kl && l > k)) g = gamma_at(gs, i, j, k, l);
                 }
                 gam[cij][ckl][t] = g;
             }
