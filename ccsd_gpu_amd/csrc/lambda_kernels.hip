@@ -74,13 +74,21 @@ __global__ void __launch_bounds__(256) l1_block_kernel(const double* __restrict_
             yr_q[u] = r < v ? yq[r] : 0.0;
             yr_p[u] = r < v ? yp[r] : 0.0;
         }
+        // clamped (always in bounds) reads, so that all 4 NC of them are in flight at once:
+        // a clamped column meets a zero weight (yqc, ypc) and its column sums are never
+        // stored, a clamped row a zero weight (yr_q, yr_p) and its row sums are never stored
+        double bv[NC][4];
 #pragma unroll
         for (int ch = 0; ch < NC; ch++) {
-            const int c = ch * 64 + lane;
+            const int c = min(ch * 64 + lane, v - 1);
+#pragma unroll
+            for (int u = 0; u < 4; u++) bv[ch][u] = B[(long long)min(r0 + 4 * u, v - 1) * v + c];
+        }
+#pragma unroll
+        for (int ch = 0; ch < NC; ch++) {
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const int r = r0 + 4 * u;
-                const double b = (c < v && r < v) ? B[(long long)r * v + c] : 0.0;
+                const double b = bv[ch][u];
                 pq[u] += b * yqc[ch];
                 pp[u] += b * ypc[ch];
                 colq[ch] += b * yr_q[u];
@@ -162,30 +170,43 @@ __global__ void __launch_bounds__(256) pair_combine_kernel(const double* __restr
     const long long vv = (long long)v * v;
     const double* Xij = X + (long long)i * x0 + (long long)j * x1;
     const double* Xji = X + (long long)j * x0 + (long long)i * x1;
+    // every read below uses indices clamped into range (the results of clamped lanes are
+    // never stored), so that a thread's reads carry no branches and are all in flight at once
+    const int vm = v - 1;
     if (c2 != 0.0 || c4 != 0.0) {
+        double t1v[PT / 8], t2v[PT / 8];
 #pragma unroll
         for (int k = 0; k < PT; k += 8) {
-            const int b = b0 + ty + k, a = a0 + tx;
-            const bool ok = (b < v) && (a < v);
-            const long long e = (long long)b * x2 + (long long)a * x3;
-            tij[ty + k][tx] = (ok && c2 != 0.0) ? Xij[e] : 0.0;
-            tji[ty + k][tx] = (ok && c4 != 0.0) ? Xji[e] : 0.0;
+            const long long e = (long long)min(b0 + ty + k, vm) * x2 + (long long)min(a0 + tx, vm) * x3;
+            t1v[k / 8] = c2 != 0.0 ? Xij[e] : 0.0;
+            t2v[k / 8] = c4 != 0.0 ? Xji[e] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k += 8) {
+            tij[ty + k][tx] = t1v[k / 8];
+            tji[ty + k][tx] = t2v[k / 8];
         }
     }
     __syncthreads();
+    double x1v[PT / 8], x3v[PT / 8], dv[PT / 8], ov[PT / 8];
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = min(a0 + ty + k, vm), b = min(b0 + tx, vm);
+        const long long e = (long long)a * x2 + (long long)b * x3;
+        x1v[k / 8] = c1 != 0.0 ? Xij[e] : 0.0;
+        x3v[k / 8] = c3 != 0.0 ? Xji[e] : 0.0;
+        dv[k / 8] = D ? D[((long long)i * o + j) * vv + (long long)a * v + b] : 1.0;
+        ov[k / 8] = beta ? out[(long long)i * s0 + (long long)j * s1 + (long long)a * s2 + (long long)b * s3] : 0.0;
+    }
 #pragma unroll
     for (int k = 0; k < PT; k += 8) {
         const int a = a0 + ty + k, b = b0 + tx;
         if (a < v && b < v) {
-            const long long e = (long long)a * x2 + (long long)b * x3;
-            double val = 0.0;
-            if (c1 != 0.0) val += c1 * Xij[e];
-            if (c3 != 0.0) val += c3 * Xji[e];
+            double val = c1 * x1v[k / 8] + c3 * x3v[k / 8];
             if (c2 != 0.0) val += c2 * tij[tx][ty + k];
             if (c4 != 0.0) val += c4 * tji[tx][ty + k];
-            if (D) val *= D[((long long)i * o + j) * vv + (long long)a * v + b];
-            double* dst = out + (long long)i * s0 + (long long)j * s1 + (long long)a * s2 + (long long)b * s3;
-            *dst = beta ? *dst + val : val;
+            out[(long long)i * s0 + (long long)j * s1 + (long long)a * s2 + (long long)b * s3] =
+                ov[k / 8] + val * dv[k / 8];
         }
     }
 }
@@ -210,11 +231,19 @@ __global__ void __launch_bounds__(256) unpack_pairs_kernel(const double* __restr
     const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;
     const long long vv = (long long)v * v;
     const double* Xp = X + p * vv;
+    // clamped reads (entries outside the v x v block are never written out), all in flight at once
+    const int vm = v - 1;
+    double r1[PT / 8], r2[PT / 8];
 #pragma unroll
     for (int k = 0; k < PT; k += 8) {
         const int r = ty + k;
-        t1[r][tx] = (a0 + r < v && b0 + tx < v) ? Xp[(long long)(a0 + r) * v + b0 + tx] : 0.0;
-        if (i == j) t2[r][tx] = (b0 + r < v && a0 + tx < v) ? Xp[(long long)(b0 + r) * v + a0 + tx] : 0.0;
+        r1[k / 8] = Xp[(long long)min(a0 + r, vm) * v + min(b0 + tx, vm)];
+        r2[k / 8] = i == j ? Xp[(long long)min(b0 + r, vm) * v + min(a0 + tx, vm)] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        t1[ty + k][tx] = r1[k / 8];
+        t2[ty + k][tx] = r2[k / 8];
     }
     __syncthreads();
     double* Oij = out + ((long long)i * o + j) * vv;
