@@ -23,7 +23,9 @@ gradient against the FP64 lambda; value = split-BF16 lambda wall-s.
 --config batch: 64 seeded C/N/O/H molecules (3-12 heavy atoms, cc-pVDZ) in one
 plan; one step = one Schwarz-screened (1e-12) fused int2e_ip1 x Gamma pass over
 the whole batch with a separable density Gamma(D) per molecule; value = surviving
-quartets/s.
+quartets/s. config.persistent_single_launch compares, on the first
+--persistent-molecules molecules, the pass as one persistent launch over every
+class with the per-class launches.
 
 --config gemm: the split-BF16 NT GEMM of the lambda / RDM contractions on the
 (H2O)8 sweep shapes (random operands); value = achieved bf16 MFMA TFLOP/s of the
@@ -115,6 +117,16 @@ def bench_eri(args):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     kern_ms = timed_fused_pass(mol, gp, -8.0)
     roof, flops, quartets, refq = roofline(mol, kern_ms)
+    # the same pass as one persistent launch of the generic kernel over every class
+    from ccsd_gpu_amd.hip.grad2e import set_persistent
+    de_c = contract_int2e_ip1(mol, gp, scale=-8.0).clone()
+    set_persistent(mol, True)
+    try:
+        pers_ms = timed_fused_pass(mol, gp, -8.0, reps=2)
+        pers_diff = float((contract_int2e_ip1(mol, gp, scale=-8.0) - de_c).abs().max())
+    finally:
+        set_persistent(mol, False)
+    log(f"one persistent launch {pers_ms:.1f} ms vs per-class launches {kern_ms:.1f} ms (max |diff| {pers_diff:.1e})")
     if args.copy_calibration:
         # a plain 256 MiB device copy (256 MiB read + 256 MiB written) in the same run: the
         # reference that scales the FETCH_SIZE / WRITE_SIZE counters of the fused pass
@@ -138,7 +150,9 @@ def bench_eri(args):
         "config": {"workload": f"fused int2e_ip1 x G backward, (H2O){args.waters or 4}/cc-pVDZ, "
                                "seeded 8-fold-symmetric N(0,1) G",
                    "quartets_evaluated": quartets, "quartets_reference_equivalent": refq,
-                   "screening": "none"},
+                   "screening": "none",
+                   "persistent_single_launch": {"persistent_ms": pers_ms, "per_class_ms": kern_ms,
+                                                "max_abs_diff": pers_diff}},
         "roofline": roof, "cpu_baseline": cpu}
 
 
@@ -162,6 +176,28 @@ def timed_batch_pass(b, dms, reps):
         ev.append((a, e))
     torch.cuda.synchronize()
     return float(np.mean([a.elapsed_time(e) for a, e in ev])), de
+
+
+def persistent_vs_per_class(mols, dms, thr):
+    """Screened separable fused pass over `mols` as per-class launches (register kernels,
+    four streams) and as one persistent launch; times (HIP events) and max |difference|."""
+    import torch
+    from ccsd_gpu_amd.batch import MoleculeBatch
+    b = MoleculeBatch(mols)
+    b.screen(thr)
+    b.grad2e_separable(dms)
+    cls_ms, de_c = timed_batch_pass(b, dms, 2)
+    de_c = torch.cat(de_c).clone()
+    b.set_persistent(True)
+    b.grad2e_separable(dms)       # first call: generic-kernel choices, work items
+    pers_ms, de_p = timed_batch_pass(b, dms, 2)
+    q = b.last_quartets()
+    diff = float((torch.cat(de_p) - de_c).abs().max())
+    b.close()
+    log(f"{len(mols)} molecules: one persistent launch {pers_ms:.1f} ms vs per-class launches {cls_ms:.1f} ms "
+        f"({q} quartets, max |diff| {diff:.1e})")
+    return {"molecules": len(mols), "quartets": q, "persistent_ms": pers_ms, "per_class_ms": cls_ms,
+            "persistent_quartets_per_s": q / (pers_ms * 1e-3), "max_abs_diff": diff}
 
 
 def bench_batch(args):
@@ -199,6 +235,10 @@ def bench_batch(args):
     achieved = flops / (kern_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
+    # the screened pass as ONE persistent launch over every class of every molecule (generic
+    # wave-per-quartet kernel, chunk length per class autotuned) against the per-class
+    # launches, on the first few molecules (the generic kernel would take minutes on all)
+    pers = persistent_vs_per_class(mols[:args.persistent_molecules], dms[:args.persistent_molecules], args.thr)
     # the same pass unscreened, for the error and gain of screening
     b.screen(0.0)
     b.grad2e_separable(dms)
@@ -226,6 +266,7 @@ def bench_batch(args):
                    "schwarz_threshold": args.thr, "quartets_evaluated": surv,
                    "quartets_unscreened": b.quartets_unscreened(), "quartets_reference_equivalent": refq,
                    "unscreened_pass_ms": full_ms, "screened_pass_ms": kern_ms,
+                   "persistent_single_launch": pers,
                    "max_abs_diff_screened_vs_unscreened": err, "max_net_force_unscreened": net},
         "roofline": roof, "cpu_baseline": cpu}
 
@@ -439,6 +480,8 @@ def main():
     ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision", "gemm"])
     ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
+    ap.add_argument("--persistent-molecules", type=int, default=4,
+                    help="batch config: molecules of the persistent-launch comparison")
     ap.add_argument("--waters", type=int, default=0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--copy-calibration", action="store_true",

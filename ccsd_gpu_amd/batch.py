@@ -86,6 +86,12 @@ class MoleculeBatch:
     def set_option(self, option, value):
         _lib.check(self._lib.ccsd_plan_set_option(self._plan, int(option), int(value)), "ccsd_plan_set_option")
 
+    def set_persistent(self, on=True):
+        """Run each fused pass as one persistent launch over every class of every
+        molecule (waves draw cost-ordered work items from a device counter), instead
+        of one launch per class on four streams (the default)."""
+        self.set_option(5, 1 if on else 0)
+
     def screen(self, thr):
         _lib.check(self._lib.ccsd_plan_screen(self._plan, float(thr), _lib.current_stream_ptr()), "ccsd_plan_screen")
 

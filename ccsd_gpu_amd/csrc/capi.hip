@@ -18,6 +18,11 @@
 
 size_t ccsd_eri_lds_bytes(const EriClassArgs& a);
 hipError_t ccsd_launch_eri_class(const EriClassArgs& a, hipStream_t stream);
+int ccsd_persistent_blocks_per_cu(size_t lds);
+hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
+                                      unsigned int* counter, const double* gamma, const double* dm,
+                                      const double* pm, double* de, double scale, int gamma_mode, int nblocks,
+                                      size_t lds, hipStream_t stream);
 hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const void* tall, int nfi_s, int nao_s,
                            long long npair_s, int nao_c, long long npair_c, int nfi_c, int ncomp, hipStream_t stream);
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
@@ -123,6 +128,7 @@ struct ChunkList {
     int n = 0;
     long long quartets = 0;
     std::vector<double> cumq;      // quartet equivalents in the first c entries (autotuner samples)
+    std::vector<float> cost;       // estimated cost of each entry (primitive products x quartets)
 };
 
 // Kernel choice of one class in the fused gradient.
@@ -132,9 +138,23 @@ struct Choice {
     int chunk = 64;  // ket pairs per chunk; bits 16+: primitive split of the register kernel (see get_chunks)
 };
 
+// The whole fused pass of a bra set as one persistent launch of the generic kernel:
+// the launch description of every class (work lists included) and the work items
+// (class, chunk range) in decreasing order of estimated cost.
+struct PersistPlan {
+    EriClassArgs* d_cls = nullptr;
+    int4* d_items = nullptr;
+    unsigned int* d_counter = nullptr;   // 8 bytes, cleared by a fill kernel before each launch
+    int nitems = 0, nblocks = 0;
+    size_t lds = 0;
+    long long quartets = 0;
+    std::vector<std::array<long long, 10>> launches;
+};
+
 struct BraSet {
     std::vector<PairClass> cls;
     std::map<std::tuple<int, int, int, int>, ChunkList> chunks;  // (bc, kc, chunk, screened)
+    std::map<int, PersistPlan> persist;                          // by screened
 };
 
 int ncart(int l) { return (l + 1) * (l + 2) / 2; }
@@ -166,6 +186,9 @@ struct ccsd_plan {
     // fused-gradient kernel choice per (ish0, ish1, bra class, ket class); class
     // numbering does not depend on the geometry or the screening threshold
     std::map<std::tuple<int, int, int, int>, Choice> choice;
+    std::map<std::tuple<int, int, int, int>, Choice> choice_lds;   // generic kernel only (persistent pass)
+    bool persistent = false;                         // fused pass as one persistent launch
+    int n_cu = 0;                                    // compute units of the plan's device
     bool has_c2s = false;
     bool fused_reg = true;                           // register kernels for the fused gradient
     bool autotune = true;                            // time the kernel choices of each class once
@@ -361,6 +384,12 @@ void free_braset(BraSet& b)
     for (auto& kv : b.chunks)
         if (kv.second.d) (void)hipFree(kv.second.d);
     b.chunks.clear();
+    for (auto& kv : b.persist) {
+        if (kv.second.d_cls) (void)hipFree(kv.second.d_cls);
+        if (kv.second.d_items) (void)hipFree(kv.second.d_items);
+        if (kv.second.d_counter) (void)hipFree(kv.second.d_counter);
+    }
+    b.persist.clear();
 }
 
 void clear_bra_cache(ccsd_plan* p)
@@ -490,6 +519,8 @@ int get_chunks(ccsd_plan* p, BraSet& bs, int bc, int kc, int code, bool screened
         std::vector<int4> h(work.size());
         for (size_t t = 0; t < work.size(); t++) h[t] = work[t].second;
         ChunkList cl;
+        cl.cost.resize(work.size());
+        for (size_t t = 0; t < work.size(); t++) cl.cost[t] = (float)work[t].first;
         cl.n = (int)h.size();
         cl.quartets = quartets;
         cl.cumq.resize(h.size() + 1);
@@ -592,16 +623,17 @@ int launch_choice(const EriClassArgs& a, const Choice& c, hipStream_t stream)
 // sample of the class's real work (the first chunks of its list, about 2^18
 // quartets, into a throw-away gradient) and the fastest per quartet is kept.
 int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClassArgs a, bool screened,
-           hipStream_t stream, Choice* out)
+           hipStream_t stream, Choice* out, bool lds_only = false)
 {
+    auto& cmap = lds_only ? p->choice_lds : p->choice;
     auto key = std::make_tuple(ish0, ish1, bc, kc);
-    auto it = p->choice.find(key);
-    if (it != p->choice.end()) {
+    auto it = cmap.find(key);
+    if (it != cmap.end()) {
         *out = it->second;
         return CCSD_OK;
     }
     std::vector<Choice> cand;
-    const bool reg = p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
+    const bool reg = !lds_only && p->fused_reg && ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
     if (reg) {
         // unsplit and adaptively split work lists (a fixed split replaces both when forced)
         for (int w : {4, 2, 1}) {
@@ -670,7 +702,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             }
         }
     }
-    p->choice[key] = best;
+    cmap[key] = best;
     *out = best;
     return CCSD_OK;
 }
@@ -783,9 +815,128 @@ int ensure_side_streams(ccsd_plan* p)
     return CCSD_OK;
 }
 
+// Work items of the persistent pass: consecutive chunks of one class are grouped until
+// their estimated cost reaches 1/8 of a resident wave's share of the whole pass (a
+// chunk costlier than that is an item of its own); items sorted by decreasing cost.
+int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassArgs& proto, bool screened,
+                  hipStream_t stream, PersistPlan* out)
+{
+    // kernel choices (chunk length per class) first: the autotuner drops the work lists
+    // of the candidates it does not take, so none is referenced before all are chosen
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
+            a.gamma_mode = proto.gamma_mode;
+            a.gamma = proto.gamma;
+            a.dm = proto.dm;
+            a.pm = proto.pm;
+            a.gamma_scale = proto.gamma_scale;
+            Choice c;
+            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c, true);
+            if (rc) return rc;
+        }
+    PersistPlan pp;
+    std::vector<EriClassArgs> cls;
+    std::vector<const ChunkList*> lists;
+    double total = 0.0;
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
+            Choice c;
+            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c, true);
+            if (rc) return rc;
+            ChunkList* cl = nullptr;
+            rc = get_chunks(p, bs, bc, kc, c.chunk, screened, true, &cl);
+            if (rc) return rc;
+            if (cl->n == 0) continue;
+            a.chunks = cl->d;
+            a.nchunks = cl->n;
+            const size_t lds = ccsd_eri_lds_bytes(a);
+            if (lds > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
+            pp.lds = std::max(pp.lds, lds);
+            pp.quartets += cl->quartets;
+            pp.launches.push_back({a.li, a.lj, a.lk, a.ll, (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0, 2, 1,
+                                   c.chunk, 0, cl->quartets});
+            for (float w : cl->cost) total += w;
+            cls.push_back(a);
+            lists.push_back(cl);
+        }
+    if (cls.empty()) {
+        *out = pp;
+        return CCSD_OK;
+    }
+    if (p->n_cu == 0) HIP_TRY(hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
+    const int bpc = ccsd_persistent_blocks_per_cu(pp.lds);
+    if (bpc <= 0) return fail(CCSD_ERR_UNSUPPORTED, "persistent kernel cannot be resident at this LDS size");
+    pp.nblocks = bpc * p->n_cu;
+    const double target = total / ((double)pp.nblocks * 8.0);
+    std::vector<std::pair<double, int4>> items;
+    for (size_t ci = 0; ci < cls.size(); ci++) {
+        const ChunkList* cl = lists[ci];
+        int c0 = 0;
+        double acc = 0.0;
+        for (int c = 0; c < cl->n; c++) {
+            acc += cl->cost[c];
+            if (acc >= target || c + 1 == cl->n) {
+                items.push_back(std::make_pair(acc, make_int4((int)ci, c0, c + 1, 0)));
+                c0 = c + 1;
+                acc = 0.0;
+            }
+        }
+    }
+    std::stable_sort(items.begin(), items.end(),
+                     [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) { return x.first > y.first; });
+    std::vector<int4> h(items.size());
+    for (size_t t = 0; t < items.size(); t++) h[t] = items[t].second;
+    pp.nitems = (int)h.size();
+    pp.nblocks = std::min(pp.nblocks, pp.nitems);
+    HIP_TRY(hipMalloc(&pp.d_cls, sizeof(EriClassArgs) * cls.size()));
+    HIP_TRY(hipMemcpy(pp.d_cls, cls.data(), sizeof(EriClassArgs) * cls.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&pp.d_items, sizeof(int4) * h.size()));
+    HIP_TRY(hipMemcpy(pp.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&pp.d_counter, sizeof(double)));
+    *out = pp;
+    return CCSD_OK;
+}
+
+// The fused pass as one launch of the persistent kernel (plan option 5).
+int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
+                         const double* gamma, const double* dm, const double* pm, hipStream_t stream)
+{
+    const bool screened = p->thr > 0.0;
+    auto it = bs.persist.find(screened ? 1 : 0);
+    if (it == bs.persist.end()) {
+        EriClassArgs proto;
+        memset(&proto, 0, sizeof(proto));
+        proto.gamma_mode = gamma_mode;
+        proto.gamma = gamma;
+        proto.dm = dm;
+        proto.pm = pm;
+        proto.gamma_scale = scale;
+        PersistPlan pp;
+        int rc = build_persist(p, bs, ish0, ish1, proto, screened, stream, &pp);
+        if (rc) {
+            if (pp.d_cls) (void)hipFree(pp.d_cls);
+            if (pp.d_items) (void)hipFree(pp.d_items);
+            if (pp.d_counter) (void)hipFree(pp.d_counter);
+            return rc;
+        }
+        it = bs.persist.emplace(screened ? 1 : 0, pp).first;
+    }
+    const PersistPlan& pp = it->second;
+    p->last_quartets = pp.quartets;
+    p->last_launches = pp.launches;
+    if (pp.nitems == 0) return CCSD_OK;
+    HIP_TRY(ccsd_launch_fill((double*)pp.d_counter, 1, 0.0, stream));
+    HIP_TRY(ccsd_launch_eri_persistent(pp.d_cls, pp.d_items, pp.nitems, pp.d_counter, gamma, dm, pm, de, scale,
+                                       gamma_mode, pp.nblocks, pp.lds, stream));
+    return CCSD_OK;
+}
+
 int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
               const double* gamma, const double* dm, const double* pm, hipStream_t stream)
 {
+    if (p->persistent) return run_fused_persistent(p, bs, ish0, ish1, de, scale, gamma_mode, gamma, dm, pm, stream);
     const bool screened = p->thr > 0.0;
     p->last_quartets = 0;
     p->last_launches.clear();
@@ -1431,12 +1582,14 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         p->fused_reg = value != 0;
         clear_bra_cache(p);
         p->choice.clear();
+        p->choice_lds.clear();
         return CCSD_OK;
     }
     if (option == 1) {
         p->autotune = value != 0;
         clear_bra_cache(p);
         p->choice.clear();
+        p->choice_lds.clear();
         return CCSD_OK;
     }
     if (option == 2) {
@@ -1447,6 +1600,7 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         p->ket_shell_order = value != 0;
         clear_bra_cache(p);
         p->choice.clear();
+        p->choice_lds.clear();
         return build_ket(p);
     }
     if (option == 4) {
@@ -1454,6 +1608,12 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         p->force_split = value;
         clear_bra_cache(p);
         p->choice.clear();
+        p->choice_lds.clear();
+        return CCSD_OK;
+    }
+    if (option == 5) {
+        p->persistent = value != 0;
+        clear_bra_cache(p);
         return CCSD_OK;
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");

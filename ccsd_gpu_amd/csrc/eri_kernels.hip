@@ -348,6 +348,73 @@ __global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
     }
 }
 
+// Wave reduction of a chunk's three derivative sums and one atomic add per component
+// into dE/dR of the bra pair's first atom (all lanes call it; lane 0 adds).
+__device__ __forceinline__ void add_chunk_gradient(const EriClassArgs& a, int bq, double gdx, double gdy, double gdz)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        gdx += __shfl_down(gdx, off, 64);
+        gdy += __shfl_down(gdy, off, 64);
+        gdz += __shfl_down(gdz, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        const int atom = a.shells[a.bra[bq].x].atom;
+        atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * gdx);
+        atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * gdy);
+        atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * gdz);
+    }
+}
+
+// Per-call arguments of the persistent fused pass (the class table is static).
+struct PersistDyn {
+    const double* gamma;
+    const double* dm;
+    const double* pm;
+    double* de;
+    double scale;
+    int gamma_mode;
+};
+
+// Fused int2e_ip1 x Gamma over every class of a pass in ONE launch. Each wave takes
+// work items from a global counter until the list is exhausted; item t covers
+// chunks [items[t].y, items[t].z) of class items[t].x (class table `classes`: the
+// launch description of each class, work list included). Items are ordered by
+// decreasing estimated cost, so the last waves to finish run short items. `counter`
+// must read 0 at launch; every wave leaves the loop once it draws t >= nitems.
+__global__ void __launch_bounds__(64) eri_persistent_kernel(const EriClassArgs* __restrict__ classes,
+                                                            const int4* __restrict__ items, int nitems,
+                                                            unsigned int* __restrict__ counter, const PersistDyn dyn)
+{
+    extern __shared__ double lds[];
+    for (;;) {
+        int t = 0;
+        if (threadIdx.x == 0) t = (int)atomicAdd(counter, 1u);
+        t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
+        if (t >= nitems) break;
+        const int4 it = items[t];
+        const int cls = __builtin_amdgcn_readfirstlane(it.x);
+        const int c0 = __builtin_amdgcn_readfirstlane(it.y), c1 = __builtin_amdgcn_readfirstlane(it.z);
+        EriClassArgs a = classes[cls];
+        a.gamma = dyn.gamma;
+        a.dm = dyn.dm;
+        a.pm = dyn.pm;
+        a.de = dyn.de;
+        a.gamma_scale = dyn.scale;
+        a.gamma_mode = dyn.gamma_mode;
+        const LdsLayout L = lds_layout(a);
+        for (int c = c0; c < c1; c++) {
+            const int4 ch = a.chunks[c];
+            double gdx = 0.0, gdy = 0.0, gdz = 0.0;
+            for (int kq = ch.y; kq < ch.z; kq++) {
+                __syncthreads();   // LDS of the previous quartet (or item) fully consumed
+                quartet(a, L, lds, ch.x, kq, gdx, gdy, gdz);
+            }
+            add_chunk_gradient(a, ch.x, gdx, gdy, gdz);
+        }
+        __syncthreads();
+    }
+}
+
 // Cartesian -> spherical transform of an s2kl tensor, one thread per output.
 // in[comp][i_c][j_c][pair_c], out[comp][i_s][j_s][pair_s]; per AO a list of
 // (Cartesian AO, coefficient) terms, at most 6 (real solid harmonics, l <= 4).
@@ -411,6 +478,33 @@ hipError_t ccsd_launch_eri_class(const EriClassArgs& a, hipStream_t stream)
     if (a.nchunks <= 0) return hipSuccess;
     const size_t lds = ccsd_eri_lds_bytes(a);
     hipLaunchKernelGGL(ccsd::eri_quartet_kernel, dim3((unsigned)a.nchunks), dim3(64), lds, stream, a);
+    return hipGetLastError();
+}
+
+// Resident one-wave blocks of the persistent kernel per CU at `lds` bytes of LDS.
+int ccsd_persistent_blocks_per_cu(size_t lds)
+{
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, ccsd::eri_persistent_kernel, 64, lds) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
+                                      unsigned int* counter, const double* gamma, const double* dm,
+                                      const double* pm, double* de, double scale, int gamma_mode, int nblocks,
+                                      size_t lds, hipStream_t stream)
+{
+    if (nitems <= 0 || nblocks <= 0) return hipSuccess;
+    ccsd::PersistDyn dyn;
+    dyn.gamma = gamma;
+    dyn.dm = dm;
+    dyn.pm = pm;
+    dyn.de = de;
+    dyn.scale = scale;
+    dyn.gamma_mode = gamma_mode;
+    hipLaunchKernelGGL(ccsd::eri_persistent_kernel, dim3((unsigned)nblocks), dim3(64), lds, stream, classes, items,
+                       nitems, counter, dyn);
     return hipGetLastError();
 }
 

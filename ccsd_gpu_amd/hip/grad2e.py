@@ -82,6 +82,12 @@ def set_screening(mol, thr):
                "ccsd_plan_screen")
 
 
+def set_persistent(mol, on=True):
+    """Fused passes of `mol`'s plan as one persistent launch of the generic kernel over
+    every class (True) or one launch per class on four streams (False, the default)."""
+    _lib.check(_lib.load().ccsd_plan_set_option(_lib.get_plan(mol), 5, 1 if on else 0), "ccsd_plan_set_option")
+
+
 def schwarz_factors(mol):
     """(nbas, nbas) device tensor q_ij = max_{a in i, b in j} sqrt((ab|ab))."""
     q = torch.empty((mol.nbas, mol.nbas), dtype=torch.float64, device="cuda")

@@ -77,7 +77,11 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * are cut into up to 16 pieces along the ket primitive-pair loop (each piece at
  * most ~1/2048 of the class's work; the autotuner decides per class), 1 turns
  * the split off, S = 2..254 splits every chunk S ways (with option 1 = 0 that
- * split register kernel is the one taken). */
+ * split register kernel is the one taken). option 5: 1 runs a fused pass as ONE
+ * persistent launch of the LDS kernel over every class (waves draw work items,
+ * runs of one class's chunks ordered by decreasing cost, from a device counter
+ * until the list is exhausted; chunk length per class from the autotuner), 0
+ * (default) one launch per class as above. */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
@@ -102,7 +106,7 @@ int ccsd_plan_quartets(const ccsd_plan* plan, long long* count);
 
 /* The launches of the last integral or fused-gradient call, 10 numbers each, in
  * launch order: li, lj, lk, ll, general-contraction variant (0/1), kernel (0 LDS,
- * 1 register), waves per block, quartets per chunk (low 16 bits; bits 16+: the
+ * 1 register, 2 class inside the one persistent launch), waves per block, quartets per chunk (low 16 bits; bits 16+: the
  * register kernel's primitive split, 0 = none), waves per quartet (register
  * kernel), quartets. *nrows receives the count; up to max_rows rows are written
  * to `rows` (may be NULL). No counterpart in the reference (profiling aid). */
