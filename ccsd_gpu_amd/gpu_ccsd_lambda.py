@@ -426,16 +426,25 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
         mycc._lambda_fp64_updates = 0
         mycc._lambda_normt = normt
         return normt < tol, l1, l2
+    # residual update after a correction d: r(l + d) = r(l) + A d - d exactly (F affine),
+    # with A d from one low-precision homogeneous update (a third of an FP64 update's
+    # cost). Its error is that precision's rounding of A d, of order 1e-5 |d| for
+    # split-BF16; convergence is accepted only with a margin of est_margin |d| on top,
+    # otherwise the FP64 residual is recomputed.
+    res_update = getattr(mycc, "lambda_residual_update", True)
+    est_margin = getattr(mycc, "lambda_est_margin", 1e-3)
+    r1 = r2 = None
     for _round in range(max(8, max_cycle // 4)):
-        r1, r2 = update(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
-        fp64_updates += 1
-        tick("fp64 residuals")
-        r1, r2 = r1 - l1, r2 - l2
-        normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))
-        mycc._lambda_history.append(("fp64-residual", False, normt))
-        if normt < tol:
-            conv = True
-            break
+        if r1 is None:
+            r1, r2 = update(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, "fp64")
+            fp64_updates += 1
+            tick("fp64 residuals")
+            r1, r2 = r1 - l1, r2 - l2
+            normt = float(torch.linalg.norm(amplitudes_to_vector(r1, r2)))
+            mycc._lambda_history.append(("fp64-residual", False, normt))
+            if normt < tol:
+                conv = True
+                break
         # correction: d = A d + r, to `lambda_corr_rtol` of the remaining residual in at most
         # `lambda_corr_max_cycle` sweeps: short rounds restart DIIS on a fresh FP64 residual
         # while its convergence is still fast (measured on (H2O)8 with the first phase to
@@ -448,6 +457,19 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
         total += it
         l1, l2 = l1 + d1, l2 + d2
         tick("correction sweeps")
+        if res_update:
+            a1, a2 = update(mycc, t1, t2, d1, d2, eris, imds, use_cpu_vvvv, precision, homogeneous=True)
+            e1, e2 = r1 + a1 - d1, r2 + a2 - d2
+            dnorm = float(torch.linalg.norm(amplitudes_to_vector(d1, d2)))
+            est = float(torch.linalg.norm(amplitudes_to_vector(e1, e2)))
+            total += 1
+            mycc._lambda_history.append(("residual-update", True, est))
+            tick("residual updates")
+            if est + est_margin * dnorm < tol:
+                normt = est
+                conv = True
+                break
+        r1 = r2 = None
     mycc._lambda_iterations = total
     mycc._lambda_fp64_updates = fp64_updates
     mycc._lambda_normt = normt
