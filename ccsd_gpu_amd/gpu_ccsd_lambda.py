@@ -47,8 +47,17 @@ def _require_gpu():
         raise RuntimeError("No usable HIP device for the lambda solve (this build has no CPU fallback).")
 
 
-def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
+def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=None):
+    """The lambda intermediates (the reference's make_intermediates, gpu_ccsd_lambda.py:68-180).
+
+    With lp_precision set (the mixed solve), the three o v^3 x o v (o^2) products of
+    wvvov are formed at that precision only (imds.wvvov then carries low-precision
+    accuracy, which is all its low-precision updates use), and the FP64 update applies
+    them to l2 in factored form from imds.wvvov_fac instead (_wvvov_l2_fp64): wvvov
+    enters the update once, in an l1 term, so its FP64 products (3.9 TFLOP at (H2O)8)
+    would serve a single FP64 residual."""
     C = lambda eq, *ops: contract(eq, *ops, precision=precision)  # noqa: E731
+    CW = C if lp_precision is None else (lambda eq, *ops: contract(eq, *ops, precision=lp_precision))  # noqa: E731
     t1 = _dev(t1)
     t2 = _dev(t2)
     nocc, nvir = t1.shape
@@ -73,13 +82,13 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     g2vovv = mfma.combine([(2, vvov.permute(0, 2, 1, 3)), (-1, vvov.permute(0, 2, 3, 1))])
     wvooo = C("cibd,jkbd->ckij", g2vovv, tau)
     del g2vovv
-    wvvov = C("jabd,jkcd->abkc", ovvv, t2) * -1.5
-    mfma.combine([(2, vvov.permute(0, 3, 2, 1)), (-1, vvov)], out=wvvov, beta=1)
+    # wvvov = wexp (the products with t2, theta and tau) + wcheap (integrals, t1 terms)
+    wexp = CW("jabd,jkcd->abkc", ovvv, t2) * -1.5
+    wcheap = mfma.combine([(2, vvov.permute(0, 3, 2, 1)), (-1, vvov)])
     g2vvov = mfma.combine([(2, vvov), (-1, ovvv.permute(1, 2, 0, 3))])
-    theta = t2 * 2 - t2.permute(0, 1, 3, 2)
-    vackb = C("acjd,kjbd->ackb", g2vvov, theta)
-    del g2vvov
-    mfma.combine([(1, vackb.permute(0, 3, 2, 1)), (-0.5, vackb)], out=wvvov, beta=1)
+    theta_b = t2 * 2 - t2.permute(0, 1, 3, 2)
+    vackb = CW("acjd,kjbd->ackb", g2vvov, theta_b)
+    mfma.combine([(1, vackb.permute(0, 3, 2, 1)), (-0.5, vackb)], out=wexp, beta=1)
     del vackb
 
     ovoo = eris.ovoo
@@ -116,8 +125,10 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     ov2 = VOov * 2 + vOOv
     wvooo = wvooo - C("jb,bikc->ckij", t1, ov1) + C("kb,bijc->ckij", t1, ov2)
     w3 = w3 + C("ckjb,kc->bj", ov2, t1)
-    wvvov = wvvov + C("ajkc,jb->abkc", ov1, t1) - C("ajkb,jc->abkc", ov2, t1)
-    wvvov = wvvov + C("laki,klbc->abic", g2ovoo, tau)
+    wcheap = wcheap + C("ajkc,jb->abkc", ov1, t1) - C("ajkb,jc->abkc", ov2, t1)
+    wexp = wexp + CW("laki,klbc->abic", g2ovoo, tau)
+    wvvov = wcheap + wexp
+    del wexp
 
     woooo = woooo + eris.oooo.permute(0, 2, 1, 3)
     w3 = w3 + C("bc,jc->bj", w1, t1) - C("kj,kb->bj", w2, t1)
@@ -125,6 +136,10 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64"):
     imds.w1, imds.w2, imds.w3, imds.w4 = w1, w2, w3, w4
     imds.woooo, imds.wvooo = woooo, wvooo
     imds.wVOov, imds.wvOOv, imds.wvvov = wVOov, wvOOv, wvvov
+    imds.wvvov_fac = None
+    if lp_precision is not None:
+        imds.wvvov_cheap = wcheap
+        imds.wvvov_fac = (g2vvov, theta_b, g2ovoo)
     _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0))
     return imds
 
@@ -228,7 +243,10 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     l2new.add_(m4, alpha=0.5)
     l1new = l1_terms(l1new.contiguous(), m4, t1, c1=2.0, c2=-1.0)
 
-    l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
+    if precision == "fp64" and getattr(imds, "wvvov_fac", None) is not None:
+        l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + _wvvov_l2_fp64(imds, t2, l2, ovvv, l2tau)
+    else:
+        l1new = l1new - C("ckij,jkca->ia", imds.wvooo, l2) + C("abkc,kibc->ia", imds.wvvov, l2)
     tmp_voov = imds.tmp_voov
     # tmp[i,a,j,b] = l2[i,j,a,b] - l2[i,j,b,a] / 2, stored [i][a][j][b]
     tmp = torch.empty((nocc, nvir, nocc, nvir), dtype=l2.dtype, device=l2.device)
@@ -245,6 +263,28 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
     l2new = pair_combine(l2new, c1=1.0, c4=1.0, d=imds.inv_d2)
     return l1new, l2new
+
+
+def _wvvov_l2_fp64(imds, t2, l2, ovvv, l2tau):
+    """sum_{bkc} wvvov[a,b,k,c] l2[k,i,b,c] in FP64 without forming wvvov's products:
+      -1.5 ovvv.t2 term:   -1.5 sum_{jbd} ovvv[j,a,b,d] X[j,d,i,b],  X = sum_{kc} t2[j,k,c,d] l2[k,i,b,c]
+      vackb terms:          sum_{bjd} g2vvov[a,b,j,d] Z[j,d,i,b],    Z = sum_{kc} theta[k,j,c,d] L'[k,i,b,c],
+                            L'[k,i,b,c] = l2[k,i,c,b] - l2[k,i,b,c] / 2
+      g2ovoo.tau term:      sum_{lkm} g2ovoo[l,a,k,m] l2tau[m,i,k,l]
+    plus the formed FP64 remainder (integrals and t1 terms). Two o^3 v^3 products
+    (0.45 TFLOP each at (H2O)8) replace 3.9 TFLOP of intermediates."""
+    C = lambda eq, *ops: contract(eq, *ops, precision="fp64")  # noqa: E731
+    g2vvov, theta_b, g2ovoo = imds.wvvov_fac
+    out = C("abkc,kibc->ia", imds.wvvov_cheap, l2)
+    x = C("jkcd,kibc->jdib", t2, l2)
+    out = out - 1.5 * C("jabd,jdib->ia", ovvv, x)
+    del x
+    lp = l2.transpose(2, 3) - 0.5 * l2
+    z = C("kjcd,kibc->jdib", theta_b, lp)
+    del lp
+    out = out + C("abjd,jdib->ia", g2vvov, z)
+    del z
+    return out + C("lakm,mikl->ia", g2ovoo, l2tau)
 
 
 # Largest doubles amplitude (o^2 v^2 elements) for which the mixed solve captures its update
@@ -318,7 +358,8 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     mycc._lambda_phases = phases
     tick = _Ticker(phases)
     # intermediates once in FP64: they enter every FP64 residual
-    imds = make_intermediates_gpu(mycc, t1, t2, eris, "fp64")
+    imds = make_intermediates_gpu(mycc, t1, t2, eris, "fp64",
+                                  lp_precision=None if precision == "fp64" else precision)
     tick("intermediates")
     diis_space = getattr(mycc, "diis_space", 6)
     use_diis = bool(getattr(mycc, "diis", True))

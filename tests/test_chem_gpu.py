@@ -291,6 +291,37 @@ def test_lambda_update_is_affine_with_homogeneous_part(gpu, h2o_pair):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("lp", ["bf16x3", "fp32"])
+def test_fp64_update_with_factored_wvvov_equals_formed(gpu, h2o_pair, lp):
+    """The mixed solve's intermediates (wvvov's products formed at low precision only)
+    give, through the factored FP64 application, the FP64 update of the fully formed
+    FP64 intermediates; the low-precision wvvov is within that precision of the FP64 one
+    (the water-tetramer test below covers sizes where it is formed at low precision)."""
+    from ccsd_gpu_amd.gpu_ccsd_lambda import make_intermediates_gpu, update_lambda_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    eris = mycc.ao2mo()
+    full = make_intermediates_gpu(mycc, mycc.t1, mycc.t2, eris)
+    split = make_intermediates_gpu(mycc, mycc.t1, mycc.t2, eris, lp_precision=lp)
+    assert full.wvvov_fac is None and split.wvvov_fac is not None
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    L1 = torch.as_tensor(l1, device="cuda") + 0.01 * torch.randn(l1.shape, dtype=torch.float64, device="cuda",
+                                                                  generator=gen)
+    L2 = torch.as_tensor(l2, device="cuda") + 0.01 * torch.randn(l2.shape, dtype=torch.float64, device="cuda",
+                                                                  generator=gen)
+    f1, f2 = update_lambda_gpu(mycc, mycc.t1, mycc.t2, L1, L2, eris, full)
+    g1, g2 = update_lambda_gpu(mycc, mycc.t1, mycc.t2, L1, L2, eris, split)
+    d1 = float((f1 - g1).abs().max()) / float(f1.abs().max())
+    d2 = float((f2 - g2).abs().max()) / float(f2.abs().max())
+    dw = float((full.wvvov - split.wvvov).abs().max()) / float(full.wvvov.abs().max())
+    print(lp, "l1", d1, "l2", d2, "wvvov", dw, flush=True)
+    assert d1 < 1e-12 and d2 < 1e-12
+    # (at this size every wvvov product is below the mixed-precision flop threshold and
+    # runs in FP64, so dw is 0 here; the (H2O)n solves form them at low precision)
+    assert dw < 1e-4
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
 def test_lambda_graph_replays_match_eager(gpu, h2o_pair, precision):
     """The HIP-graph replay of the lambda update equals the eager update on every
@@ -386,3 +417,26 @@ def test_contract_acc_scatter_matches_einsum(gpu, eq, shapes, monkeypatch):
     err = float((got - want).abs().max() / ref.abs().max())
     print(eq, err, flush=True)
     assert err < 2e-4
+
+
+@pytest.mark.gpu
+def test_fp64_update_with_factored_wvvov_water_tetramer(gpu):
+    """(H2O)4 / cc-pVDZ (o 20, v 76), where wvvov's products run at low precision in the
+    mixed solve: the factored FP64 application still gives the formed FP64 update, and the
+    low-precision wvvov differs from the FP64 one by that precision's rounding only."""
+    from ccsd_gpu_amd import pipeline, systems
+    from ccsd_gpu_amd.gpu_ccsd_lambda import make_intermediates_gpu, update_lambda_gpu
+    mol = systems.water_cluster(4, seed=1, cart=False)
+    mf, mycc = pipeline.setup(mol)
+    eris = mycc.ao2mo()
+    t1, t2 = mycc.t1, mycc.t2
+    full = make_intermediates_gpu(mycc, t1, t2, eris)
+    split = make_intermediates_gpu(mycc, t1, t2, eris, lp_precision="bf16x3")
+    f1, f2 = update_lambda_gpu(mycc, t1, t2, t1, t2, eris, full)
+    g1, g2 = update_lambda_gpu(mycc, t1, t2, t1, t2, eris, split)
+    d1 = float((f1 - g1).abs().max()) / float(f1.abs().max())
+    d2 = float((f2 - g2).abs().max()) / float(f2.abs().max())
+    dw = float((full.wvvov - split.wvvov).abs().max()) / float(full.wvvov.abs().max())
+    print("(H2O)4 factored vs formed: l1", d1, "l2", d2, "bf16x3 wvvov", dw, flush=True)
+    assert d1 < 1e-12 and d2 < 1e-12
+    assert 0 < dw < 1e-4
