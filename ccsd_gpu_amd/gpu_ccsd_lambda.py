@@ -152,6 +152,12 @@ def _static_pieces(imds, t1, t2, eris, level_shift):
     imds.tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
     imds.tmp_voov = (imds.wVOov * 2 + imds.wvOOv).contiguous()
     imds.l2_const = (eris.ovvo.permute(0, 3, 1, 2) * 0.5).contiguous()     # voov.permute(1, 2, 0, 3) / 2
+    # integral combinations that two matrix-vector terms of every update read in two index
+    # orders: one combined pass instead of two
+    ovvv = eris.ovvv
+    imds.g2ovvv = mfma.combine([(2, ovvv), (-1, ovvv.permute(0, 3, 1, 2))])      # 2 (ia|bc) - (ib|ca)
+    voov = eris.ovvo.permute(1, 0, 3, 2)
+    imds.g2voov_b = mfma.combine([(2, voov), (-1, voov.permute(3, 1, 2, 0))])    # 2 v[a,i,j,b] - v[b,i,j,a]
     e_o = eris.mo_energy[:nocc]
     e_v = eris.mo_energy[nocc:] + level_shift
     eia = e_o[:, None] - e_v
@@ -224,7 +230,7 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
 
     l1new = l1new - C("jb,jiab->ia", l1, eris.oovv)
     ovvv = eris.ovvv
-    l1new = l1new + C("iabc,bc->ia", ovvv, mba1) * 2 - C("ibca,bc->ia", ovvv, mba1)
+    l1new = l1new + C("iabc,bc->ia", imds.g2ovvv, mba1)
     acc(l2new, "jbac,ic->jiba", ovvv, l1)
     m4 = C("ijkd,kadb->ijab", l2t1, ovvv).contiguous()
     l2new.sub_(m4)
@@ -238,8 +244,10 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
         l2new.add_(imds.l2_const)
     acc(l2new, "bjic,ca->jiba", voov, mba1, -1.0)
     acc(l2new, "bjka,ik->jiba", voov, mij1, -1.0)
-    l1new = l1new + C("aijb,jb->ia", voov, mia1) * 2 - C("bija,jb->ia", voov, mia1)
+    l1new = l1new + C("aijb,jb->ia", imds.g2voov_b, mia1)
+    # l2tau.voov and woooo.l2 enter l2new (x 1/2) and the singles terms alike: one buffer
     m4 = C("ijkl,aklb->ijab", l2tau, voov).contiguous()
+    acc(m4, "ijkl,klab->ijab", imds.woooo, l2)
     l2new.add_(m4, alpha=0.5)
     l1new = l1_terms(l1new.contiguous(), m4, t1, c1=2.0, c2=-1.0)
 
@@ -255,9 +263,6 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     tmp = C("jkca,bikc->jiba", l2, imds.wvOOv)
     pair_combine(tmp, c1=1.0, c3=0.5, out=l2new, beta=1)         # l2new += tmp + tmp[j,i,a,b] / 2
 
-    m3 = C("ijkl,klab->ijab", imds.woooo, l2).contiguous()
-    l2new.add_(m3, alpha=0.5)
-    l1new = l1_terms(l1new.contiguous(), m3, t1, c1=2.0, c2=-1.0)
 
     l1new = l1new / imds.eia
     # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
@@ -421,7 +426,7 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
 
     with static_operands(t1, t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo, imds.w1,
                          imds.w2, imds.w3, imds.w4, imds.wvvov, imds.wvooo, imds.wVOov, imds.wvOOv, imds.woooo,
-                         imds.theta, imds.tau, imds.tmp_voov):
+                         imds.theta, imds.tau, imds.tmp_voov, imds.g2ovvv, imds.g2voov_b):
         if precision == "fp64":
             conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
             tick("fp64 updates")
