@@ -113,13 +113,17 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     del goovv
 
     gvvvv = C("ijab,ijcd->abcd", l2, tau)
-    gvovv = C("adbc,id->aibc", gvvvv, -t1)
+    # -sum_d g[a,d,b,c] t1[i,d], read as g[d,a,c,b] through g's (ab)(cd) symmetry: one
+    # contiguous FP64 GEMM over g's slowest index (no permuted copy of the v^4 tensor)
+    nocc, nvir = t1.shape
+    gv = torch.mm(t1, gvvvv.reshape(nvir, nvir ** 3)).view(nocc, nvir, nvir, nvir)    # [i, a, c, b]
+    gvovv = mfma.combine([(-1.0, gv.permute(1, 0, 3, 2))])
+    del gv
     acc(gvovv, "akic,kb->aibc", pvoOV, t1)
     acc(gvovv, "akib,kc->aibc", pvOOv, t1, -1.0)
-    acc(gvovv, "ja,jibc->aibc", l1, t2)
-    acc(gvovv, "ja,jbic->aibc", l1, E("jb,ic->jbic", t1, t1))
+    # l1[j,a] (t2 + t1 t1)[j,i,b,c] + t1[j,a] l2[j,i,b,c]: one product over K = 2 nocc
+    acc(gvovv, "ja,jibc->aibc", torch.cat([l1, t1]), torch.cat([tau, l2]))
     acc(gvovv, "ba,ic->aibc", mvv * 0.5, t1)
-    acc(gvovv, "ja,jibc->aibc", t1, l2)
     dovvv = mfma.combine([(2, gvovv.permute(1, 3, 0, 2)), (-1, gvovv.permute(1, 2, 0, 3))])
     return (dovov, gvvvv, doooo, doovv, dovvo, None, dovvv, dooov)
 
