@@ -131,11 +131,16 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
 # ---------------------------------------------------------------------------
 # Phase 2: full 2-RDM and MO -> AO
 # ---------------------------------------------------------------------------
-def _assemble_dm2_mo(d2):
+def _assemble_dm2_mo(d2, presym=False):
     """Full MO 2-RDM (chemist's order) from the density blocks (vvvv as gvvvv), as
     ccsd_gpu/gpu_ccsd_grad.py:272-300; each of the 16 occupied/virtual blocks is
     written once by the combine kernel (no zero fill, transposed blocks through
-    LDS tiles)."""
+    LDS tiles).
+
+    presym=True is for callers that apply the (pq)(rs) symmetrisation next: the vvvv
+    block's third and fourth terms are the (pq)(rs) images of its first two, so
+    4 A - 2 B replaces 2 A - B + 2 C - D there (two reads of gvvvv instead of four);
+    the symmetrised tensor is the same, the unsymmetrised one is not."""
     dovov, gvvvv, doooo, doovv, dovvo, _, dovvv, dooov = d2
     nocc, nvir = dovov.shape[:2]
     nmo = nocc + nvir
@@ -152,8 +157,11 @@ def _assemble_dm2_mo(d2):
     # through its (ab)(cd) symmetry g[a,b,c,d] = g[b,a,d,c] so that every view is contiguous
     # along d or along c (one LDS-tiled dimension)
     g = gvvvv
-    cb([(2, g.permute(0, 2, 1, 3)), (-1, g.permute(1, 2, 0, 3)), (2, g.permute(2, 0, 3, 1)),
-        (-1, g.permute(2, 1, 3, 0))], out=dm2[v, v, v, v])
+    if presym:
+        cb([(4, g.permute(0, 2, 1, 3)), (-2, g.permute(1, 2, 0, 3))], out=dm2[v, v, v, v])
+    else:
+        cb([(2, g.permute(0, 2, 1, 3)), (-1, g.permute(1, 2, 0, 3)), (2, g.permute(2, 0, 3, 1)),
+            (-1, g.permute(2, 1, 3, 0))], out=dm2[v, v, v, v])
     cb([(2, doooo), (2, doooo.permute(1, 0, 3, 2))], out=dm2[o, o, o, o])
     cb([(1, dovvv)], out=dm2[o, v, v, v])
     cb([(1, dovvv.permute(2, 3, 0, 1))], out=dm2[v, v, o, v])
@@ -196,7 +204,7 @@ def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
     to the MO tensor while p >= q is packed out of it: the r and s transforms then
     run on half the rows. Their product is repacked (rs pair packed, pq unpacked)
     and the p and q transforms write the packed layout directly."""
-    dm2 = _assemble_dm2_mo(d2)
+    dm2 = _assemble_dm2_mo(d2, presym=True)
     c = _dev(mo_coeff)
     nmo = c.shape[1]
     tp = _pack_pairs_sym(dm2)                    # (pq r s), p >= q, symmetrised

@@ -24,3 +24,27 @@ def test_gamma2_and_assembly_match_oracle(monkeypatch):
     T = lambda a: torch.as_tensor(a)  # noqa: E731
     got = G._assemble_dm2_mo(G._gamma2_intermediates_gpu(None, T(t1), T(t2), T(l1), T(l2))).numpy()
     assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_presym_assembly_equals_full_after_pair_symmetrisation(monkeypatch):
+    """The packed path's assembly (vvvv block as 4 A - 2 B) differs from the full one
+    elementwise but not after the (pq)(rs) symmetrisation that follows it."""
+    import ccsd_gpu_amd.gpu_ccsd_grad as G
+    monkeypatch.setattr(G, "_dev", lambda x: torch.as_tensor(x))
+    monkeypatch.setattr(G, "DEV", "cpu")
+    o, v = 2, 5
+    rng = np.random.default_rng(9)
+    t1, l1 = rng.standard_normal((o, v)) * 0.1, rng.standard_normal((o, v)) * 0.1
+    t2 = rng.standard_normal((o, o, v, v)) * 0.1
+    t2 = t2 + t2.transpose(1, 0, 3, 2)
+    l2 = rng.standard_normal((o, o, v, v)) * 0.1
+    l2 = l2 + l2.transpose(1, 0, 3, 2)
+    T = lambda a: torch.as_tensor(a)  # noqa: E731
+    d2 = G._gamma2_intermediates_gpu(None, T(t1), T(t2), T(l1), T(l2))
+    full = G._assemble_dm2_mo(d2)
+    pre = G._assemble_dm2_mo(d2, presym=True)
+    sym = lambda x: (x + x.permute(1, 0, 2, 3) + x.permute(0, 1, 3, 2) + x.permute(1, 0, 3, 2)) / 2  # noqa: E731
+    assert float((pre - full).abs().max()) > 1e-6
+    assert float((sym(pre) - sym(full)).abs().max()) <= 1e-13 * float(full.abs().max())
+    # and the packed layout the gradient uses, from either
+    assert float((G._pack_pairs_sym(pre) - G._pack_pairs_sym(full)).abs().max()) <= 1e-13 * float(full.abs().max())
