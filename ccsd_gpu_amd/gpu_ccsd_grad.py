@@ -100,9 +100,11 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     acc(goovv, "bd,ijad->ijab", mvv * 0.5, tau, -1.0)
     goovv.addcmul_(mia[:, None, :, None], t1[None, :, None, :])
     mfma.combine([(0.5, tau), (0.5, l2)], out=goovv, beta=1)
-    t1t1 = E("jc,kb->jckb", t1, t1)
-    gOvvO = C("kiac,jckb->iabj", l2, t1t1) + pvOOv.permute(1, 0, 3, 2)
-    govVO = E("ia,jb->iabj", l1, t1) - C("ikac,jckb->iabj", l2, t1t1) + pvoOV.permute(1, 0, 3, 2)
+    # the l2 (t1 t1) terms through w[p,q,a,j] = sum_c l2[p,q,a,c] t1[j,c] (t1 t1 is rank one)
+    w = C("pqac,jc->pqaj", l2, t1)
+    gOvvO = C("kiaj,kb->iabj", w, t1) + pvOOv.permute(1, 0, 3, 2)
+    govVO = E("ia,jb->iabj", l1, t1) - C("ikaj,kb->iabj", w, t1) + pvoOV.permute(1, 0, 3, 2)
+    del w
     dovvo = 2 * govVO + gOvvO
     doovv = mfma.combine([(-2, gOvvO.permute(3, 0, 1, 2)), (-1, govVO.permute(3, 0, 1, 2))])
     tau2 = tau - t2 * 0.5

@@ -114,9 +114,12 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     w2 = w2 + C("jkbc,bikc->ij", t2, g2voov) + C("ib,jb->ij", tmpw4, t1)
     w3 = w3 + t1.T @ tmpw4 @ t1.T
     w4 = w4 + tmpw4
-    t1t1 = torch.einsum("kd,lc->kdlc", t1, t1)
-    wvOOv = wvOOv + C("bljd,kdlc->bjkc", voov, t1t1)
-    wVOov = wVOov - C("bjld,kdlc->bjkc", voov, t1t1)
+    # sum_ld voov[b,l,j,d] t1[k,d] t1[l,c] and sum_ld voov[b,j,l,d] t1[k,d] t1[l,c] through
+    # y[b,p,q,k] = sum_d voov[b,p,q,d] t1[k,d] (t1 t1 is rank one: o^2 v^2 x o v work, not o^3 v^3)
+    y = C("bpqd,kd->bpqk", voov, t1)
+    wvOOv = wvOOv + C("bljk,lc->bjkc", y, t1)
+    wVOov = wVOov - C("bjlk,lc->bjkc", y, t1)
+    del y
     VOov = C("bjld,klcd->bjkc", g2voov, t2) - C("bjld,kldc->bjkc", voov, t2) + voov
     vOOv = C("bljd,kldc->bjkc", voov, t2) - eris.oovv.permute(2, 1, 0, 3)
     wVOov = wVOov + VOov
