@@ -46,6 +46,7 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
 hipError_t ccsd_launch_sum_slices(double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
 hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream);
+hipError_t ccsd_launch_add_separable_packed(double* g, const double* D, const double* P, int n, hipStream_t stream);
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream);
 size_t ccsd_l1_terms_lds(int v);
@@ -1500,6 +1501,15 @@ int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream)
     if (n < 0 || m < 0 || (long long)n * (n + 1) / 2 > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "bad dimensions");
     hipError_t e = ccsd_launch_repack_pairs(z, n, m, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "repack_pairs_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_add_separable_packed(double* gamma, const double* dm, const double* pm, int n, void* stream)
+{
+    if (!gamma || !dm || !pm) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || n > 2048) return fail(CCSD_ERR_SHAPE, "n must be 0..2048");
+    hipError_t e = ccsd_launch_add_separable_packed(gamma, dm, pm, n, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "add_separable_packed_kernel");
     return CCSD_OK;
 }
 

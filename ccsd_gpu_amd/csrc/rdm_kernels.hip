@@ -118,7 +118,51 @@ __global__ void __launch_bounds__(256) repack_pairs_kernel(const double* __restr
     }
 }
 
+// g[i][j][p] += w_kl * Gsep(i, j, k, l) for p = k (k + 1) / 2 + l, k >= l, w_kl = 1/2 on the
+// diagonal k == l, else 1, with the separable two-particle density of D and P
+//   Gsep = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4
+// (the packed, diagonal-halved form the fused gradient kernel reads). One block per (i, j);
+// D[i][:], P[i][:], D[j][:] and P[:][j] staged in LDS, D_kl / P_kl read through the cache.
+__global__ void __launch_bounds__(256) add_separable_packed_kernel(double* __restrict__ g,
+                                                                  const double* __restrict__ D,
+                                                                  const double* __restrict__ P, int n)
+{
+    extern __shared__ double rows[];
+    double* Di = rows;
+    double* Pi = rows + n;
+    double* Dj = rows + 2 * n;
+    double* Pcj = rows + 3 * n;
+    const int i = blockIdx.x / n, j = blockIdx.x - (blockIdx.x / n) * n;
+    for (int t = threadIdx.x; t < n; t += 256) {
+        Di[t] = D[(long long)i * n + t];
+        Pi[t] = P[(long long)i * n + t];
+        Dj[t] = D[(long long)j * n + t];
+        Pcj[t] = P[(long long)t * n + j];
+    }
+    __syncthreads();
+    const double dij = D[(long long)i * n + j], pij = P[(long long)i * n + j];
+    const long long npair = (long long)n * (n + 1) / 2;
+    double* G = g + ((long long)i * n + j) * npair;
+    for (long long p = threadIdx.x; p < npair; p += 256) {
+        int k, l;
+        pair_of(p, k, l);
+        const long long kl = (long long)k * n + l;
+        const double s = dij * P[kl] + pij * D[kl] -
+                         0.25 * (Di[l] * Pcj[k] + Di[k] * Pcj[l] + Pi[l] * Dj[k] + Pi[k] * Dj[l]);
+        G[p] += (k == l) ? 0.5 * s : s;
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_add_separable_packed(double* g, const double* D, const double* P, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if ((long long)n * n > 0x7fffffffLL || n > 2048) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(add_separable_packed_kernel, dim3((unsigned)((long long)n * n)), dim3(256),
+                       sizeof(double) * 4 * n, stream, g, D, P, n);
+    return hipGetLastError();
+}
 
 hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream)
 {

@@ -82,6 +82,19 @@ def set_screening(mol, thr):
                "ccsd_plan_screen")
 
 
+def add_separable(gamma_cart, dm, pm):
+    """In place: gamma_cart (nc, nc, nc(nc+1)/2, packed k >= l, diagonal halved) +=
+    the separable density of dm and pm (nc x nc) in the same packed form, so that an
+    explicit pass of contract_int2e_ip1 over it equals the pass with dm= and pm=."""
+    n = dm.shape[0]
+    if gamma_cart.shape != (n, n, n * (n + 1) // 2) or not gamma_cart.is_contiguous():
+        raise ValueError("gamma_cart must be a contiguous (n, n, n(n+1)/2) tensor")
+    dm, pm = dm.contiguous(), pm.contiguous()
+    _lib.check(_lib.load().ccsd_add_separable_packed(_lib.ptr(gamma_cart), _lib.ptr(dm), _lib.ptr(pm), n,
+                                                     _lib.current_stream_ptr()), "ccsd_add_separable_packed")
+    return gamma_cart
+
+
 def set_persistent(mol, on=True):
     """Fused passes of `mol`'s plan as one persistent launch of the generic kernel over
     every class (True) or one launch per class on four streams (False, the default)."""

@@ -191,6 +191,14 @@ int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* l
  * pair's remaining transforms. */
 int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream);
 
+/* gamma[i][j][k(k+1)/2 + l] += w_kl * Gsep_ijkl for k >= l, w_kl = 1/2 if k == l else 1,
+ *   Gsep_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4,
+ * gamma (n, n, n(n+1)/2), dm = D and pm = P (n, n) row-major, all device arrays, n <= 2048.
+ * The HF-density terms of the gradient (the reference's vhf1 blocks contracted with dm1p,
+ * ccsd_gpu/gpu_ccsd_grad.py:462-489, :570, :583) added into the packed CCSD density once,
+ * so that one explicit ccsd_grad_ip1_gamma pass covers both. */
+int ccsd_add_separable_packed(double* gamma, const double* dm, const double* pm, int n, void* stream);
+
 /* Singles terms of the lambda update that contract a doubles-shaped tensor
  * x (o, o, v, v, contiguous) with a singles-shaped y (y[j][b] at j*sy0 + b*sy1):
  *   out[i][a] += sum_{j,b} (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) y[j,b]

@@ -180,3 +180,28 @@ def test_sym_pairs_kernel_matches_torch(gpu):
         ref = 0.5 * (t + t.permute(1, 0, 2, 3) + t.permute(0, 1, 3, 2) + t.permute(1, 0, 3, 2))
         got = _lib_sym_pairs(t.clone(), 0.5)
         assert float((got - ref).abs().max()) < 1e-14
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("make", [molecules.h2o_ccpvdz, molecules.ch4_sto3g])
+def test_add_separable_then_explicit_pass_equals_in_kernel_separable(gpu, make):
+    """gamma += Gsep(D, P) (packed, diagonal halved) by the HIP kernel equals the torch
+    construction, and an explicit fused pass over it equals the pass that forms the
+    separable part inside the kernel (dm=, pm=)."""
+    from ccsd_gpu_amd.gpu_ccsd_grad import _separable_gamma
+    from ccsd_gpu_amd.hip.grad2e import add_separable, pack_s2kl
+    mol = make()
+    nc = mol.nao_cart()
+    rng = np.random.default_rng(17)
+    D = rng.standard_normal((nc, nc)); D = D + D.T
+    P = rng.standard_normal((nc, nc)) * 0.3          # not symmetric: the kernel must not assume it
+    g0 = torch.as_tensor(_random_gamma(mol, 29), device="cuda")
+    Dt, Pt = torch.as_tensor(D, device="cuda"), torch.as_tensor(P, device="cuda")
+    ref_g = g0 + pack_s2kl(_separable_gamma(Dt, Pt), halve_diag=True)
+    got_g = add_separable(g0.clone(), Dt, Pt)
+    assert float((got_g - ref_g).abs().max()) <= 1e-13 * float(ref_g.abs().max())
+    ref = contract_int2e_ip1(mol, g0, scale=-2.0, dm=Dt, pm=Pt).cpu().numpy()
+    got = contract_int2e_ip1(mol, got_g, scale=-2.0).cpu().numpy()
+    scale = np.abs(ref).max()
+    print(make.__name__, "add_separable + explicit vs in-kernel separable", np.abs(got - ref).max() / scale, flush=True)
+    assert np.abs(got - ref).max() <= 1e-11 * scale
