@@ -355,6 +355,10 @@ def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
 # ---------------------------------------------------------------------------
 # Phases 3-5
 # ---------------------------------------------------------------------------
+# largest Cartesian AO count of ccsd_add_separable_packed (it stages four AO rows in LDS)
+ADD_SEPARABLE_MAX_NAO = 2048
+
+
 def _separable_gamma(D, P):
     """Gamma_sep(ij,kl) = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl)/4."""
     g = E("ij,kl->ijkl", D, P)
@@ -439,11 +443,15 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     de2 = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
     Dc = c2s @ hf_dm1 @ c2s.T
     Pc = c2s @ dm1p @ c2s.T
-    if fused:
+    if fused and Dc.shape[0] <= ADD_SEPARABLE_MAX_NAO:
         # one explicit pass over the CCSD density with the D/P separable part added into it
         # (one HBM pass; forming it per element inside the pass cost 40% more there)
         add_separable(gp, Dc, Pc)
         contract_int2e_ip1(mol, gp, scale=-2.0, de=de2)
+        del gp
+    elif fused:
+        # the separable part formed inside the fused pass (no row-staging limit)
+        contract_int2e_ip1(mol, gp, scale=-2.0, de=de2, dm=Dc.contiguous(), pm=Pc.contiguous())
         del gp
     else:
         _grad2e_unfused(mol, gp, Dc, Pc, de2)
