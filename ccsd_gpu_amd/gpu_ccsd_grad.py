@@ -439,6 +439,40 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     dm1 = mo @ dm1mo @ mo.T
     dm1p = hf_dm1 + dm1 * 2
 
+    def one_electron():
+        """Phase 5: one-electron and overlap terms (they need the relaxed density only)."""
+        s1 = -int1e(mol, "ipovlp", c2s)
+        vrinv_all = int1e(mol, "ipnuc_atom", c2s)
+        h1 = -(int1e(mol, "ipkin", c2s) + vrinv_all.sum(dim=0))
+        zeta = (mo_energy[:, None] + mo_energy[None, :]) * 0.5
+        zeta[nocc:, :nocc] = mo_energy[:nocc]
+        zeta[:nocc, nocc:] = mo_energy[:nocc].reshape(-1, 1)
+        zeta = mo @ (zeta * dm1mo) @ mo.T
+        p1 = mo[:, :nocc] @ mo[:, :nocc].T
+        j, k = ints.get_jk(dm1 + dm1.T)
+        vhf_s1occ = p1 @ (j - 0.5 * k) @ p1
+        dm1_tot = dm1 + hf_dm1
+        mocc = mo[:, :nocc]
+        zeta = zeta + (mocc * (mo_energy[:nocc] * 2.0)) @ mocc.T
+        # every term is a sum over the AO rows i of atom A (the reference's per-atom loop,
+        # gpu_ccsd_grad.py:554-583): form all row sums at once, then add them up by atom
+        offs = mol.offset_nr_by_atom()
+        ao_atom = torch.as_tensor(np.repeat(np.arange(mol.natm), offs[:, 3] - offs[:, 2]), device=DEV)
+        sym_dm = dm1_tot + dm1_tot.T
+        w = im1 + im1.T - zeta - zeta.T - 2.0 * vhf_s1occ
+        rows = torch.einsum("xij,ij->ix", s1, w) + torch.einsum("xij,ij->ix", h1, sym_dm)   # (nao, 3)
+        de_all = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
+        de_all.index_add_(0, ao_atom, rows)
+        de_all += torch.einsum("axij,ij->ax", vrinv_all, sym_dm)
+        return de_all
+
+    # Phase 5 runs on a side stream of default priority under the two-electron pass: it
+    # waits for the work so far (not for the pass), and is launched after the pass is
+    # enqueued, since some of its host steps wait for the device
+    main = torch.cuda.current_stream()
+    densities_ready = torch.cuda.Event()
+    densities_ready.record(main)
+
     # Phase 3b: two-electron derivative contribution
     de2 = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
     Dc = c2s @ hf_dm1 @ c2s.T
@@ -456,32 +490,14 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     else:
         _grad2e_unfused(mol, gp, Dc, Pc, de2)
         del gp
-    t0 = mark("grad2e", t0)
+    side = torch.cuda.Stream()
+    side.wait_event(densities_ready)
+    with torch.cuda.stream(side):
+        de_all = one_electron()
+    main.wait_stream(side)
+    de_all.record_stream(main)
+    t0 = mark("grad2e", t0)     # (includes the one-electron terms, run under the pass)
 
-    # Phase 5: one-electron and overlap terms
-    s1 = -int1e(mol, "ipovlp", c2s)
-    vrinv_all = int1e(mol, "ipnuc_atom", c2s)
-    h1 = -(int1e(mol, "ipkin", c2s) + vrinv_all.sum(dim=0))
-    zeta = (mo_energy[:, None] + mo_energy[None, :]) * 0.5
-    zeta[nocc:, :nocc] = mo_energy[:nocc]
-    zeta[:nocc, nocc:] = mo_energy[:nocc].reshape(-1, 1)
-    zeta = mo @ (zeta * dm1mo) @ mo.T
-    p1 = mo[:, :nocc] @ mo[:, :nocc].T
-    j, k = ints.get_jk(dm1 + dm1.T)
-    vhf_s1occ = p1 @ (j - 0.5 * k) @ p1
-    dm1_tot = dm1 + hf_dm1
-    mocc = mo[:, :nocc]
-    zeta = zeta + (mocc * (mo_energy[:nocc] * 2.0)) @ mocc.T
-    # every term is a sum over the AO rows i of atom A (the reference's per-atom loop,
-    # gpu_ccsd_grad.py:554-583): form all row sums at once, then add them up by atom
-    offs = mol.offset_nr_by_atom()
-    ao_atom = torch.as_tensor(np.repeat(np.arange(mol.natm), offs[:, 3] - offs[:, 2]), device=DEV)
-    sym_dm = dm1_tot + dm1_tot.T
-    w = im1 + im1.T - zeta - zeta.T - 2.0 * vhf_s1occ
-    rows = torch.einsum("xij,ij->ix", s1, w) + torch.einsum("xij,ij->ix", h1, sym_dm)   # (nao, 3)
-    de_all = torch.zeros((mol.natm, 3), dtype=torch.float64, device=DEV)
-    de_all.index_add_(0, ao_atom, rows)
-    de_all += torch.einsum("axij,ij->ax", vrinv_all, sym_dm)
     de_all += de2
     atoms = range(mol.natm) if atmlst is None else list(atmlst)
     de = de_all[list(atoms)]
