@@ -128,8 +128,9 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     ov2 = VOov * 2 + vOOv
     wvooo = wvooo - C("jb,bikc->ckij", t1, ov1) + C("kb,bijc->ckij", t1, ov2)
     w3 = w3 + C("ckjb,kc->bj", ov2, t1)
-    wcheap = wcheap + C("ajkc,jb->abkc", ov1, t1) - C("ajkb,jc->abkc", ov2, t1)
-    wexp = wexp + CW("laki,klbc->abic", g2ovoo, tau)
+    contract_acc(wcheap, "ajkc,jb->abkc", ov1, t1, precision=precision)
+    contract_acc(wcheap, "ajkb,jc->abkc", ov2, t1, alpha=-1.0, precision=precision)
+    contract_acc(wexp, "laki,klbc->abic", g2ovoo, tau, precision=lp_precision or precision)
     wvvov = wcheap + wexp
     del wexp
 
