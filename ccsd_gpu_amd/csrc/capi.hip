@@ -836,7 +836,8 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
             int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c, true);
             if (rc) return rc;
         }
-    PersistPlan pp;
+    // built in place in *out, so that the caller frees whatever was allocated if a step fails
+    PersistPlan& pp = *out;
     std::vector<EriClassArgs> cls;
     std::vector<const ChunkList*> lists;
     double total = 0.0;
@@ -862,10 +863,7 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
             cls.push_back(a);
             lists.push_back(cl);
         }
-    if (cls.empty()) {
-        *out = pp;
-        return CCSD_OK;
-    }
+    if (cls.empty()) return CCSD_OK;
     if (p->n_cu == 0) HIP_TRY(hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
     const int bpc = ccsd_persistent_blocks_per_cu(pp.lds);
     if (bpc <= 0) return fail(CCSD_ERR_UNSUPPORTED, "persistent kernel cannot be resident at this LDS size");
@@ -896,7 +894,6 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
     HIP_TRY(hipMalloc(&pp.d_items, sizeof(int4) * h.size()));
     HIP_TRY(hipMemcpy(pp.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&pp.d_counter, sizeof(double)));
-    *out = pp;
     return CCSD_OK;
 }
 
