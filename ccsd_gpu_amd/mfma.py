@@ -152,6 +152,43 @@ def k_splits(tiles, k, want=1024, min_chunk=1024):
     return max(1, min(-(-want // tiles), k // min_chunk))
 
 
+_KSPLIT_TUNED = {}
+
+
+def tuned_ksplit(A, B, candidates=(1, 2, 3, 4), reps=2):
+    """K split of gemm_nt(A, B) chosen by timing the candidates once per shape (HIP events,
+    into a throw-away output) and cached; the k_splits heuristic when not timeable (under
+    HIP-graph capture, or off the GPU). A product whose output tiles fill a non-integer
+    number of rounds of the chip's resident blocks can gain from a split (the (H2O)8 l2.vvvv
+    product, 7 x 181 tiles: 3.98 -> 3.56 ms at a split of 3), others lose the extra sum."""
+    nb, m, k = A.hi.shape
+    n = B.hi.shape[1]
+    key = (nb, m, n, k)
+    if key in _KSPLIT_TUNED:
+        return _KSPLIT_TUNED[key]
+    if not A.hi.is_cuda or torch.cuda.is_current_stream_capturing():
+        return None
+    out = torch.empty((nb, m, n), dtype=torch.float64, device=A.hi.device)
+    stream = torch.cuda.current_stream()
+    best, best_ms = None, float("inf")
+    for ks in candidates:
+        if ks > 1 and k < 2 * 1024 * ks:
+            continue
+        gemm_nt(A, B, out=out, ksplit=ks)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            gemm_nt(A, B, out=out, ksplit=ks)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        if ms < best_ms:
+            best, best_ms = ks, ms
+    _KSPLIT_TUNED[key] = best
+    return best
+
+
 def _launch_nt(lib, A, B, C, a_off, b_off, m, n, kc, lda, ldb, sA, sB, sC, ldc, batch, beta):
     ptr = _lib.c_void_p
     e = A.hi.element_size()

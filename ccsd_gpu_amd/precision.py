@@ -333,7 +333,8 @@ def vvvv_term(l2, vvvv, precision, scale=1.0):
         X = A[0] @ B
     elif precision == "bf16x3":
         B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
-        X = mfma.gemm_nt(mfma.split(A), B)[0]
+        As = mfma.split(A)
+        X = mfma.gemm_nt(As, B, ksplit=mfma.tuned_ksplit(As, B))[0]
     else:
         B = _cached(vvvv, ("vvvv-abcd-f32",),
                     lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).float().contiguous())
