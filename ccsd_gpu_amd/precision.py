@@ -318,6 +318,47 @@ def contract(eq, *ops, precision="fp64"):
     return cur
 
 
+_FP64_KSPLIT = {}
+
+
+def _fp64_matmul_ksplit(A, B):
+    """A @ B (FP64, A contiguous (m, K), B contiguous (K, n)) with K cut into `ks` equal
+    chunks run as one batched GEMM and summed; ks from {1, 2, 4, 8} (dividing K) timed
+    once per shape and cached (not timed under HIP-graph capture: ks = 1 there). The
+    (H2O)8 l2.vvvv shape, 820 x 23104 x 23104: 16.0 ms unsplit, 13.8 ms at ks = 4."""
+    m, K = A.shape
+    n = B.shape[1]
+
+    def run(ks):
+        if ks == 1:
+            return A @ B
+        kc = K // ks
+        return torch.bmm(A.view(m, ks, kc).transpose(0, 1), B.view(ks, kc, n)).sum(0)
+
+    key = (m, K, n)
+    ks = _FP64_KSPLIT.get(key)
+    if ks is None:
+        if not A.is_cuda or torch.cuda.is_current_stream_capturing():
+            return run(1)
+        best, best_ms = 1, float("inf")
+        stream = torch.cuda.current_stream()
+        for c in (1, 2, 4, 8):
+            if K % c or (c > 1 and K // c < 1024):
+                continue
+            run(c)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            run(c)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            if ms < best_ms:
+                best, best_ms = c, ms
+        _FP64_KSPLIT[key] = ks = best
+    return run(ks)
+
+
 def vvvv_term(l2, vvvv, precision, scale=1.0):
     """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update).
 
@@ -330,7 +371,7 @@ def vvvv_term(l2, vvvv, precision, scale=1.0):
     if precision == "fp64":
         # B[(a, b), (c, d)] = (ac|bd) is symmetric: X = A B (DGEMM on the cached matrix)
         B = _cached(vvvv, ("vvvv-abcd-f64",), lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).contiguous())
-        X = A[0] @ B
+        X = _fp64_matmul_ksplit(A[0], B)
     elif precision == "bf16x3":
         B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
         As = mfma.split(A)
