@@ -110,6 +110,46 @@ def _dgemm(Av, Bv):
                 mfma.gemv(Bv[b].T, Av[b, 0, :], out[b, 0, :])
         return out
     s = mfma.k_splits(nb * -(-m // 64) * -(-n // 64), k, want=512, min_chunk=2048) if nb <= 8 else 1
+    if s <= 1 and nb == 1 and Av.is_cuda and k >= 8192 and m > 1 and n > 1:
+        s = _tuned_dgemm_split(Av, Bv)
+    return _dgemm_run(Av, Bv, s)
+
+
+_DGEMM_SPLIT = {}
+
+
+def _tuned_dgemm_split(Av, Bv, candidates=(1, 2, 4)):
+    """K split of a long-K FP64 product whose output tiles leave the last round of resident
+    workgroups part empty (e.g. 6080 x 1600 outputs over K = 23104): the candidates are
+    timed once per shape and operand layout and the fastest cached; 1 under capture."""
+    _, m, k = Av.shape
+    n = Bv.shape[2]
+    key = (m, k, n, tuple(Av.stride()), tuple(Bv.stride()))
+    s = _DGEMM_SPLIT.get(key)
+    if s is not None:
+        return s
+    if torch.cuda.is_current_stream_capturing():
+        return 1
+    stream = torch.cuda.current_stream()
+    best, best_ms = 1, float("inf")
+    for c in candidates:
+        _dgemm_run(Av, Bv, c)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        _dgemm_run(Av, Bv, c)
+        e1.record(stream)
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        if ms < best_ms:
+            best, best_ms = c, ms
+    _DGEMM_SPLIT[key] = best
+    return best
+
+
+def _dgemm_run(Av, Bv, s):
+    nb, m, k = Av.shape
+    n = Bv.shape[2]
     if s <= 1:
         return torch.mm(Av[0], Bv[0]).unsqueeze(0) if nb == 1 else torch.bmm(Av, Bv)
     kc = k // s
