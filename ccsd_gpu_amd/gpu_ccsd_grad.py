@@ -114,10 +114,20 @@ def _gamma2_intermediates_gpu(mycc, t1, t2, l1, l2, precision="fp64"):
     dovov = mfma.combine([(2, goovv.permute(0, 2, 1, 3)), (-1, goovv.permute(1, 2, 0, 3))])
     del goovv
 
-    gvvvv = C("ijab,ijcd->abcd", l2, tau)
+    nocc, nvir = t1.shape
+    if l2.is_cuda:
+        # g[b,a,d,c] = g[a,b,c,d]: only the rows a <= b are formed (half the o^2 v^4 product),
+        # the other half mirrored by the pair-unpack kernel
+        from .pairops import unpack_pairs
+        iu, ju = torch.triu_indices(nvir, nvir, device=l2.device)
+        half = C("ijp,ijq->pq", l2.reshape(nocc, nocc, nvir * nvir)[:, :, iu * nvir + ju],
+                 tau.reshape(nocc, nocc, nvir * nvir))
+        gvvvv = unpack_pairs(half.reshape(-1, nvir, nvir).contiguous(), iu, ju, nvir)
+        del half
+    else:
+        gvvvv = C("ijab,ijcd->abcd", l2, tau)
     # -sum_d g[a,d,b,c] t1[i,d], read as g[d,a,c,b] through g's (ab)(cd) symmetry: one
     # contiguous FP64 GEMM over g's slowest index (no permuted copy of the v^4 tensor)
-    nocc, nvir = t1.shape
     gv = torch.mm(t1, gvvvv.reshape(nvir, nvir ** 3)).view(nocc, nvir, nvir, nvir)    # [i, a, c, b]
     gvovv = mfma.combine([(-1.0, gv.permute(1, 0, 3, 2))])
     del gv
