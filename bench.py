@@ -292,7 +292,7 @@ def bench_precision(args):
         mycc.lambda_refine = refine
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        l1, l2 = solve_lambda_gpu(mycc, t1, t2, precision=prec)
+        l1, l2 = solve_lambda_gpu(mycc, t1, t2, precision=prec, return_tensors=True)
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         mycc.lambda_refine = True

@@ -19,26 +19,59 @@ def E(eq, *ops):
     return contract(eq, *ops)
 
 
-class ERIs:
-    """MO integrals on the device (from the Cartesian AO ERIs of the RHF object)."""
+def frozen_mask(nmo, frozen):
+    """Active-orbital mask of PySCF's `frozen` convention: None / 0 -> all active; an
+    int n -> the n lowest orbitals frozen; a sequence -> those orbital indices frozen."""
+    mask = np.ones(nmo, dtype=bool)
+    if frozen is None or (np.isscalar(frozen) and int(frozen) == 0):
+        return mask
+    if np.isscalar(frozen):
+        n = int(frozen)
+        if n < 0 or n > nmo:
+            raise ValueError(f"frozen={frozen} out of range for {nmo} orbitals")
+        mask[:n] = False
+        return mask
+    idx = np.asarray(list(frozen), dtype=np.int64)
+    if idx.size and (idx.min() < 0 or idx.max() >= nmo):
+        raise ValueError(f"frozen orbital indices {idx.tolist()} out of range for {nmo} orbitals")
+    mask[idx] = False
+    return mask
 
-    def __init__(self, mf, mo_coeff=None):
-        ints = mf._ints()
-        c = torch.as_tensor(np.asarray(mf.mo_coeff if mo_coeff is None else mo_coeff), dtype=torch.float64,
-                            device=DEV)
-        cc = ints.c2s @ c                                   # Cartesian AO -> MO
-        nocc = int((mf.mo_occ > 0).sum())
+
+class ERIs:
+    """MO integrals on the device over the active orbitals, in the layout of PySCF's
+    _ChemistsERIs (chemist's-notation blocks, fock, mo_energy), built from the
+    Cartesian AO ERIs. The Fock matrix is that of the HF density of ALL occupied
+    orbitals (frozen ones included), projected on the active orbitals, as PySCF's
+    CCSD eris hold it."""
+
+    def __init__(self, mf, mo_coeff=None, mask=None, ints=None):
+        from .scf import get_ints
+        ints = get_ints(mf) if ints is None else ints
+        c_all = torch.as_tensor(np.asarray(mf.mo_coeff if mo_coeff is None else mo_coeff), dtype=torch.float64,
+                                device=DEV)
+        occ = np.asarray(mf.mo_occ, dtype=np.float64)
+        mask = np.ones(c_all.shape[1], dtype=bool) if mask is None else np.asarray(mask, dtype=bool)
+        act = torch.as_tensor(np.nonzero(mask)[0], device=DEV)
+        c = c_all[:, act]
+        nocc = int((occ[mask] > 0).sum())
         self.nocc = nocc
+        cc = ints.c2s @ c                                   # Cartesian AO -> active MO
         eri = torch.tensordot(ints.eri_c, cc, dims=([3], [0]))              # (p q r l)
         eri = torch.tensordot(eri, cc, dims=([2], [0])).permute(0, 1, 3, 2)  # (p q k l)
         eri = torch.tensordot(cc, eri, dims=([0], [1]))                      # (j p k l)
         eri = torch.tensordot(cc, eri, dims=([0], [1]))                      # (i j k l)
         eri = eri.contiguous()
-        dm = 2.0 * c[:, :nocc] @ c[:, :nocc].T
+        occ_t = torch.as_tensor(occ, dtype=torch.float64, device=DEV)
+        dm = (c_all * occ_t) @ c_all.T
         j, k = ints.get_jk(dm)
         fock_ao = ints.T + ints.V + j - 0.5 * k
         self.fock = c.T @ fock_ao @ c
         self.mo_energy = torch.diagonal(self.fock).clone()
+        self._set_blocks(eri, nocc)
+        del eri
+
+    def _set_blocks(self, eri, nocc):
         o, v = slice(0, nocc), slice(nocc, None)
         self.oooo = eri[o, o, o, o].contiguous()
         self.ovoo = eri[o, v, o, o].contiguous()
@@ -47,10 +80,84 @@ class ERIs:
         self.ovov = eri[o, v, o, v].contiguous()
         self.ovvv = eri[o, v, v, v].contiguous()
         self.vvvv = eri[v, v, v, v].contiguous()
-        del eri
 
     def get_ovvv(self, *slices):
         return self.ovvv[slices] if slices else self.ovvv
+
+    @classmethod
+    def from_blocks(cls, eris, nocc, nvir, vvvv=None):
+        """Device copy of a PySCF-like eris object (NumPy blocks, the reference's input,
+        ccsd_gpu/gpu_ccsd_lambda.py:73-103, 223-225): ovvv through get_ovvv (PySCF keeps
+        it pair-packed over its last two indices), vvvv full, 4-fold packed
+        (npair, npair) or taken from `vvvv` when the object holds none."""
+        self = cls.__new__(cls)
+        self.nocc = nocc
+
+        def dev(a):
+            return torch.as_tensor(np.asarray(a), dtype=torch.float64, device=DEV).contiguous()
+
+        self.fock = dev(eris.fock)
+        me = getattr(eris, "mo_energy", None)
+        self.mo_energy = dev(me) if me is not None else torch.diagonal(self.fock).clone()
+        self.oooo, self.ovoo, self.ovvo = dev(eris.oooo), dev(eris.ovoo), dev(eris.ovvo)
+        self.oovv, self.ovov = dev(eris.oovv), dev(eris.ovov)
+        get = getattr(eris, "get_ovvv", None)
+        ovvv = dev(get() if callable(get) else eris.ovvv)
+        if ovvv.dim() == 3:
+            ovvv = _unpack_pairs_last(ovvv, nvir)
+        self.ovvv = ovvv.reshape(nocc, nvir, nvir, nvir).contiguous()
+        raw = getattr(eris, "vvvv", None)
+        if raw is None:
+            if vvvv is None:
+                raise ValueError("eris holds no vvvv block and none was supplied")
+            self.vvvv = vvvv
+        else:
+            w = dev(raw)
+            if w.dim() == 2:
+                w = _unpack_pairs_last(w, nvir)                                    # [(ab), c, d]
+                w = _unpack_pairs_last(w.permute(1, 2, 0).contiguous(), nvir)      # [c, d, a, b]
+                w = w.permute(2, 3, 0, 1)
+            self.vvvv = w.reshape(nvir, nvir, nvir, nvir).contiguous()
+        return self
+
+
+def _unpack_pairs_last(x, n):
+    """(..., n(n+1)/2) lower-triangle packed (PySCF's lib.pack_tril order) -> (..., n, n)."""
+    a, b = torch.tril_indices(n, n, device=x.device)
+    out = torch.empty(x.shape[:-1] + (n, n), dtype=x.dtype, device=x.device)
+    out[..., a, b] = x
+    out[..., b, a] = x
+    return out
+
+
+def device_eris(mycc, eris=None):
+    """The MO integrals a lambda solve or gradient runs on, as device tensors. `eris` of
+    this build is used as it is; a PySCF-like eris object (NumPy blocks) is copied to
+    the device; with none, mycc.ao2mo() when mycc is this build's CCSD, else the
+    integrals are built here from mycc's orbitals and the device AO integrals (a
+    PySCF CCSD's ao2mo would run on the host)."""
+    if isinstance(eris, ERIs):
+        return eris
+    if eris is None and isinstance(mycc, CCSD):
+        return mycc.ao2mo()
+    mask = _frozen_mask_of(mycc)
+    if eris is None:
+        return ERIs(mycc._scf, mycc.mo_coeff, mask)
+    occ = np.asarray(mycc.mo_occ)[mask]
+    nocc = int((occ > 0).sum())
+    nvir = int(mask.sum()) - nocc
+    vvvv = None
+    if getattr(eris, "vvvv", None) is None:
+        vvvv = ERIs(mycc._scf, mycc.mo_coeff, mask).vvvv
+    return ERIs.from_blocks(eris, nocc, nvir, vvvv)
+
+
+def _frozen_mask_of(mycc):
+    get = getattr(mycc, "get_frozen_mask", None)
+    nmo = np.asarray(mycc.mo_coeff).shape[1] if not torch.is_tensor(mycc.mo_coeff) else mycc.mo_coeff.shape[1]
+    if callable(get):
+        return np.asarray(get(), dtype=bool)
+    return frozen_mask(nmo, getattr(mycc, "frozen", None))
 
 
 def ccsd_energy(t1, t2, eris):
@@ -214,15 +321,19 @@ def vector_to_amplitudes(v, nocc, nvir):
 
 
 class CCSD:
-    """Closed-shell CCSD with the attributes the reference's lambda/gradient code reads."""
+    """Closed-shell CCSD with the attributes the reference's lambda/gradient code reads.
+
+    `frozen` follows PySCF: None / 0, an int (that many lowest orbitals frozen) or a
+    list of frozen orbital indices. nocc and nmo count active orbitals; mo_coeff and
+    mo_occ stay those of all orbitals, as on PySCF's CCSD object."""
 
     def __init__(self, mf, frozen=None):
-        if frozen:
-            raise NotImplementedError("frozen orbitals are not supported")
         self._scf = mf
         self.mol = mf.mol
         self.mo_coeff = mf.mo_coeff
         self.mo_occ = mf.mo_occ
+        self.frozen = frozen
+        frozen_mask(np.asarray(mf.mo_coeff).shape[1], frozen)      # validates
         self.max_cycle = 100
         self.conv_tol = 1e-10
         self.conv_tol_normt = 1e-8
@@ -235,32 +346,37 @@ class CCSD:
         self.converged_lambda = False
         self._eris = None
 
+    def get_frozen_mask(self):
+        return frozen_mask(np.asarray(self.mo_coeff).shape[1], self.frozen)
+
     @property
     def nocc(self):
-        return int((self.mo_occ > 0).sum())
+        return int((np.asarray(self.mo_occ)[self.get_frozen_mask()] > 0).sum())
 
     @property
     def nmo(self):
-        return self.mo_coeff.shape[1]
-
-    def get_frozen_mask(self):
-        return np.ones(self.nmo, dtype=bool)
+        return int(self.get_frozen_mask().sum())
 
     def ao2mo(self, mo_coeff=None):
-        """MO integrals of the SCF orbitals, built once per orbital set: they are kept on
-        the SCF object (keyed by its mo_coeff object) so that every CCSD of the same
-        orbitals -- the T solve, the lambda solve, the gradient's Z-vector -- shares them."""
+        """MO integrals of the active orbitals, built once per orbital set: they are kept
+        on the SCF object (keyed by its mo_coeff object and the frozen mask) so that
+        every CCSD of the same orbitals -- the T solve, the lambda solve, the
+        gradient's Z-vector -- shares them."""
+        mask = self.get_frozen_mask()
         if mo_coeff is not None:
-            return ERIs(self._scf, mo_coeff)
+            return ERIs(self._scf, mo_coeff, mask)
         if self._eris is not None:
             return self._eris
         cache = getattr(self._scf, "_cc_eris", None)
-        if cache is not None and cache[0] is self._scf.mo_coeff:
-            self._eris = cache[1]
+        if cache is not None and cache[0] is self._scf.mo_coeff and np.array_equal(cache[1], mask):
+            self._eris = cache[2]
             return self._eris
-        eris = ERIs(self._scf, None)
+        eris = ERIs(self._scf, None, mask)
         self._eris = eris
-        self._scf._cc_eris = (self._scf.mo_coeff, eris)
+        try:
+            self._scf._cc_eris = (self._scf.mo_coeff, mask, eris)
+        except AttributeError:
+            pass
         return eris
 
     def amplitudes_to_vector(self, t1, t2):

@@ -26,17 +26,19 @@ organised differently, MI355X-first:
 per atom block, GEMV, explicit vhf1): the plain FP64 comparison path.
 """
 import time
+import warnings
 
 import numpy as np
 import torch
 
+from .cc import _frozen_mask_of
 from .gpu_ccsd_lambda import _dev
 from .hip.grad2e import add_separable, contract_int2e_ip1
 from .hip.int1e_gpu import int1e
 from .hip.int2e_ip1_gpu import compute_int2e_ip1_gpu
 from . import mfma
 from .precision import contract, contract_acc, split_memo
-from .scf import DEV
+from .scf import DEV, get_ints, grad_nuc
 
 E = torch.einsum
 
@@ -309,29 +311,43 @@ def _lib_sym_pairs(t, scale):
 # ---------------------------------------------------------------------------
 # Phase 4: Z-vector
 # ---------------------------------------------------------------------------
-def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
-    """Solve (e_a - e_i) x_ai + fvind(x)_ai = -Xvo_ai by preconditioned CG on the device.
+def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200, mo_energy=None, mo_coeff=None):
+    """Solve (e_a - e_i) x_ai + fvind(x)_ai = -Xvo_ai by preconditioned CG on the device,
+    over all occupied x virtual orbitals (frozen ones included).
 
     fvind is the reference's 2 C_v^T (J - K/2)[D] C_o with D = C_v x C_o^T + transpose
-    (gpu_ccsd_grad.py:644-651). Its vo block needs only MO integrals the CCSD
-    already holds: 4 (ai|bj) x_bj - (ab|ij) x_bj - (aj|ib) x_bj, i.e. contractions
-    over ovov and oovv (o^2 v^2 each) instead of JK builds over the nao^4 AO tensor.
-    The three pieces are summed once into one (vo, vo) matrix, so each CG step is
-    one matrix-vector product."""
-    mf = mycc._scf
-    nocc = mycc.nocc
-    c = _dev(mycc.mo_coeff)
-    e = _dev(mf.mo_energy)
-    eai = e[nocc:, None] - e[None, :nocc]
-    eris = mycc.ao2mo()
-    ovov, oovv = eris.ovov, eris.oovv
-    nvir = eai.shape[0]
-    # H[a,i,b,j] = 4 (ia|jb) - (ij|ab) - (ja|ib)
-    H = mfma.combine([(4.0, ovov.permute(1, 0, 3, 2)), (-1.0, oovv.permute(2, 0, 3, 1)),
-                      (-1.0, ovov.permute(1, 2, 3, 0))]).reshape(nvir * nocc, nvir * nocc)
+    (gpu_ccsd_grad.py:644-651). With this build's CCSD and no frozen orbitals its vo
+    block comes from MO integrals the CCSD already holds: 4 (ai|bj) x_bj - (ab|ij) x_bj
+    - (aj|ib) x_bj, i.e. contractions over ovov and oovv (o^2 v^2 each) summed once into
+    one (vo, vo) matrix, so each CG step is one matrix-vector product. Otherwise (frozen
+    orbitals, whose CCSD integrals cover the active space only, or a PySCF-like mycc)
+    each step builds J and K of D from the resident AO integrals (two GEMVs), as the
+    reference's frozen-orbital response does (:593-594, :655-660).
 
-    def fvind(x):
-        return (H @ x.reshape(-1)).reshape(nvir, nocc)
+    Sets mycc._zvector_iterations and mycc._zvector_converged; an unconverged solve
+    warns (RuntimeWarning) and returns its last iterate."""
+    from .cc import CCSD, _frozen_mask_of
+    mf = mycc._scf
+    nvir, nocc = Xvo.shape
+    c = _dev(mycc.mo_coeff if mo_coeff is None else mo_coeff)
+    e = _dev(mf.mo_energy if mo_energy is None else mo_energy)
+    eai = e[nocc:, None] - e[None, :nocc]
+    if isinstance(mycc, CCSD) and _frozen_mask_of(mycc).all():
+        eris = mycc.ao2mo()
+        ovov, oovv = eris.ovov, eris.oovv
+        # H[a,i,b,j] = 4 (ia|jb) - (ij|ab) - (ja|ib)
+        H = mfma.combine([(4.0, ovov.permute(1, 0, 3, 2)), (-1.0, oovv.permute(2, 0, 3, 1)),
+                          (-1.0, ovov.permute(1, 2, 3, 0))]).reshape(nvir * nocc, nvir * nocc)
+
+        def fvind(x):
+            return (H @ x.reshape(-1)).reshape(nvir, nocc)
+    else:
+        cv, co = c[:, nocc:], c[:, :nocc]
+
+        def fvind(x):
+            d = cv @ x @ co.T
+            j, k = ints.get_jk(d + d.T)
+            return (cv.T @ (j - 0.5 * k) @ co) * 2.0
 
     def A(x):
         return eai * x + fvind(x)
@@ -343,8 +359,13 @@ def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
     p = z.clone()
     rz = torch.sum(r * z)
     bn = float(torch.linalg.norm(b))
-    for it in range(max_cycle):
+    converged = False
+    it = 0
+    for it in range(max_cycle + 1):
         if float(torch.linalg.norm(r)) <= tol * max(bn, 1e-300):
+            converged = True
+            break
+        if it == max_cycle:
             break
         Ap = A(p)
         alpha = rz / torch.sum(p * Ap)
@@ -354,12 +375,24 @@ def _response_dm1_gpu(mycc, Xvo, ints, tol=1e-11, max_cycle=200):
         rz_new = torch.sum(r * z)
         p = z + (rz_new / rz) * p
         rz = rz_new
-    mycc._zvector_iterations = it
+    _set_attr(mycc, "_zvector_iterations", it)
+    _set_attr(mycc, "_zvector_converged", converged)
+    if not converged:
+        warnings.warn(f"Z-vector (CPHF) solve not converged in {max_cycle} iterations "
+                      f"(|r|/|b| = {float(torch.linalg.norm(r)) / max(bn, 1e-300):.2e} > {tol:.1e})",
+                      RuntimeWarning, stacklevel=2)
     nmo = c.shape[1]
     dm1 = torch.zeros((nmo, nmo), dtype=torch.float64, device=DEV)
     dm1[nocc:, :nocc] = x
     dm1[:nocc, nocc:] = x.T
     return dm1
+
+
+def _set_attr(obj, name, value):
+    try:
+        setattr(obj, name, value)
+    except AttributeError:
+        pass
 
 
 # ---------------------------------------------------------------------------
@@ -398,12 +431,21 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     l2 = _dev(mycc.l2 if l2 is None else l2)
     mol = mycc.mol
     mf = mycc._scf
-    ints = mf._ints()
+    ints = get_ints(mf)
     c2s = ints.c2s
     mo = _dev(mycc.mo_coeff)
     mo_energy = _dev(mf.mo_energy)
+    mo_occ = np.asarray(mycc.mo_occ, dtype=np.float64)
     nao, nmo = mo.shape
-    nocc = mycc.nocc
+    nocc = int((mo_occ > 0).sum())              # every occupied orbital (the response space)
+    # frozen / active orbital sets, as ccsd_gpu/gpu_ccsd_grad.py:373-377
+    mask = _frozen_mask_of(mycc)
+    with_frozen = not mask.all()
+    OA, VA = np.nonzero(mask & (mo_occ > 0))[0], np.nonzero(mask & (mo_occ == 0))[0]
+    OF, VF = np.nonzero(~mask & (mo_occ > 0))[0], np.nonzero(~mask & (mo_occ == 0))[0]
+    if t1.shape != (len(OA), len(VA)):
+        raise ValueError(f"t1 has shape {tuple(t1.shape)}, the active space is {len(OA)} x {len(VA)}")
+    mo_act = mo[:, torch.as_tensor(np.concatenate([OA, VA]), device=DEV)] if with_frozen else mo
     nc = c2s.shape[0]
     t0 = time.perf_counter()
 
@@ -414,10 +456,10 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     doo, dov, dvo, dvv = d1
     t0 = mark("rdm", t0)
 
-    # Phase 2: Cartesian AO 2-RDM (symmetrized)
+    # Phase 2: Cartesian AO 2-RDM (symmetrized) of the active orbitals
     # FP64 by default: a split-BF16 operand keeps 16 mantissa bits, which the
     # nao^4 contraction with the derivative integrals turns into ~1e-7 Eh/Bohr
-    gp = _rdm2_mo2ao_packed_gpu(mycc, d2, c2s @ mo, mo2ao_precision)
+    gp = _rdm2_mo2ao_packed_gpu(mycc, d2, c2s @ mo_act, mo2ao_precision)
     del d2
     t0 = mark("mo2ao", t0)
 
@@ -432,14 +474,26 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     S = ints.S
     Imo = -(mo.T @ imat @ S @ mo)
     dm1mo = torch.zeros((nmo, nmo), dtype=torch.float64, device=DEV)
-    dm1mo[:nocc, :nocc] = doo + doo.T
-    dm1mo[nocc:, nocc:] = dvv + dvv.T
+    if with_frozen:
+        # frozen-occupied / frozen-virtual rotations from Imat (gpu_ccsd_grad.py:510-518)
+        oa, va, of, vf = (torch.as_tensor(x, device=DEV) for x in (OA, VA, OF, VF))
+        dco = Imo[of][:, oa] / (mo_energy[of][:, None] - mo_energy[oa][None, :])
+        dfv = Imo[vf][:, va] / (mo_energy[vf][:, None] - mo_energy[va][None, :])
+        dm1mo[oa[:, None], oa] = doo + doo.T
+        dm1mo[of[:, None], oa] = dco
+        dm1mo[oa[:, None], of] = dco.T
+        dm1mo[va[:, None], va] = dvv + dvv.T
+        dm1mo[vf[:, None], va] = dfv
+        dm1mo[va[:, None], vf] = dfv.T
+    else:
+        dm1mo[:nocc, :nocc] = doo + doo.T
+        dm1mo[nocc:, nocc:] = dvv + dvv.T
     dm1 = mo @ dm1mo @ mo.T
     j, k = ints.get_jk(dm1)
     vhf = (j - 0.5 * k) * 2
     Xvo = mo[:, nocc:].T @ vhf @ mo[:, :nocc]
     Xvo = Xvo + Imo[:nocc, nocc:].T - Imo[nocc:, :nocc]
-    dm1mo = dm1mo + _response_dm1_gpu(mycc, Xvo, ints)
+    dm1mo = dm1mo + _response_dm1_gpu(mycc, Xvo, ints, mo_energy=mo_energy, mo_coeff=mo)
     Imo[nocc:, :nocc] = Imo[:nocc, nocc:].T
     im1 = mo @ Imo @ mo.T
     t0 = mark("zvector", t0)
@@ -542,12 +596,16 @@ def _grad2e_unfused(mol, gp, Dc, Pc, de2):
 
 def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None):
     """Total dE/dR (natm, 3) in Eh/Bohr (despite the name, not -dE/dR), as
-    ccsd_gpu/gpu_ccsd_grad.py:711-739: electronic gradient + nuclear repulsion."""
+    ccsd_gpu/gpu_ccsd_grad.py:711-739: electronic gradient + nuclear repulsion.
+
+    mf is this build's RHF or any converged RHF-like object exposing mol, mo_coeff,
+    mo_occ and mo_energy (a PySCF RHF); amplitudes may be NumPy arrays or tensors.
+    The nuclear term is computed here (the reference asks mf.nuc_grad_method())."""
     from .cc import CCSD
     mycc = CCSD(mf)
     mycc.t1, mycc.t2, mycc.l1, mycc.l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
     mycc.converged = True
     mycc.converged_lambda = True
     de = grad_elec_gpu(mycc, t1, t2, l1, l2, verbose=0, fused=fused, precision=precision, timings=timings)
-    de += mf.nuc_grad_method().grad_nuc()
+    de += grad_nuc(mf.mol)
     return np.array(de)

@@ -17,6 +17,9 @@ mixed path.
 Differences of behaviour, on purpose:
   * there is no CPU fallback: without a HIP device every call raises, whatever
     `fallback_to_cpu` says (the reference falls back to PySCF);
+  * eris may be this build's device ERIs or a PySCF-like object with NumPy
+    blocks (copied to the device once per solve); with none, integrals are built
+    on the device from mycc's orbitals;
   * use_cpu_vvvv=True includes the l2.vvvv term (computed on the GPU);
     use_cpu_vvvv=False omits it, as the reference does (:244-245).
 """
@@ -354,8 +357,8 @@ class GraphedUpdate:
 def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=None, tol=None, verbose=0,
                use_cpu_vvvv=True, precision="fp64", use_graph=None):
     _require_gpu()
-    if eris is None:
-        eris = mycc.ao2mo()
+    from .cc import device_eris
+    eris = device_eris(mycc, eris)
     t1 = _dev(mycc.t1 if t1 is None else t1)
     t2 = _dev(mycc.t2 if t2 is None else t2)
     l1 = t1.clone() if l1 is None else _dev(l1)
@@ -527,17 +530,35 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
 
 
 def solve_lambda_gpu(mycc, t1=None, t2=None, l1=None, l2=None, eris=None, use_cpu_vvvv=True,
-                     fallback_to_cpu=True, precision="fp64"):
-    """Solve the CCSD lambda equations on the GPU.
+                     fallback_to_cpu=True, precision="fp64", return_tensors=False):
+    """Solve the CCSD lambda equations on the GPU (ccsd_gpu/gpu_ccsd_lambda.py:411-449).
 
+    mycc is duck-typed as in the reference: this build's CCSD or a PySCF-like object
+    (t1, t2, mo_coeff, mo_occ, _scf.mo_energy, max_cycle, conv_tol_normt, diis_space,
+    level_shift; eris through ao2mo() or the `eris` argument, NumPy blocks accepted).
     Side effects as the reference: mycc.converged_lambda, mycc.l1, mycc.l2 and
-    mycc._lambda_solver_mode ("gpu" here). Returns (l1, l2) as device tensors.
+    mycc._lambda_solver_mode = "gpu-hybrid"; mycc._lambda_precision names the
+    contraction precision. Returns (l1, l2) as NumPy arrays, like the reference, or
+    as device tensors with return_tensors=True.
+
+    Without a HIP device this raises RuntimeError whatever `fallback_to_cpu` says:
+    the reference then falls back to PySCF's CPU solver (or raises when
+    fallback_to_cpu=False); this build has no CPU path.
     """
-    _require_gpu()
+    if not torch.cuda.is_available():
+        if not fallback_to_cpu:
+            raise RuntimeError("No usable HIP device for the lambda solve.")
+        raise RuntimeError("No usable HIP device for the lambda solve, and this build has no CPU fallback "
+                           "(fallback_to_cpu=True cannot be honoured).")
+    from .cc import device_eris
+    eris = device_eris(mycc, eris)
     conv, l1o, l2o = kernel_gpu(mycc, eris=eris, t1=t1, t2=t2, l1=l1, l2=l2, max_cycle=mycc.max_cycle,
                                 tol=mycc.conv_tol_normt, use_cpu_vvvv=use_cpu_vvvv, precision=precision)
     mycc.converged_lambda = conv
+    if not return_tensors:
+        l1o, l2o = l1o.cpu().numpy(), l2o.cpu().numpy()
     mycc.l1 = l1o
     mycc.l2 = l2o
-    mycc._lambda_solver_mode = "gpu" if precision == "fp64" else f"gpu-{precision}"
+    mycc._lambda_solver_mode = "gpu-hybrid"
+    mycc._lambda_precision = precision
     return mycc.l1, mycc.l2

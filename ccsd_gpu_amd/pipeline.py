@@ -51,7 +51,7 @@ def ccsd_gradient(mf, mycc, mode="mixed", precision="bf16x3", lambda_tol=1e-8):
     t0 = time.perf_counter()
     mycc.conv_tol_normt = lambda_tol
     prec = "fp64" if mode == "plain" else precision
-    l1, l2 = solve_lambda_gpu(mycc, precision=prec)
+    l1, l2 = solve_lambda_gpu(mycc, precision=prec, return_tensors=True)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     timings = {}

@@ -90,6 +90,46 @@ class AOIntegrals:
         return self._jkpp
 
 
+# AO integrals of molecules that arrive without this build's RHF object (PySCF-like
+# mf / mycc objects): built once per geometry and basis, the latest one kept
+_FOREIGN_INTS = {}
+
+
+def _mol_key(mol):
+    return (np.asarray(mol._atm).tobytes(), np.asarray(mol._bas).tobytes(), np.asarray(mol._env).tobytes(),
+            bool(mol.cart))
+
+
+def get_ints(mf=None, mol=None):
+    """Device AO integrals for an RHF-like object: its own when it is this build's RHF,
+    otherwise computed here from `mf.mol` (a PySCF-like object exposes mol, mo_coeff,
+    mo_occ, mo_energy and nothing device-side). Only the most recent foreign geometry
+    is kept (the Cartesian ERI tensor is nao_c^4 doubles)."""
+    own = getattr(mf, "_ints", None) if mf is not None else None
+    if callable(own):
+        return own()
+    mol = mf.mol if mol is None else mol
+    key = _mol_key(mol)
+    ints = _FOREIGN_INTS.get(key)
+    if ints is None:
+        _FOREIGN_INTS.clear()
+        ints = AOIntegrals(mol)
+        _FOREIGN_INTS[key] = ints
+    return ints
+
+
+def grad_nuc(mol):
+    """Nuclear-repulsion gradient dE_nn/dR (natm, 3): what PySCF's rhf gradient object's
+    grad_nuc returns (the reference adds it in compute_ml_ccsd_forces_gpu,
+    ccsd_gpu/gpu_ccsd_grad.py:736-737)."""
+    z = np.asarray(mol.atom_charges(), dtype=np.float64)
+    r = np.asarray(mol.atom_coords(), dtype=np.float64)
+    d = r[:, None, :] - r[None, :, :]
+    dist = np.linalg.norm(d, axis=-1)
+    np.fill_diagonal(dist, np.inf)
+    return -np.einsum("a,b,abx->ax", z, z, d / dist[..., None] ** 3)
+
+
 class RHF:
     def __init__(self, mol, ints=None):
         self.mol = mol
@@ -196,10 +236,4 @@ class _RHFGradHelpers:
         self.mf = mf
 
     def grad_nuc(self, mol=None):
-        mol = self.mf.mol if mol is None else mol
-        z = mol.atom_charges()
-        r = mol.atom_coords()
-        d = r[:, None, :] - r[None, :, :]
-        dist = np.linalg.norm(d, axis=-1)
-        np.fill_diagonal(dist, np.inf)
-        return -np.einsum("a,b,abx->ax", z, z, d / dist[..., None] ** 3)
+        return grad_nuc(self.mf.mol if mol is None else mol)

@@ -23,6 +23,12 @@ import numpy as np
 from . import integrals as oi
 
 
+def _einsum(eq, *ops, **kw):
+    """numpy einsum through its BLAS-backed contraction path (same sums, reordered)."""
+    kw.setdefault("optimize", True)
+    return np.einsum(eq, *ops, **kw)
+
+
 # ---------------------------------------------------------------------------
 # RHF
 # ---------------------------------------------------------------------------
@@ -48,7 +54,7 @@ class RHF:
 
     def get_jk(self, dm):
         eri = self.eri()
-        return np.einsum("ijkl,kl->ij", eri, dm), np.einsum("ikjl,kl->ij", eri, dm)
+        return _einsum("ijkl,kl->ij", eri, dm), _einsum("ikjl,kl->ij", eri, dm)
 
     def get_veff(self, mol=None, dm=None):
         j, k = self.get_jk(dm)
@@ -117,12 +123,30 @@ class RHF:
 # ---------------------------------------------------------------------------
 # MO integrals (PySCF _ChemistsERIs layout)
 # ---------------------------------------------------------------------------
+def frozen_mask(nmo, frozen):
+    """PySCF's `frozen` convention (an int: that many lowest orbitals; a list: those
+    indices) as a mask of active orbitals."""
+    mask = np.ones(nmo, dtype=bool)
+    if frozen is None:
+        return mask
+    if np.isscalar(frozen):
+        mask[:int(frozen)] = False
+    else:
+        mask[np.asarray(list(frozen), dtype=int)] = False
+    return mask
+
+
 class ERIs:
-    def __init__(self, mf, mo_coeff=None):
-        c = mf.mo_coeff if mo_coeff is None else mo_coeff
-        nocc = int((mf.mo_occ > 0).sum())
-        eri = np.einsum("pqrs,pi,qj,rk,sl->ijkl", mf.eri(), c, c, c, c, optimize=True)
-        dm = mf.make_rdm1(c, mf.mo_occ)
+    """MO integrals over the active orbitals; the Fock matrix is that of the HF density
+    of all occupied orbitals (frozen ones included) in the active-orbital basis."""
+
+    def __init__(self, mf, mo_coeff=None, mask=None):
+        c_all = mf.mo_coeff if mo_coeff is None else mo_coeff
+        mask = np.ones(c_all.shape[1], dtype=bool) if mask is None else mask
+        c = c_all[:, mask]
+        nocc = int((mf.mo_occ[mask] > 0).sum())
+        eri = _einsum("pqrs,pi,qj,rk,sl->ijkl", mf.eri(), c, c, c, c, optimize=True)
+        dm = mf.make_rdm1(c_all, mf.mo_occ)
         fock_ao = mf.get_hcore() + mf.get_veff(dm=dm)
         self.fock = c.T @ fock_ao @ c
         self.mo_energy = self.fock.diagonal().copy()
@@ -146,10 +170,10 @@ class ERIs:
 # ---------------------------------------------------------------------------
 def ccsd_energy(t1, t2, eris):
     nocc = t1.shape[0]
-    e = 2 * np.einsum("ia,ia", eris.fock[:nocc, nocc:], t1)
-    tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
-    e += 2 * np.einsum("ijab,iajb", tau, eris.ovov)
-    e -= np.einsum("ijab,ibja", tau, eris.ovov)
+    e = 2 * _einsum("ia,ia", eris.fock[:nocc, nocc:], t1)
+    tau = t2 + _einsum("ia,jb->ijab", t1, t1)
+    e += 2 * _einsum("ijab,iajb", tau, eris.ovov)
+    e -= _einsum("ijab,ibja", tau, eris.ovov)
     return e
 
 
@@ -163,71 +187,71 @@ def ccsd_update_amps(t1, t2, eris):
     fvv = fock[nocc:, nocc:]
     ovov, ovoo, ovvo, oovv, ovvv = eris.ovov, eris.ovoo, eris.ovvo, eris.oovv, eris.ovvv
 
-    Foo = foo + 2 * np.einsum("kcld,ilcd->ki", ovov, t2) - np.einsum("kdlc,ilcd->ki", ovov, t2) \
-        + 2 * np.einsum("kcld,ic,ld->ki", ovov, t1, t1) - np.einsum("kdlc,ic,ld->ki", ovov, t1, t1)
-    Fvv = fvv - 2 * np.einsum("kcld,klad->ac", ovov, t2) + np.einsum("kdlc,klad->ac", ovov, t2) \
-        - 2 * np.einsum("kcld,ka,ld->ac", ovov, t1, t1) + np.einsum("kdlc,ka,ld->ac", ovov, t1, t1)
-    Fov = fov + 2 * np.einsum("kcld,ld->kc", ovov, t1) - np.einsum("kdlc,ld->kc", ovov, t1)
-    Loo = Foo + np.einsum("kc,ic->ki", fov, t1) + 2 * np.einsum("lcki,lc->ki", ovoo, t1) \
-        - np.einsum("kcli,lc->ki", ovoo, t1)
-    Lvv = Fvv - np.einsum("kc,ka->ac", fov, t1) + 2 * np.einsum("kdac,kd->ac", ovvv, t1) \
-        - np.einsum("kcad,kd->ac", ovvv, t1)
+    Foo = foo + 2 * _einsum("kcld,ilcd->ki", ovov, t2) - _einsum("kdlc,ilcd->ki", ovov, t2) \
+        + 2 * _einsum("kcld,ic,ld->ki", ovov, t1, t1) - _einsum("kdlc,ic,ld->ki", ovov, t1, t1)
+    Fvv = fvv - 2 * _einsum("kcld,klad->ac", ovov, t2) + _einsum("kdlc,klad->ac", ovov, t2) \
+        - 2 * _einsum("kcld,ka,ld->ac", ovov, t1, t1) + _einsum("kdlc,ka,ld->ac", ovov, t1, t1)
+    Fov = fov + 2 * _einsum("kcld,ld->kc", ovov, t1) - _einsum("kdlc,ld->kc", ovov, t1)
+    Loo = Foo + _einsum("kc,ic->ki", fov, t1) + 2 * _einsum("lcki,lc->ki", ovoo, t1) \
+        - _einsum("kcli,lc->ki", ovoo, t1)
+    Lvv = Fvv - _einsum("kc,ka->ac", fov, t1) + 2 * _einsum("kdac,kd->ac", ovvv, t1) \
+        - _einsum("kcad,kd->ac", ovvv, t1)
     Foo = Foo - np.diag(e_o)
     Fvv = Fvv - np.diag(e_v)
     Loo = Loo - np.diag(e_o)
     Lvv = Lvv - np.diag(e_v)
 
-    t1n = -2 * np.einsum("kc,ka,ic->ia", fov, t1, t1)
-    t1n += np.einsum("ac,ic->ia", Fvv, t1)
-    t1n -= np.einsum("ki,ka->ia", Foo, t1)
-    t1n += 2 * np.einsum("kc,kica->ia", Fov, t2)
-    t1n -= np.einsum("kc,ikca->ia", Fov, t2)
-    t1n += np.einsum("kc,ic,ka->ia", Fov, t1, t1)
+    t1n = -2 * _einsum("kc,ka,ic->ia", fov, t1, t1)
+    t1n += _einsum("ac,ic->ia", Fvv, t1)
+    t1n -= _einsum("ki,ka->ia", Foo, t1)
+    t1n += 2 * _einsum("kc,kica->ia", Fov, t2)
+    t1n -= _einsum("kc,ikca->ia", Fov, t2)
+    t1n += _einsum("kc,ic,ka->ia", Fov, t1, t1)
     t1n += fov
-    t1n += 2 * np.einsum("kcai,kc->ia", ovvo, t1)
-    t1n -= np.einsum("kiac,kc->ia", oovv, t1)
-    t1n += 2 * np.einsum("kdac,ikcd->ia", ovvv, t2)
-    t1n -= np.einsum("kcad,ikcd->ia", ovvv, t2)
-    t1n += 2 * np.einsum("kdac,kd,ic->ia", ovvv, t1, t1)
-    t1n -= np.einsum("kcad,kd,ic->ia", ovvv, t1, t1)
-    t1n -= 2 * np.einsum("lcki,klac->ia", ovoo, t2)
-    t1n += np.einsum("kcli,klac->ia", ovoo, t2)
-    t1n -= 2 * np.einsum("lcki,lc,ka->ia", ovoo, t1, t1)
-    t1n += np.einsum("kcli,lc,ka->ia", ovoo, t1, t1)
+    t1n += 2 * _einsum("kcai,kc->ia", ovvo, t1)
+    t1n -= _einsum("kiac,kc->ia", oovv, t1)
+    t1n += 2 * _einsum("kdac,ikcd->ia", ovvv, t2)
+    t1n -= _einsum("kcad,ikcd->ia", ovvv, t2)
+    t1n += 2 * _einsum("kdac,kd,ic->ia", ovvv, t1, t1)
+    t1n -= _einsum("kcad,kd,ic->ia", ovvv, t1, t1)
+    t1n -= 2 * _einsum("lcki,klac->ia", ovoo, t2)
+    t1n += _einsum("kcli,klac->ia", ovoo, t2)
+    t1n -= 2 * _einsum("lcki,lc,ka->ia", ovoo, t1, t1)
+    t1n += _einsum("kcli,lc,ka->ia", ovoo, t1, t1)
 
-    tmp2 = np.einsum("kibc,ka->abic", oovv, -t1) + ovvv.transpose(1, 3, 0, 2)
-    tmp = np.einsum("abic,jc->ijab", tmp2, t1)
+    tmp2 = _einsum("kibc,ka->abic", oovv, -t1) + ovvv.transpose(1, 3, 0, 2)
+    tmp = _einsum("abic,jc->ijab", tmp2, t1)
     t2n = tmp + tmp.transpose(1, 0, 3, 2)
-    tmp2 = np.einsum("kcai,jc->akij", ovvo, t1) + ovoo.transpose(1, 3, 0, 2)
-    tmp = np.einsum("akij,kb->ijab", tmp2, t1)
+    tmp2 = _einsum("kcai,jc->akij", ovvo, t1) + ovoo.transpose(1, 3, 0, 2)
+    tmp = _einsum("akij,kb->ijab", tmp2, t1)
     t2n -= tmp + tmp.transpose(1, 0, 3, 2)
     t2n += ovov.transpose(0, 2, 1, 3)
 
-    Woooo = np.einsum("lcki,jc->klij", ovoo, t1) + np.einsum("kclj,ic->klij", ovoo, t1) \
-        + np.einsum("kcld,ijcd->klij", ovov, t2) + np.einsum("kcld,ic,jd->klij", ovov, t1, t1) \
+    Woooo = _einsum("lcki,jc->klij", ovoo, t1) + _einsum("kclj,ic->klij", ovoo, t1) \
+        + _einsum("kcld,ijcd->klij", ovov, t2) + _einsum("kcld,ic,jd->klij", ovov, t1, t1) \
         + eris.oooo.transpose(0, 2, 1, 3)
-    Wvvvv = -np.einsum("kdac,kb->abcd", ovvv, t1) - np.einsum("kcbd,ka->abcd", ovvv, t1) \
+    Wvvvv = -_einsum("kdac,kb->abcd", ovvv, t1) - _einsum("kcbd,ka->abcd", ovvv, t1) \
         + eris.vvvv.transpose(0, 2, 1, 3)
-    Wvoov = np.einsum("kcad,id->akic", ovvv, t1) - np.einsum("kcli,la->akic", ovoo, t1) \
-        + ovvo.transpose(2, 0, 3, 1) - 0.5 * np.einsum("ldkc,ilda->akic", ovov, t2) \
-        - 0.5 * np.einsum("lckd,ilad->akic", ovov, t2) - np.einsum("ldkc,id,la->akic", ovov, t1, t1) \
-        + np.einsum("ldkc,ilad->akic", ovov, t2)
-    Wvovo = np.einsum("kdac,id->akci", ovvv, t1) - np.einsum("lcki,la->akci", ovoo, t1) \
-        + oovv.transpose(2, 0, 3, 1) - 0.5 * np.einsum("lckd,ilda->akci", ovov, t2) \
-        - np.einsum("lckd,id,la->akci", ovov, t1, t1)
+    Wvoov = _einsum("kcad,id->akic", ovvv, t1) - _einsum("kcli,la->akic", ovoo, t1) \
+        + ovvo.transpose(2, 0, 3, 1) - 0.5 * _einsum("ldkc,ilda->akic", ovov, t2) \
+        - 0.5 * _einsum("lckd,ilad->akic", ovov, t2) - _einsum("ldkc,id,la->akic", ovov, t1, t1) \
+        + _einsum("ldkc,ilad->akic", ovov, t2)
+    Wvovo = _einsum("kdac,id->akci", ovvv, t1) - _einsum("lcki,la->akci", ovoo, t1) \
+        + oovv.transpose(2, 0, 3, 1) - 0.5 * _einsum("lckd,ilda->akci", ovov, t2) \
+        - _einsum("lckd,id,la->akci", ovov, t1, t1)
 
-    tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
-    t2n += np.einsum("klij,klab->ijab", Woooo, tau)
-    t2n += np.einsum("abcd,ijcd->ijab", Wvvvv, tau)
-    tmp = np.einsum("ac,ijcb->ijab", Lvv, t2)
+    tau = t2 + _einsum("ia,jb->ijab", t1, t1)
+    t2n += _einsum("klij,klab->ijab", Woooo, tau)
+    t2n += _einsum("abcd,ijcd->ijab", Wvvvv, tau)
+    tmp = _einsum("ac,ijcb->ijab", Lvv, t2)
     t2n += tmp + tmp.transpose(1, 0, 3, 2)
-    tmp = np.einsum("ki,kjab->ijab", Loo, t2)
+    tmp = _einsum("ki,kjab->ijab", Loo, t2)
     t2n -= tmp + tmp.transpose(1, 0, 3, 2)
-    tmp = 2 * np.einsum("akic,kjcb->ijab", Wvoov, t2) - np.einsum("akci,kjcb->ijab", Wvovo, t2)
+    tmp = 2 * _einsum("akic,kjcb->ijab", Wvoov, t2) - _einsum("akci,kjcb->ijab", Wvovo, t2)
     t2n += tmp + tmp.transpose(1, 0, 3, 2)
-    tmp = np.einsum("akic,kjbc->ijab", Wvoov, t2)
+    tmp = _einsum("akic,kjbc->ijab", Wvoov, t2)
     t2n -= tmp + tmp.transpose(1, 0, 3, 2)
-    tmp = np.einsum("bkci,kjac->ijab", Wvovo, t2)
+    tmp = _einsum("bkci,kjac->ijab", Wvovo, t2)
     t2n -= tmp + tmp.transpose(1, 0, 3, 2)
 
     eia = e_o[:, None] - e_v
@@ -275,13 +299,15 @@ def _unpack(v, nocc, nvir):
 
 
 class CCSD:
-    """Closed-shell CCSD (no frozen orbitals) with the attributes the reference's code reads."""
+    """Closed-shell CCSD with the attributes the reference's code reads; `frozen`
+    follows PySCF (int or list of frozen orbital indices)."""
 
-    def __init__(self, mf):
+    def __init__(self, mf, frozen=None):
         self._scf = mf
         self.mol = mf.mol
         self.mo_coeff = mf.mo_coeff
         self.mo_occ = mf.mo_occ
+        self.frozen = frozen
         self.max_cycle = 200
         self.conv_tol = 1e-11
         self.conv_tol_normt = 1e-9
@@ -290,12 +316,15 @@ class CCSD:
         self.verbose = 0
         self.t1 = self.t2 = self.l1 = self.l2 = None
 
+    def get_frozen_mask(self):
+        return frozen_mask(self.mo_coeff.shape[1], self.frozen)
+
     @property
     def nocc(self):
-        return int((self.mo_occ > 0).sum())
+        return int((self.mo_occ[self.get_frozen_mask()] > 0).sum())
 
     def ao2mo(self, mo_coeff=None):
-        return ERIs(self._scf, mo_coeff)
+        return ERIs(self._scf, mo_coeff, self.get_frozen_mask())
 
     def kernel(self, eris=None):
         eris = self.ao2mo() if eris is None else eris
@@ -336,86 +365,86 @@ def make_intermediates(t1, t2, eris):
     nocc, nvir = t1.shape
     fock = eris.fock
     foo, fov, fvv = fock[:nocc, :nocc], fock[:nocc, nocc:], fock[nocc:, nocc:]
-    w1 = fvv - np.einsum("ja,jb->ba", fov, t1)
-    w2 = foo + np.einsum("ib,jb->ij", fov, t1)
-    w3 = np.einsum("kc,jkbc->bj", fov, t2) * 2 + fov.T
-    w3 -= np.einsum("kc,kjbc->bj", fov, t2)
-    w3 += np.einsum("kc,kb,jc->bj", fov, t1, t1)
+    w1 = fvv - _einsum("ja,jb->ba", fov, t1)
+    w2 = foo + _einsum("ib,jb->ij", fov, t1)
+    w3 = _einsum("kc,jkbc->bj", fov, t2) * 2 + fov.T
+    w3 -= _einsum("kc,kjbc->bj", fov, t2)
+    w3 += _einsum("kc,kb,jc->bj", fov, t1, t1)
     w4 = fov.copy()
 
     ovvv = eris.ovvv
     vvov = ovvv.transpose(2, 3, 0, 1)
-    tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
+    tau = t2 + _einsum("ia,jb->ijab", t1, t1)
     woooo = np.zeros((nocc,) * 4)
     wvooo = np.zeros((nvir, nocc, nocc, nocc))
 
-    w1 += np.einsum("jcba,jc->ba", ovvv, t1 * 2)
-    w1 -= np.einsum("jabc,jc->ba", ovvv, t1)
+    w1 += _einsum("jcba,jc->ba", ovvv, t1 * 2)
+    w1 -= _einsum("jabc,jc->ba", ovvv, t1)
     theta = t2 * 2 - t2.transpose(1, 0, 2, 3)
-    w3 += np.einsum("jkcd,kdcb->bj", theta, ovvv)
-    wVOov = np.einsum("jbcd,kd->bjkc", ovvv, t1)
-    wvOOv = np.einsum("cbjd,kd->cjkb", vvov, -t1)
+    w3 += _einsum("jkcd,kdcb->bj", theta, ovvv)
+    wVOov = _einsum("jbcd,kd->bjkc", ovvv, t1)
+    wvOOv = _einsum("cbjd,kd->cjkb", vvov, -t1)
     g2vovv = vvov.transpose(0, 2, 1, 3) * 2 - vvov.transpose(0, 2, 3, 1)
-    wvooo += np.einsum("cibd,jkbd->ckij", g2vovv, tau)
-    wvvov = np.einsum("jabd,jkcd->abkc", ovvv, t2) * -1.5
+    wvooo += _einsum("cibd,jkbd->ckij", g2vovv, tau)
+    wvvov = _einsum("jabd,jkcd->abkc", ovvv, t2) * -1.5
     wvvov += vvov.transpose(0, 3, 2, 1) * 2
     wvvov -= vvov
     g2vvov = vvov * 2 - ovvv.transpose(1, 2, 0, 3)
     theta = t2 * 2 - t2.transpose(0, 1, 3, 2)
-    vackb = np.einsum("acjd,kjbd->ackb", g2vvov, theta)
+    vackb = _einsum("acjd,kjbd->ackb", g2vvov, theta)
     wvvov += vackb.transpose(0, 3, 2, 1)
     wvvov -= vackb * 0.5
 
     ovoo = eris.ovoo
-    w2 += np.einsum("kbij,kb->ij", ovoo, t1) * 2
-    w2 -= np.einsum("ibkj,kb->ij", ovoo, t1)
+    w2 += _einsum("kbij,kb->ij", ovoo, t1) * 2
+    w2 -= _einsum("ibkj,kb->ij", ovoo, t1)
     theta = t2.transpose(1, 0, 2, 3) * 2 - t2
-    w3 -= np.einsum("lckj,klcb->bj", ovoo, theta)
-    tmp = np.einsum("lc,jcik->ijkl", t1, ovoo)
+    w3 -= _einsum("lckj,klcb->bj", ovoo, theta)
+    tmp = _einsum("lc,jcik->ijkl", t1, ovoo)
     woooo += tmp
     woooo += tmp.transpose(1, 0, 3, 2)
-    wvOOv += np.einsum("lbjk,lc->bjkc", ovoo, t1)
-    wVOov -= np.einsum("jbkl,lc->bjkc", ovoo, t1)
+    wvOOv += _einsum("lbjk,lc->bjkc", ovoo, t1)
+    wVOov -= _einsum("jbkl,lc->bjkc", ovoo, t1)
     wvooo += ovoo.transpose(1, 3, 2, 0) * 2
     wvooo -= ovoo.transpose(1, 0, 2, 3)
-    wvooo -= np.einsum("klbc,iblj->ckij", t2, ovoo * 1.5)
+    wvooo -= _einsum("klbc,iblj->ckij", t2, ovoo * 1.5)
     g2ovoo = ovoo * 2 - ovoo.transpose(2, 1, 0, 3)
     theta = t2 * 2 - t2.transpose(1, 0, 2, 3)
-    vcjik = np.einsum("jlcb,lbki->cjki", theta, g2ovoo)
+    vcjik = _einsum("jlcb,lbki->cjki", theta, g2ovoo)
     wvooo += vcjik.transpose(0, 3, 2, 1)
     wvooo -= vcjik * 0.5
 
     voov = eris.ovvo.transpose(1, 0, 3, 2)
-    woooo += np.einsum("cijd,klcd->ijkl", voov, tau)
+    woooo += _einsum("cijd,klcd->ijkl", voov, tau)
     g2voov = voov * 2 - voov.transpose(0, 2, 1, 3)
-    tmpw4 = np.einsum("ckld,ld->kc", g2voov, t1)
-    w1 -= np.einsum("ckja,kjcb->ba", g2voov, t2)
-    w1 -= np.einsum("ja,jb->ba", tmpw4, t1)
-    w2 += np.einsum("jkbc,bikc->ij", t2, g2voov)
-    w2 += np.einsum("ib,jb->ij", tmpw4, t1)
+    tmpw4 = _einsum("ckld,ld->kc", g2voov, t1)
+    w1 -= _einsum("ckja,kjcb->ba", g2voov, t2)
+    w1 -= _einsum("ja,jb->ba", tmpw4, t1)
+    w2 += _einsum("jkbc,bikc->ij", t2, g2voov)
+    w2 += _einsum("ib,jb->ij", tmpw4, t1)
     w3 += reduce(np.dot, (t1.T, tmpw4, t1.T))
     w4 += tmpw4
-    wvOOv += np.einsum("bljd,kd,lc->bjkc", voov, t1, t1)
-    wVOov -= np.einsum("bjld,kd,lc->bjkc", voov, t1, t1)
-    VOov = np.einsum("bjld,klcd->bjkc", g2voov, t2)
-    VOov -= np.einsum("bjld,kldc->bjkc", voov, t2)
+    wvOOv += _einsum("bljd,kd,lc->bjkc", voov, t1, t1)
+    wVOov -= _einsum("bjld,kd,lc->bjkc", voov, t1, t1)
+    VOov = _einsum("bjld,klcd->bjkc", g2voov, t2)
+    VOov -= _einsum("bjld,kldc->bjkc", voov, t2)
     VOov += voov
-    vOOv = np.einsum("bljd,kldc->bjkc", voov, t2)
+    vOOv = _einsum("bljd,kldc->bjkc", voov, t2)
     vOOv -= eris.oovv.transpose(2, 1, 0, 3)
     wVOov += VOov
     wvOOv += vOOv
     ov1 = vOOv * 2 + VOov
     ov2 = VOov * 2 + vOOv
-    wvooo -= np.einsum("jb,bikc->ckij", t1, ov1)
-    wvooo += np.einsum("kb,bijc->ckij", t1, ov2)
-    w3 += np.einsum("ckjb,kc->bj", ov2, t1)
-    wvvov += np.einsum("ajkc,jb->abkc", ov1, t1)
-    wvvov -= np.einsum("ajkb,jc->abkc", ov2, t1)
-    wvvov += np.einsum("laki,klbc->abic", g2ovoo, tau)
+    wvooo -= _einsum("jb,bikc->ckij", t1, ov1)
+    wvooo += _einsum("kb,bijc->ckij", t1, ov2)
+    w3 += _einsum("ckjb,kc->bj", ov2, t1)
+    wvvov += _einsum("ajkc,jb->abkc", ov1, t1)
+    wvvov -= _einsum("ajkb,jc->abkc", ov2, t1)
+    wvvov += _einsum("laki,klbc->abic", g2ovoo, tau)
 
     woooo += eris.oooo.transpose(0, 2, 1, 3)
-    w3 += np.einsum("bc,jc->bj", w1, t1)
-    w3 -= np.einsum("kj,kb->bj", w2, t1)
+    w3 += _einsum("bc,jc->bj", w1, t1)
+    w3 -= _einsum("kj,kb->bj", w2, t1)
     imds = IMDS()
     imds.w1, imds.w2, imds.w3, imds.w4 = w1, w2, w3, w4
     imds.woooo, imds.wvooo, imds.wVOov, imds.wvOOv, imds.wvvov = woooo, wvooo, wVOov, wvOOv, wvvov
@@ -428,80 +457,80 @@ def update_lambda(t1, t2, l1, l2, eris, imds):
     fov = eris.fock[:nocc, nocc:]
     e_o, e_v = eris.mo_energy[:nocc], eris.mo_energy[nocc:]
     theta = t2 * 2 - t2.transpose(0, 1, 3, 2)
-    mba = np.einsum("klca,klcb->ba", l2, theta)
-    mij = np.einsum("ikcd,jkcd->ij", l2, theta)
-    mba1 = np.einsum("jc,jb->bc", l1, t1) + mba
-    mij1 = np.einsum("kb,jb->kj", l1, t1) + mij
-    mia1 = t1 + np.einsum("kc,jkbc->jb", l1, t2) * 2
-    mia1 -= np.einsum("kc,jkcb->jb", l1, t2)
+    mba = _einsum("klca,klcb->ba", l2, theta)
+    mij = _einsum("ikcd,jkcd->ij", l2, theta)
+    mba1 = _einsum("jc,jb->bc", l1, t1) + mba
+    mij1 = _einsum("kb,jb->kj", l1, t1) + mij
+    mia1 = t1 + _einsum("kc,jkbc->jb", l1, t2) * 2
+    mia1 -= _einsum("kc,jkcb->jb", l1, t2)
     mia1 -= reduce(np.dot, (t1, l1.T, t1))
-    mia1 -= np.einsum("bd,jd->jb", mba, t1)
-    mia1 -= np.einsum("lj,lb->jb", mij, t1)
+    mia1 -= _einsum("bd,jd->jb", mba, t1)
+    mia1 -= _einsum("lj,lb->jb", mij, t1)
 
-    l2new = np.einsum("ijcd,acbd->ijab", l2, eris.vvvv)
-    l1new = np.einsum("ijab,jb->ia", l2new, t1) * 2
-    l1new -= np.einsum("jiab,jb->ia", l2new, t1)
+    l2new = _einsum("ijcd,acbd->ijab", l2, eris.vvvv)
+    l1new = _einsum("ijab,jb->ia", l2new, t1) * 2
+    l1new -= _einsum("jiab,jb->ia", l2new, t1)
     l2new *= 0.5
 
     w1 = imds.w1 - np.diag(e_v)
     w2 = imds.w2 - np.diag(e_o)
     l1new += fov
-    l1new += np.einsum("ib,ba->ia", l1, w1)
-    l1new -= np.einsum("ja,ij->ia", l1, w2)
-    l1new -= np.einsum("ik,ka->ia", mij, imds.w4)
-    l1new -= np.einsum("ca,ic->ia", mba, imds.w4)
-    l1new += np.einsum("ijab,bj->ia", l2, imds.w3) * 2
-    l1new -= np.einsum("ijba,bj->ia", l2, imds.w3)
-    l2new += np.einsum("ia,jb->ijab", l1, imds.w4)
-    l2new += np.einsum("jibc,ca->jiba", l2, w1)
-    l2new -= np.einsum("jk,kiba->jiba", w2, l2)
+    l1new += _einsum("ib,ba->ia", l1, w1)
+    l1new -= _einsum("ja,ij->ia", l1, w2)
+    l1new -= _einsum("ik,ka->ia", mij, imds.w4)
+    l1new -= _einsum("ca,ic->ia", mba, imds.w4)
+    l1new += _einsum("ijab,bj->ia", l2, imds.w3) * 2
+    l1new -= _einsum("ijba,bj->ia", l2, imds.w3)
+    l2new += _einsum("ia,jb->ijab", l1, imds.w4)
+    l2new += _einsum("jibc,ca->jiba", l2, w1)
+    l2new -= _einsum("jk,kiba->jiba", w2, l2)
 
     ovoo = eris.ovoo
-    l1new -= np.einsum("iajk,kj->ia", ovoo, mij1) * 2
-    l1new += np.einsum("jaik,kj->ia", ovoo, mij1)
-    l2new -= np.einsum("jbki,ka->jiba", ovoo, l1)
+    l1new -= _einsum("iajk,kj->ia", ovoo, mij1) * 2
+    l1new += _einsum("jaik,kj->ia", ovoo, mij1)
+    l2new -= _einsum("jbki,ka->jiba", ovoo, l1)
 
-    tau = t2 + np.einsum("ia,jb->ijab", t1, t1)
-    l2tau = np.einsum("ijcd,klcd->ijkl", l2, tau)
-    l2t1 = np.einsum("jidc,kc->ijkd", l2, t1)
+    tau = t2 + _einsum("ia,jb->ijab", t1, t1)
+    l2tau = _einsum("ijcd,klcd->ijkl", l2, tau)
+    l2t1 = _einsum("jidc,kc->ijkd", l2, t1)
 
-    l1new -= np.einsum("jb,jiab->ia", l1, eris.oovv)
+    l1new -= _einsum("jb,jiab->ia", l1, eris.oovv)
     ovvv = eris.ovvv
-    l1new += np.einsum("iabc,bc->ia", ovvv, mba1) * 2
-    l1new -= np.einsum("ibca,bc->ia", ovvv, mba1)
-    l2new += np.einsum("jbac,ic->jiba", ovvv, l1)
-    m4 = np.einsum("ijkd,kadb->ijab", l2t1, ovvv)
+    l1new += _einsum("iabc,bc->ia", ovvv, mba1) * 2
+    l1new -= _einsum("ibca,bc->ia", ovvv, mba1)
+    l2new += _einsum("jbac,ic->jiba", ovvv, l1)
+    m4 = _einsum("ijkd,kadb->ijab", l2t1, ovvv)
     l2new -= m4
-    l1new -= np.einsum("ijab,jb->ia", m4, t1) * 2
-    l1new -= np.einsum("ijab,ia->jb", m4, t1) * 2
-    l1new += np.einsum("jiab,jb->ia", m4, t1)
-    l1new += np.einsum("jiab,ia->jb", m4, t1)
+    l1new -= _einsum("ijab,jb->ia", m4, t1) * 2
+    l1new -= _einsum("ijab,ia->jb", m4, t1) * 2
+    l1new += _einsum("jiab,jb->ia", m4, t1)
+    l1new += _einsum("jiab,ia->jb", m4, t1)
 
     voov = eris.ovvo.transpose(1, 0, 3, 2)
-    l1new += np.einsum("jb,aijb->ia", l1, voov) * 2
+    l1new += _einsum("jb,aijb->ia", l1, voov) * 2
     l2new += voov.transpose(1, 2, 0, 3) * 0.5
-    l2new -= np.einsum("bjic,ca->jiba", voov, mba1)
-    l2new -= np.einsum("bjka,ik->jiba", voov, mij1)
-    l1new += np.einsum("aijb,jb->ia", voov, mia1) * 2
-    l1new -= np.einsum("bija,jb->ia", voov, mia1)
-    m4 = np.einsum("ijkl,aklb->ijab", l2tau, voov)
+    l2new -= _einsum("bjic,ca->jiba", voov, mba1)
+    l2new -= _einsum("bjka,ik->jiba", voov, mij1)
+    l1new += _einsum("aijb,jb->ia", voov, mia1) * 2
+    l1new -= _einsum("bija,jb->ia", voov, mia1)
+    m4 = _einsum("ijkl,aklb->ijab", l2tau, voov)
     l2new += m4 * 0.5
-    l1new += np.einsum("ijab,jb->ia", m4, t1) * 2
-    l1new -= np.einsum("ijba,jb->ia", m4, t1)
+    l1new += _einsum("ijab,jb->ia", m4, t1) * 2
+    l1new -= _einsum("ijba,jb->ia", m4, t1)
 
-    l1new -= np.einsum("ckij,jkca->ia", imds.wvooo, l2)
-    l1new += np.einsum("abkc,kibc->ia", imds.wvvov, l2)
+    l1new -= _einsum("ckij,jkca->ia", imds.wvooo, l2)
+    l1new += _einsum("abkc,kibc->ia", imds.wvvov, l2)
     tmp_voov = imds.wVOov * 2 + imds.wvOOv
     tmp = l2.transpose(0, 2, 1, 3) - l2.transpose(0, 3, 1, 2) * 0.5
-    l2new += np.einsum("iakc,bjkc->jiba", tmp, tmp_voov)
-    tmp = np.einsum("jkca,bikc->jiba", l2, imds.wvOOv)
+    l2new += _einsum("iakc,bjkc->jiba", tmp, tmp_voov)
+    tmp = _einsum("jkca,bikc->jiba", l2, imds.wvOOv)
     l2new += tmp
     l2new += tmp.transpose(1, 0, 2, 3) * 0.5
 
-    m3 = np.einsum("ijkl,klab->ijab", imds.woooo, l2)
+    m3 = _einsum("ijkl,klab->ijab", imds.woooo, l2)
     l2new += m3 * 0.5
-    l1new += np.einsum("ijab,jb->ia", m3, t1) * 2
-    l1new -= np.einsum("ijba,jb->ia", m3, t1)
+    l1new += _einsum("ijab,jb->ia", m3, t1) * 2
+    l1new -= _einsum("ijba,jb->ia", m3, t1)
 
     eia = e_o[:, None] - e_v
     l1new /= eia
@@ -532,79 +561,79 @@ def solve_lambda(t1, t2, eris, max_cycle=200, tol=1e-10):
 # RDMs (restates ccsd_gpu/gpu_ccsd_grad.py:101-336)
 # ---------------------------------------------------------------------------
 def gamma1_intermediates(t1, t2, l1, l2):
-    doo = -np.einsum("ja,ia->ij", t1, l1)
-    dvv = np.einsum("ia,ib->ab", t1, l1)
-    xtv = np.einsum("ie,me->im", t1, l1)
-    dvo = t1.T - np.einsum("im,ma->ai", xtv, t1)
+    doo = -_einsum("ja,ia->ij", t1, l1)
+    dvv = _einsum("ia,ib->ab", t1, l1)
+    xtv = _einsum("ie,me->im", t1, l1)
+    dvo = t1.T - _einsum("im,ma->ai", xtv, t1)
     theta = t2 * 2 - t2.transpose(0, 1, 3, 2)
-    doo -= np.einsum("jkab,ikab->ij", theta, l2)
-    dvv += np.einsum("jica,jicb->ab", theta, l2)
-    xt1 = np.einsum("mnef,inef->mi", l2, theta)
-    xt2 = np.einsum("mnaf,mnef->ea", l2, theta)
-    dvo += np.einsum("imae,me->ai", theta, l1)
-    dvo -= np.einsum("mi,ma->ai", xt1, t1)
-    dvo -= np.einsum("ie,ae->ai", t1, xt2)
+    doo -= _einsum("jkab,ikab->ij", theta, l2)
+    dvv += _einsum("jica,jicb->ab", theta, l2)
+    xt1 = _einsum("mnef,inef->mi", l2, theta)
+    xt2 = _einsum("mnaf,mnef->ea", l2, theta)
+    dvo += _einsum("imae,me->ai", theta, l1)
+    dvo -= _einsum("mi,ma->ai", xt1, t1)
+    dvo -= _einsum("ie,ae->ai", t1, xt2)
     return doo, l1.copy(), dvo, dvv
 
 
 def gamma2_intermediates(t1, t2, l1, l2):
     nocc, nvir = t1.shape
-    pvOOv = np.einsum("ikca,jkcb->aijb", l2, t2)
-    moo = np.einsum("dljd->jl", pvOOv) * 2
-    mvv = np.einsum("blld->db", pvOOv) * 2
-    gooov = np.einsum("kc,cija->jkia", t1, pvOOv)
-    pvoOV = -np.einsum("ikca,jkbc->aijb", l2, t2)
+    pvOOv = _einsum("ikca,jkcb->aijb", l2, t2)
+    moo = _einsum("dljd->jl", pvOOv) * 2
+    mvv = _einsum("blld->db", pvOOv) * 2
+    gooov = _einsum("kc,cija->jkia", t1, pvOOv)
+    pvoOV = -_einsum("ikca,jkbc->aijb", l2, t2)
     theta = t2 * 2 - t2.transpose(0, 1, 3, 2)
-    pvoOV += np.einsum("ikac,jkbc->aijb", l2, theta)
-    moo += np.einsum("dljd->jl", pvoOV)
-    mvv += np.einsum("blld->db", pvoOV)
-    gooov -= np.einsum("jc,cika->jkia", t1, pvoOV)
-    mia = np.einsum("kc,ikac->ia", l1, t2) * 2 - np.einsum("kc,ikca->ia", l1, t2)
-    mab = np.einsum("kc,kb->cb", l1, t1)
-    mij = np.einsum("kc,jc->jk", l1, t1) + moo * 0.5
-    tau = np.einsum("ia,jb->ijab", t1, t1) + t2
-    goooo = np.einsum("ijab,klab->ijkl", tau, l2) * 0.5
+    pvoOV += _einsum("ikac,jkbc->aijb", l2, theta)
+    moo += _einsum("dljd->jl", pvoOV)
+    mvv += _einsum("blld->db", pvoOV)
+    gooov -= _einsum("jc,cika->jkia", t1, pvoOV)
+    mia = _einsum("kc,ikac->ia", l1, t2) * 2 - _einsum("kc,ikca->ia", l1, t2)
+    mab = _einsum("kc,kb->cb", l1, t1)
+    mij = _einsum("kc,jc->jk", l1, t1) + moo * 0.5
+    tau = _einsum("ia,jb->ijab", t1, t1) + t2
+    goooo = _einsum("ijab,klab->ijkl", tau, l2) * 0.5
     doooo = goooo.transpose(0, 2, 1, 3) * 2 - goooo.transpose(0, 3, 1, 2)
-    gooov += np.einsum("ji,ka->jkia", -0.5 * moo, t1)
-    gooov += np.einsum("la,jkil->jkia", 2 * t1, goooo)
-    gooov -= np.einsum("ib,jkba->jkia", l1, tau)
-    gooov -= np.einsum("jkba,ib->jkia", l2, t1)
+    gooov += _einsum("ji,ka->jkia", -0.5 * moo, t1)
+    gooov += _einsum("la,jkil->jkia", 2 * t1, goooo)
+    gooov -= _einsum("ib,jkba->jkia", l1, tau)
+    gooov -= _einsum("jkba,ib->jkia", l2, t1)
     dooov = gooov.transpose(0, 2, 1, 3) * 2 - gooov.transpose(1, 2, 0, 3)
 
-    goovv = np.einsum("ia,jb->ijab", mia, t1)
-    tmp = np.einsum("ijkl,klab->ijab", goooo, tau)
-    tmp -= np.einsum("jk,ikab->ijab", mij, tau)
-    tmp -= np.einsum("cb,ijac->ijab", mab, t2)
-    tmp -= np.einsum("bd,ijad->ijab", mvv * 0.5, tau)
+    goovv = _einsum("ia,jb->ijab", mia, t1)
+    tmp = _einsum("ijkl,klab->ijab", goooo, tau)
+    tmp -= _einsum("jk,ikab->ijab", mij, tau)
+    tmp -= _einsum("cb,ijac->ijab", mab, t2)
+    tmp -= _einsum("bd,ijad->ijab", mvv * 0.5, tau)
     tmp += 0.5 * tau
     tmp += 0.5 * l2
     goovv += tmp
-    gOvvO = np.einsum("kiac,jc,kb->iabj", l2, t1, t1)
-    gOvvO += np.einsum("aijb->iabj", pvOOv)
-    govVO = np.einsum("ia,jb->iabj", l1, t1)
-    govVO -= np.einsum("ikac,jc,kb->iabj", l2, t1, t1)
-    govVO += np.einsum("aijb->iabj", pvoOV)
+    gOvvO = _einsum("kiac,jc,kb->iabj", l2, t1, t1)
+    gOvvO += _einsum("aijb->iabj", pvOOv)
+    govVO = _einsum("ia,jb->iabj", l1, t1)
+    govVO -= _einsum("ikac,jc,kb->iabj", l2, t1, t1)
+    govVO += _einsum("aijb->iabj", pvoOV)
     dovvo = 2 * govVO + gOvvO
     doovv = (-2 * gOvvO - govVO).transpose(3, 0, 1, 2)
     tau2 = tau - t2 * 0.5
-    goovv += np.einsum("dlib,jlda->ijab", pvOOv, tau2)
-    goovv -= np.einsum("dlia,jldb->ijab", pvoOV, tau2)
+    goovv += _einsum("dlib,jlda->ijab", pvOOv, tau2)
+    goovv -= _einsum("dlia,jldb->ijab", pvoOV, tau2)
     tmp = pvoOV + pvOOv * 0.5
-    goovv += np.einsum("dlia,jlbd->ijab", tmp, t2)
+    goovv += _einsum("dlia,jlbd->ijab", tmp, t2)
     dovov = goovv.transpose(0, 2, 1, 3) * 2 - goovv.transpose(1, 2, 0, 3)
 
-    gvvvv = np.einsum("ijab,ijcd->abcd", l2, t2)
-    jabc = np.einsum("ijab,ic->jabc", l2, t1)
-    gvvvv += np.einsum("jabc,jd->abcd", jabc, t1)
+    gvvvv = _einsum("ijab,ijcd->abcd", l2, t2)
+    jabc = _einsum("ijab,ic->jabc", l2, t1)
+    gvvvv += _einsum("jabc,jd->abcd", jabc, t1)
     vvv = gvvvv.transpose(0, 2, 1, 3)                 # dvvvv[i] = gvvvv[i].T(1,0,2) per slice
     dvvvv = vvv - vvv.transpose(0, 3, 2, 1) * 0.5
-    gvovv = np.einsum("adbc,id->aibc", gvvvv, -t1)
-    gvovv += np.einsum("akic,kb->aibc", pvoOV, t1)
-    gvovv -= np.einsum("akib,kc->aibc", pvOOv, t1)
-    gvovv += np.einsum("ja,jibc->aibc", l1, t2)
-    gvovv += np.einsum("ja,jb,ic->aibc", l1, t1, t1)
-    gvovv += np.einsum("ba,ic->aibc", mvv * 0.5, t1)
-    gvovv += np.einsum("ja,jibc->aibc", t1, l2)
+    gvovv = _einsum("adbc,id->aibc", gvvvv, -t1)
+    gvovv += _einsum("akic,kb->aibc", pvoOV, t1)
+    gvovv -= _einsum("akib,kc->aibc", pvOOv, t1)
+    gvovv += _einsum("ja,jibc->aibc", l1, t2)
+    gvovv += _einsum("ja,jb,ic->aibc", l1, t1, t1)
+    gvovv += _einsum("ba,ic->aibc", mvv * 0.5, t1)
+    gvovv += _einsum("ja,jibc->aibc", t1, l2)
     dovvv = gvovv.transpose(1, 3, 0, 2) * 2 - gvovv.transpose(1, 2, 0, 3)
     return dovov, dvvvv, doooo, doovv, dovvo, None, dovvv, dooov
 
@@ -670,7 +699,7 @@ def rdm2_ao(d2, mo_coeff):
     """MO->AO transform + ij/kl symmetrization (gpu_ccsd_grad.py:313-326), full (nao^4)."""
     dm2 = assemble_dm2_mo(d2).transpose(1, 0, 3, 2)
     c = mo_coeff
-    dm2 = np.einsum("pqrs,ip,jq,kr,ls->ijkl", dm2, c, c, c, c, optimize=True)
+    dm2 = _einsum("pqrs,ip,jq,kr,ls->ijkl", dm2, c, c, c, c, optimize=True)
     dm2 = dm2 + dm2.transpose(1, 0, 2, 3)
     dm2 = dm2 + dm2.transpose(0, 1, 3, 2)
     return dm2 * 0.5
@@ -712,35 +741,53 @@ def grad_elec(mycc, t1, t2, l1, l2, return_parts=False):
     mo_coeff, mo_energy, mo_occ = mycc.mo_coeff, mf.mo_energy, mycc.mo_occ
     nao, nmo = mo_coeff.shape
     nocc = int((mo_occ > 0).sum())
+    # frozen / active orbital index sets (gpu_ccsd_grad.py:373-377)
+    mask = mycc.get_frozen_mask() if hasattr(mycc, "get_frozen_mask") else np.ones(nmo, dtype=bool)
+    with_frozen = not mask.all()
+    OA = np.nonzero(mask & (mo_occ > 0))[0]
+    VA = np.nonzero(mask & (mo_occ == 0))[0]
+    OF = np.nonzero(~mask & (mo_occ > 0))[0]
+    VF = np.nonzero(~mask & (mo_occ == 0))[0]
     d1 = gamma1_intermediates(t1, t2, l1, l2)
     doo, dov, dvo, dvv = d1
     d2 = gamma2_intermediates(t1, t2, l1, l2)
-    dm2ao = rdm2_ao(d2, mo_coeff)                         # (nao^4), symmetrized
+    dm2ao = rdm2_ao(d2, mo_coeff[:, np.hstack((OA, VA))])   # (nao^4), symmetrized
     hf_dm1 = mf.make_rdm1(mo_coeff, mo_occ)
     eri = mf.eri()
     ip1 = oi.ip1_full(mol)                               # (3, i, j, k, l)
     aoa = mol.ao_atom_index()
     # Phase 3: Imat and the 2e derivative (full-tensor form of the block loop)
     # Imat[p,q] = sum_{j, k>=l} (pj|kl) G'(qj|kl), G' with halved kl diagonal
-    Imat = 0.5 * np.einsum("ijkl,qjkl->iq", eri, dm2ao)
+    Imat = 0.5 * _einsum("ijkl,qjkl->iq", eri, dm2ao)
     de = np.zeros((mol.natm, 3))
-    per_ao = np.einsum("xijkl,ijkl->xi", ip1, dm2ao)
+    per_ao = _einsum("xijkl,ijkl->xi", ip1, dm2ao)
     np.add.at(de, aoa, -per_ao.T)                          # = -2 sum_{i in A, j, k>=l} ip1 G'
     # HF separable part vhf1[A] (C1 / X2 / C3 / X4) contracted later with dm1p
     vhf1 = np.zeros((mol.natm, 3, nao, nao))
     for a in range(mol.natm):
         m = (aoa == a).astype(float)
         e = ip1 * m[None, :, None, None, None]
-        vj1 = np.einsum("xijkl,ij->xkl", e, hf_dm1)        # C1
-        vk1 = np.einsum("xijkl,il->xkj", e, hf_dm1)        # X2
-        vj2 = np.einsum("xijkl,kl->xij", e, hf_dm1)        # C3
-        vk2 = np.einsum("xijkl,jk->xil", e, hf_dm1)        # X4
+        vj1 = _einsum("xijkl,ij->xkl", e, hf_dm1)        # C1
+        vk1 = _einsum("xijkl,il->xkj", e, hf_dm1)        # X2
+        vj2 = _einsum("xijkl,kl->xij", e, hf_dm1)        # C3
+        vk2 = _einsum("xijkl,jk->xil", e, hf_dm1)        # X4
         vhf1[a] = vj1 - 0.5 * vk1 + vj2 - 0.5 * vk2
     # Phase 4
     Imat = reduce(np.dot, (mo_coeff.T, Imat, mf.get_ovlp(), mo_coeff)) * -1
     dm1mo = np.zeros((nmo, nmo))
-    dm1mo[:nocc, :nocc] = doo + doo.T
-    dm1mo[nocc:, nocc:] = dvv + dvv.T
+    if with_frozen:
+        # frozen-core / frozen-virtual rotations (gpu_ccsd_grad.py:510-518)
+        dco = Imat[OF[:, None], OA] / (mo_energy[OF, None] - mo_energy[OA])
+        dfv = Imat[VF[:, None], VA] / (mo_energy[VF, None] - mo_energy[VA])
+        dm1mo[OA[:, None], OA] = doo + doo.T
+        dm1mo[OF[:, None], OA] = dco
+        dm1mo[OA[:, None], OF] = dco.T
+        dm1mo[VA[:, None], VA] = dvv + dvv.T
+        dm1mo[VF[:, None], VA] = dfv
+        dm1mo[VA[:, None], VF] = dfv.T
+    else:
+        dm1mo[:nocc, :nocc] = doo + doo.T
+        dm1mo[nocc:, nocc:] = dvv + dvv.T
     dm1 = reduce(np.dot, (mo_coeff, dm1mo, mo_coeff.T))
     vhf = mf.get_veff(mol, dm1) * 2
     Xvo = reduce(np.dot, (mo_coeff[:, nocc:].T, vhf, mo_coeff[:, :nocc]))
@@ -777,21 +824,21 @@ def grad_elec(mycc, t1, t2, l1, l2, return_parts=False):
         vrinv = vrinv_all[k].copy()
         vrinv[:, p0:p1_] += h1[:, p0:p1_]
         h1ao = vrinv + vrinv.transpose(0, 2, 1)
-        de[k] += np.einsum("xij,ij->x", s1[:, p0:p1_], im1[p0:p1_])
-        de[k] += np.einsum("xji,ij->x", s1[:, p0:p1_], im1[:, p0:p1_])
-        de[k] += np.einsum("xij,ji->x", h1ao, dm1)
-        de[k] -= np.einsum("xij,ij->x", s1[:, p0:p1_], zeta[p0:p1_])
-        de[k] -= np.einsum("xji,ij->x", s1[:, p0:p1_], zeta[:, p0:p1_])
-        de[k] -= np.einsum("xij,ij->x", s1[:, p0:p1_], vhf_s1occ[p0:p1_]) * 2
-        de[k] -= np.einsum("xij,ij->x", vhf1[k], dm1p)
+        de[k] += _einsum("xij,ij->x", s1[:, p0:p1_], im1[p0:p1_])
+        de[k] += _einsum("xji,ij->x", s1[:, p0:p1_], im1[:, p0:p1_])
+        de[k] += _einsum("xij,ji->x", h1ao, dm1)
+        de[k] -= _einsum("xij,ij->x", s1[:, p0:p1_], zeta[p0:p1_])
+        de[k] -= _einsum("xji,ij->x", s1[:, p0:p1_], zeta[:, p0:p1_])
+        de[k] -= _einsum("xij,ij->x", s1[:, p0:p1_], vhf_s1occ[p0:p1_]) * 2
+        de[k] -= _einsum("xij,ij->x", vhf1[k], dm1p)
     if return_parts:
         return de, {"dm2ao": dm2ao, "Imat": Imat, "dm1mo": dm1mo}
     return de
 
 
-def ccsd_total_energy(mol):
+def ccsd_total_energy(mol, frozen=None):
     mf = RHF(mol)
     mf.kernel()
-    cc = CCSD(mf)
+    cc = CCSD(mf, frozen=frozen)
     cc.kernel()
     return cc.e_tot, mf, cc

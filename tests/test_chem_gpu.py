@@ -168,7 +168,7 @@ def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision):
     mol, omf, occ, (l1, l2), gmf = h2o_pair
     mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
     mycc.conv_tol_normt = 1e-10
-    g1, g2 = solve_lambda_gpu(mycc, precision=precision)
+    g1, g2 = solve_lambda_gpu(mycc, precision=precision, return_tensors=True)
     assert mycc.converged_lambda
     e1 = float(np.abs(g1.cpu().numpy() - l1).max())
     e2 = float(np.abs(g2.cpu().numpy() - l2).max())
@@ -191,7 +191,7 @@ def test_lambda_residual_update_accepts_only_true_convergence(gpu, h2o_pair, pre
     mycc.conv_tol_normt = tol
     mycc.lambda_corr_rtol = 1e-4      # a tight correction, so that the estimate is what accepts
     mycc.lambda_corr_max_cycle = 30
-    g1, g2 = solve_lambda_gpu(mycc, precision=precision)
+    g1, g2 = solve_lambda_gpu(mycc, precision=precision, return_tensors=True)
     hist = list(mycc._lambda_history)
     assert mycc.converged_lambda
     assert hist[-1][0] == "residual-update"
@@ -215,7 +215,7 @@ def test_lambda_mixed_solve_with_and_without_graph(gpu, h2o_pair, graph):
     mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
     mycc.conv_tol_normt = 1e-10
     mycc.lambda_graph = graph
-    g1, g2 = solve_lambda_gpu(mycc, precision="bf16x3")
+    g1, g2 = solve_lambda_gpu(mycc, precision="bf16x3", return_tensors=True)
     assert mycc.converged_lambda
     assert (mycc._lambda_graph_replays > 0) == graph
     assert float(np.abs(g1.cpu().numpy() - l1).max()) < 1e-8
@@ -262,7 +262,7 @@ def test_forces_entry_point_end_to_end(gpu):
     mf.kernel()
     mycc = cc.CCSD(mf)
     mycc.kernel()
-    l1, l2 = solve_lambda_gpu(mycc)
+    l1, l2 = solve_lambda_gpu(mycc, return_tensors=True)
     de = compute_ml_ccsd_forces_gpu(mf, mycc.t1, mycc.t2, l1, l2)
     e0, omf, occ = chem.ccsd_total_energy(mol)
     conv, ol1, ol2 = chem.solve_lambda(occ.t1, occ.t2, occ.ao2mo())
