@@ -28,6 +28,9 @@ from . import mfma
 
 MIXED_MIN_FLOPS = 4e9
 _STATIC = None
+# number of products issued at each low precision (read by tests to confirm that a
+# mixed-precision run really used the MFMA / FP32 path)
+LOWP_PRODUCTS = {"bf16x3": 0, "fp32": 0}
 
 
 @contextmanager
@@ -265,6 +268,7 @@ def _gemm_contract(eq, A, B, precision):
         As = _split_operand(A, a, batch, left, contr, size, "nt-a")
         Bs = _split_operand(B, b, batch, right, contr, size, "nt-b")
         Cp = mfma.gemm_nt(As, Bs)
+        LOWP_PRODUCTS["bf16x3"] += 1
     else:
         Av = _cached(A, (a, "mk", tuple(batch), tuple(left), tuple(contr)),
                      lambda: _view(A, a, batch, left, contr, size)[0])
@@ -275,8 +279,10 @@ def _gemm_contract(eq, A, B, precision):
             Af = _cached(A, (a, "mk32", tuple(batch), tuple(left), tuple(contr)), lambda: Av.float())
             Bf = _cached(B, (b, "kn32", tuple(batch), tuple(contr), tuple(right)), lambda: Bv.float())
             Cp = torch.bmm(Af, Bf).double()
+            LOWP_PRODUCTS["fp32"] += 1
         elif precision == "fp32" and flops >= MIXED_MIN_FLOPS:
             Cp = mfma.bmm(Av, Bv, "fp32")
+            LOWP_PRODUCTS["fp32"] += 1
         else:
             Cp = _dgemm(Av, Bv)
     C = Cp.reshape([size[c] for c in batch + left + right])
@@ -333,6 +339,7 @@ def contract_acc(out, eq, A, B, alpha=1.0, precision="fp64"):
         Bs = _split_operand(B2, b, batch, right, contr, size, "nt-b")
         mfma.gemm_nt_scatter(As, Bs, out, _offsets(out, oidx, batch, size), _offsets(out, oidx, left, size),
                              _offsets(out, oidx, right, size), alpha=alpha, beta=1)
+        LOWP_PRODUCTS["bf16x3"] += 1
         return out
     if (precision == "fp64" or flops < MIXED_MIN_FLOPS) and out.dtype == torch.float64 and \
             _acc_fp64_direct(out, oidx, A2, B2, a, b, size, batch, left, contr, right, alpha):
@@ -416,10 +423,12 @@ def vvvv_term(l2, vvvv, precision, scale=1.0):
         B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
         As = mfma.split(A)
         X = mfma.gemm_nt(As, B, ksplit=mfma.tuned_ksplit(As, B))[0]
+        LOWP_PRODUCTS["bf16x3"] += 1
     else:
         B = _cached(vvvv, ("vvvv-abcd-f32",),
                     lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).float().contiguous())
         X = (A[0].float() @ B).double()
+        LOWP_PRODUCTS["fp32"] += 1
     X = X.reshape(-1, v, v)
     if X.is_cuda:
         from .pairops import unpack_pairs

@@ -103,8 +103,11 @@ def test_vvvv_term_symmetric_half_matches_full(gpu):
 
 
 @pytest.mark.gpu
-def test_contract_mixed_matches_einsum(gpu):
+def test_contract_mixed_matches_einsum(gpu, monkeypatch):
+    from ccsd_gpu_amd import precision
     from ccsd_gpu_amd.precision import contract
+    monkeypatch.setattr(precision, "MIXED_MIN_FLOPS", 0.0)     # every product at low precision
+    n0 = dict(precision.LOWP_PRODUCTS)
     gen = torch.Generator(device="cuda").manual_seed(9)
     o, v = 5, 9
     t2 = torch.randn((o, o, v, v), dtype=torch.float64, device="cuda", generator=gen)
@@ -119,6 +122,7 @@ def test_contract_mixed_matches_einsum(gpu):
             got = contract(eq, *ops, precision=prec)
             err = float((got - ref).abs().max() / (ref.abs().max() + 1e-30))
             assert err < tol, (eq, prec, err)
+    assert all(precision.LOWP_PRODUCTS[p] > n0[p] for p in ("fp32", "bf16x3"))
 
 
 @pytest.fixture(scope="module")
@@ -162,9 +166,14 @@ def _gpu_cc_with_oracle_orbitals(gmf, omf, occ):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
-def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision):
-    """Mixed precision converges, through FP64-residual refinement, to the FP64 lambda."""
+def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision, monkeypatch):
+    """Mixed precision converges, through FP64-residual refinement, to the FP64 lambda;
+    every contraction of the low-precision sweeps runs at that precision (the flop
+    threshold that keeps small products in FP64 is set to 0)."""
+    from ccsd_gpu_amd import precision as P
     from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    monkeypatch.setattr(P, "MIXED_MIN_FLOPS", 0.0)
+    n0 = dict(P.LOWP_PRODUCTS)
     mol, omf, occ, (l1, l2), gmf = h2o_pair
     mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
     mycc.conv_tol_normt = 1e-10
@@ -175,16 +184,20 @@ def test_lambda_gpu_matches_oracle(gpu, h2o_pair, precision):
     print(precision, "l1", e1, "l2", e2, "iters", mycc._lambda_iterations, "fp64 updates",
           mycc._lambda_fp64_updates, flush=True)
     assert e1 < 1e-8 and e2 < 1e-8
+    if precision != "fp64":
+        assert P.LOWP_PRODUCTS[precision] - n0[precision] > 20 * mycc._lambda_iterations
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
-def test_lambda_residual_update_accepts_only_true_convergence(gpu, h2o_pair, precision):
+def test_lambda_residual_update_accepts_only_true_convergence(gpu, h2o_pair, precision, monkeypatch):
     """Convergence accepted on the updated residual r + A d - d (A d in low precision,
     no FP64 recompute): the FP64 residual of the answer, measured as the first change
     norm of an FP64 solve started from it, is below the tolerance and agrees with the
     estimate, and the answer is the oracle's lambda."""
+    from ccsd_gpu_amd import precision as P
     from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    monkeypatch.setattr(P, "MIXED_MIN_FLOPS", 0.0)     # real low-precision noise in A d
     mol, omf, occ, (l1, l2), gmf = h2o_pair
     mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
     tol = 1e-8
@@ -238,9 +251,13 @@ def test_gradient_gpu_matches_oracle(gpu, h2o_pair, fused):
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,mo2ao,tol", [("bf16x3", "fp64", 1e-8), ("fp32", "fp64", 1e-7),
                                                   ("bf16x3", "bf16x3", 1e-6), ("fp32", "fp32", 1e-6)])
-def test_gradient_mixed_precision_matches_oracle(gpu, h2o_pair, precision, mo2ao, tol):
-    """RDM contractions (and optionally the MO->AO transform) on the MFMA path."""
+def test_gradient_mixed_precision_matches_oracle(gpu, h2o_pair, precision, mo2ao, tol, monkeypatch):
+    """RDM contractions (and optionally the MO->AO transform) on the MFMA path, every
+    RDM product at that precision (flop threshold 0)."""
+    from ccsd_gpu_amd import precision as P
     from ccsd_gpu_amd.gpu_ccsd_grad import grad_elec_gpu
+    monkeypatch.setattr(P, "MIXED_MIN_FLOPS", 0.0)
+    n0 = P.LOWP_PRODUCTS[precision]
     mol, omf, occ, (l1, l2), gmf = h2o_pair
     ref = chem.grad_elec(occ, occ.t1, occ.t2, l1, l2)
     mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
@@ -248,6 +265,7 @@ def test_gradient_mixed_precision_matches_oracle(gpu, h2o_pair, precision, mo2ao
     err = np.abs(de - ref).max()
     print(precision, "mo2ao", mo2ao, "grad err", err, flush=True)
     assert err < tol
+    assert P.LOWP_PRODUCTS[precision] - n0 >= 20
 
 
 @pytest.mark.gpu
