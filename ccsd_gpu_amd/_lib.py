@@ -53,6 +53,8 @@ _SIGNATURES = {
                              c_void_p]),
     "ccsd_gemv": (c_int, [c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, c_void_p,
                           ctypes.c_longlong, c_void_p, c_double, c_int, c_void_p, c_int, c_void_p]),
+    "ccsd_gemv_noalloc": (c_int, [c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, c_void_p,
+                                  ctypes.c_longlong, c_void_p, c_double, c_int, c_int, c_void_p]),
     "ccsd_l1_terms": (c_int, [c_void_p, c_int, c_int, c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_double,
                               c_double, c_double, c_double, c_void_p, c_void_p, c_void_p]),
     "ccsd_unpack_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.c_longlong, c_double,

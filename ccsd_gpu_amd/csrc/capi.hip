@@ -9,6 +9,7 @@
 #include <array>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -91,6 +92,30 @@ int hip_fail(hipError_t e, const char* where)
         hipError_t e_ = (expr);                            \
         if (e_ != hipSuccess) return hip_fail(e_, #expr);  \
     } while (0)
+
+// Slice partials of ccsd_gemv_noalloc: a file-scope device array instead of an
+// allocation (callers under graph capture or in allocation-free loops). Each device
+// holds its own copy; its address is taken with hipGetSymbolAddress once per device,
+// on that device, and only that address is handed to kernels.
+static __device__ __align__(256) double scratch[CCSD_GEMV_SCRATCH_DOUBLES];
+std::mutex g_scratch_mu;
+double* g_scratch_dev[CCSD_MAX_DEVICES] = {};
+
+int scratch_address(double** out)
+{
+    int dev = -1;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= CCSD_MAX_DEVICES) return fail(CCSD_ERR_UNSUPPORTED, "device ordinal beyond CCSD_MAX_DEVICES");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(CCSD_ERR_HIP, "hipGetSymbolAddress returned a null scratch address");
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return CCSD_OK;
+}
 
 struct C2sTermHost {
     int n;
@@ -1473,6 +1498,35 @@ int ccsd_gemv(int cols, const double* A, long long M, long long K, long long lda
     if (K == 0) nslices = 1;   // the kernel writes beta * y (no memset: callers may capture this in a graph)
     if ((cols ? (M + 255) / 256 * nslices : (M * nslices + 3) / 4) > 0x7fffffffLL)
         return fail(CCSD_ERR_SHAPE, "GEMV too large for one launch");
+    hipError_t e = ccsd_launch_gemv(cols, A, M, K, lda, x, incx, y, alpha, beta, ws, nslices, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "gemv_kernel");
+    if (nslices > 1) {
+        e = ccsd_launch_sum_slices(ws, y, M, nslices, beta, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "sum_slices_kernel");
+    }
+    return CCSD_OK;
+}
+
+int ccsd_gemv_noalloc(int cols, const double* A, long long M, long long K, long long lda, const double* x,
+                      long long incx, double* y, double alpha, int beta, int nslices, void* stream)
+{
+    if (!A || !x || !y) return fail(CCSD_ERR_NULL, "null pointer");
+    if (M < 0 || K < 0 || (beta != 0 && beta != 1) || nslices < 1 || lda < (cols ? M : K))
+        return fail(CCSD_ERR_SHAPE, "bad GEMV arguments");
+    if (K == 0) nslices = 1;
+    if (nslices > 1 && (long long)nslices * M > CCSD_GEMV_SCRATCH_DOUBLES)
+        return fail(CCSD_ERR_SHAPE, "nslices * M = " + std::to_string((long long)nslices * M) +
+                                        " exceeds the GEMV scratch of " +
+                                        std::to_string((long long)CCSD_GEMV_SCRATCH_DOUBLES) + " doubles");
+    if ((cols ? (M + 255) / 256 * nslices : (M * nslices + 3) / 4) > 0x7fffffffLL)
+        return fail(CCSD_ERR_SHAPE, "GEMV too large for one launch");
+    double* ws = nullptr;
+    if (nslices > 1) {
+        const int rc = scratch_address(&ws);
+        if (rc != CCSD_OK) return rc;
+    }
+    // every slice partial (nslices * M) is written by the GEMV kernel before the slice
+    // sum reads it, so the scratch needs no clearing between calls
     hipError_t e = ccsd_launch_gemv(cols, A, M, K, lda, x, incx, y, alpha, beta, ws, nslices, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemv_kernel");
     if (nslices > 1) {

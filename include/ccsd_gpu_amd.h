@@ -258,6 +258,22 @@ int ccsd_combine(int ndim, const long long* sizes, double* dst, const long long*
 int ccsd_gemv(int cols, const double* A, long long M, long long K, long long lda, const double* x, long long incx,
               double* y, double alpha, int beta, double* ws, int nslices, void* stream);
 
+/* Size (doubles) of the device scratch of ccsd_gemv_noalloc (4 MiB per device) and
+ * the largest device ordinal it serves. */
+#define CCSD_GEMV_SCRATCH_DOUBLES (1 << 19)
+#define CCSD_MAX_DEVICES 64
+
+/* ccsd_gemv without a workspace argument and without any allocation: the slice
+ * partials go to a file-scope device array of CCSD_GEMV_SCRATCH_DOUBLES doubles
+ * (one per device, held until the process ends). No counterpart in the reference:
+ * for callers that cannot allocate (code under HIP graph capture, allocation-free
+ * serving loops). Same arguments and result as ccsd_gemv with the same nslices;
+ * nslices * M above CCSD_GEMV_SCRATCH_DOUBLES (with nslices > 1) is rejected with
+ * CCSD_ERR_SHAPE. Calls on one device share its scratch: calls that overlap on one
+ * device (concurrent streams or threads) race. */
+int ccsd_gemv_noalloc(int cols, const double* A, long long M, long long K, long long lda, const double* x,
+                      long long incx, double* y, double alpha, int beta, int nslices, void* stream);
+
 /* C[b] (+)= A[b] B[b]^T from split-BF16 operands (A planes [batch][M][K] with row
  * stride lda, B planes [batch][N][K] with row stride ldb, both K-contiguous;
  * strides in elements), three bf16 MFMA products per tile
@@ -269,9 +285,6 @@ int ccsd_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const
                         int K, long long lda, long long ldb, long long strideA, long long strideB, long long strideC,
                         long long ldc, int batch, int beta, void* stream);
 
-/* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
- * arrays; r, w are (count, n)). Counterpart of rys_roots_simple
- * (ccsd_gpu/cuda/int2e_ip1_module.py:192-244), n = 1..9. */
 /* ccsd_gemm_bf16x3_nt with a scattered destination: element (row, col) of batch b
  * is written to C[off_b[b] + off_r[row] + off_c[col]] (offset tables in device
  * memory, elements), as C = alpha * AB^T (beta 0) or C += alpha * AB^T (beta 1).
@@ -295,6 +308,9 @@ int ccsd_split_bf16_strided(const double* x, void* hi, void* lo, int ndim, const
 int ccsd_scatter_sum(const double* ws, int nslices, int batch, int M, int N, double* C, const long long* off_b,
                      const long long* off_r, const long long* off_c, double alpha, int beta, void* stream);
 
+/* Rys roots (u = t^2, ascending) and weights for `count` arguments T (device
+ * arrays; r, w are (count, n)). Counterpart of rys_roots_simple
+ * (ccsd_gpu/cuda/int2e_ip1_module.py:192-244), n = 1..9. */
 int ccsd_rys_roots(int n, const double* T, int count, double* r, double* w, void* stream);
 
 #ifdef __cplusplus
