@@ -3,7 +3,7 @@
     python -m ccsd_gpu_amd.build
 
 compiles ccsd_gpu_amd/csrc/*.hip with hipcc, one object per translation unit
-in parallel (the register-resident kernels once per (l_i, l_j, l_k)),
+in parallel (the register-resident fused and store kernels once per (l_i, l_j, l_k)),
 into ccsd_gpu_amd/_build/ and links ccsd_gpu_amd/libccsd_gpu_amd.so.
 """
 import os
@@ -20,6 +20,9 @@ UNITS = [(s, s.replace(".hip", ""), []) for s in
          ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip",
           "rdm_kernels.hip", "lambda_kernels.hip", "tensor_kernels.hip", "capi.hip"]]
 UNITS += [("fused_ip1_reg_tu.hip", f"fused_ip1_reg_l{i}{j}{k}",
+           [f"-DCCSD_REG_LI={i}", f"-DCCSD_REG_LJ={j}", f"-DCCSD_REG_LK={k}"])
+          for i in range(3) for j in range(3) for k in range(3)]
+UNITS += [("store_reg_tu.hip", f"store_reg_l{i}{j}{k}",
            [f"-DCCSD_REG_LI={i}", f"-DCCSD_REG_LJ={j}", f"-DCCSD_REG_LK={k}"])
           for i in range(3) for j in range(3) for k in range(3)]
 ARCH = os.environ.get("CCSD_GPU_ARCH", "gfx950")

@@ -30,6 +30,8 @@ hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double*
 size_t ccsd_c2s_term_size();
 hipError_t ccsd_launch_fill(double* p, long long n, double v, hipStream_t stream);
 int ccsd_fused_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_nckl);
+int ccsd_store_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_nckl, int deriv);
+hipError_t ccsd_launch_store_reg(const EriClassArgs& a, int wpb, hipStream_t stream);
 size_t ccsd_fused_reg_lds(int max_nij, int wpb);
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int waves_per_block, hipStream_t stream);
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
@@ -216,7 +218,7 @@ struct ccsd_plan {
     bool persistent = false;                         // fused pass as one persistent launch
     int n_cu = 0;                                    // compute units of the plan's device
     bool has_c2s = false;
-    bool fused_reg = true;                           // register kernels for the fused gradient
+    bool fused_reg = true;                           // register kernels (fused gradient and s2kl store)
     bool autotune = true;                            // time the kernel choices of each class once
     // Schwarz screening of the fused gradient
     double thr = 0.0;
@@ -797,7 +799,9 @@ int ensure_screening(ccsd_plan* p, hipStream_t stream)
 void record_launch(ccsd_plan* p, const EriClassArgs& a, const Choice& c, const ChunkList* cl)
 {
     const int gc = (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0;
-    const int parts = c.kind == 1 ? ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) : 0;
+    const int parts = c.kind != 1 ? 0
+                      : a.out ? ccsd_store_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl, a.deriv)
+                              : ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl);
     p->last_launches.push_back({a.li, a.lj, a.lk, a.ll, gc, c.kind, c.kind == 1 ? c.wpb : 1, c.chunk, parts,
                                 cl->quartets});
 }
@@ -809,18 +813,28 @@ int run_eri(ccsd_plan* p, BraSet& bs, int deriv, double* out, int i_base, long l
     for (int bc = 0; bc < (int)bs.cls.size(); bc++) {
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], deriv);
+            // register-resident store kernel (one lane per quartet, 64 ket pairs per wave)
+            // where the class has one, else the LDS kernel (one quartet per wave)
+            Choice c;
+            c.kind = (p->fused_reg &&
+                      ccsd_store_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl, deriv) > 0 &&
+                      ccsd_fused_reg_lds(a.max_nij, c.wpb) <= 65536) ? 1 : 0;
+            c.chunk = c.kind == 1 ? 64 : 1;
             ChunkList* cl = nullptr;
-            int rc = get_chunks(p, bs, bc, kc, 1, false, false, &cl);
+            int rc = get_chunks(p, bs, bc, kc, c.chunk, false, false, &cl);
             if (rc) return rc;
             a.chunks = cl->d;
             a.nchunks = cl->n;
             a.out = out;
             a.i_base = i_base;
             a.comp_stride = comp_stride;
-            Choice c;
-            c.kind = 0;
-            rc = launch_choice(a, c, stream);
-            if (rc) return rc;
+            if (c.kind == 1) {
+                hipError_t e = ccsd_launch_store_reg(a, c.wpb, stream);
+                if (e != hipSuccess) return hip_fail(e, "store_reg_kernel launch");
+            } else {
+                rc = launch_choice(a, c, stream);
+                if (rc) return rc;
+            }
             p->last_quartets += cl->quartets;
             record_launch(p, a, c, cl);
         }

@@ -449,6 +449,251 @@ hipError_t launch(const EriClassArgs& a, int wpb, hipStream_t stream)
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------
+// Store mode: the same per-lane quartet evaluation writing the integrals to the s2kl
+// tensor out[comp][i - i_base][j][k(k+1)/2 + l] (the int2e / int2e_ip1 drop-ins and
+// the unfused gradient). A lane accumulates its quartet's outputs of one part over
+// all primitive quartets and roots (NC = 3 derivative components or 1 integral per
+// Cartesian function and contraction-column combination), then writes each once:
+// no ket-primitive split, every quartet of the work list is written (zeros where
+// no primitive pair passes the Gaussian-product cut).
+constexpr int kStoreBudget = 128;
+constexpr int kStoreMaxParts = 64;
+
+// true when every part of a p-way split fits the store budget (stops at the first part
+// that does not: the compile-time evaluation stays short for the large classes)
+constexpr bool fits_store(int li, int lj, int lk, int ll, int gc, int p, int nc)
+{
+    const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
+    const int m = gc ? mcol(li) * mcol(lj) * mcol(lk) * mcol(ll) : 1;
+    if (m * ((nf + p - 1) / p) * nc > kStoreBudget) return false;
+    for (int q = 0; q < p; q++) {
+        const int f0 = nf * q / p, f1 = nf * (q + 1) / p, nfp = f1 - f0;
+        if (m * nfp * nc + g_live(li, lj, lk, ll, f0, f1) > kStoreBudget) return false;
+    }
+    return true;
+}
+
+constexpr int min_parts_store(int li, int lj, int lk, int ll, int gc, int nc)
+{
+    if (li > 2 || lj > 2 || lk > 2 || ll > 2) return 0;
+    const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
+    for (int p = 1; p <= kStoreMaxParts && p <= nf; p *= 2)
+        if (fits_store(li, lj, lk, ll, gc, p, nc)) return p;
+    return 0;
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int P, int Q, int DERIV>
+__device__ __forceinline__ void quartet_store(const EriClassArgs& a, const int4 ch, int lane, const double* tab,
+                                              int nij)
+{
+    constexpr int NFI = ncart_c(LI), NFJ = ncart_c(LJ), NFK = ncart_c(LK), NFL = ncart_c(LL);
+    constexpr int NF = NFI * NFJ * NFK * NFL;
+    constexpr int F0 = NF * Q / P, F1 = NF * (Q + 1) / P, NFP = F1 - F0;
+    constexpr int NC = DERIV ? 3 : 1;
+    constexpr int NROOTS = (LI + LJ + LK + LL + DERIV) / 2 + 1;
+    constexpr int MIJ = GC ? mcol(LI) * mcol(LJ) : 1;
+    constexpr int MKL = GC ? mcol(LK) * mcol(LL) : 1;
+    constexpr CartTab<LI> ci_t{};
+    constexpr CartTab<LJ> cj_t{};
+    constexpr CartTab<LK> ck_t{};
+    constexpr CartTab<LL> cl_t{};
+
+    const int kq = ch.y + lane;
+    if (kq >= ch.z) return;
+    const int2 bp = a.bra[ch.x];
+    const int2 kp = a.ket[kq];
+    const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+    const ShellInfo sk = a.shells[kp.x], sl = a.shells[kp.y];
+    const double* RA = a.coords + 3 * si.atom;
+    const double* RB = a.coords + 3 * sj.atom;
+    const double* RC = a.coords + 3 * sk.atom;
+    const double* RD = a.coords + 3 * sl.atom;
+    const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
+    const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
+    const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
+
+    double acc[MIJ][MKL][NFP][NC];
+#pragma unroll
+    for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+        for (int ckl = 0; ckl < MKL; ckl++)
+#pragma unroll
+            for (int t = 0; t < NFP; t++)
+#pragma unroll
+                for (int c = 0; c < NC; c++) acc[cij][ckl][t][c] = 0.0;
+
+    const int t1 = a.ket_pp_off[kq + 1];
+    for (int t = a.ket_pp_off[kq]; t < t1; t++) {
+        const double* kpd = a.ket_pp + (size_t)t * 10;
+        const double akl = kpd[0];
+        const double Qc[3] = {kpd[1], kpd[2], kpd[3]};
+        const double KcdN = kpd[4];
+        const double hkl = kpd[5];
+        const double QC[3] = {Qc[0] - RC[0], Qc[1] - RC[1], Qc[2] - RC[2]};
+        double ck[MKL];
+#pragma unroll
+        for (int ckl = 0; ckl < MKL; ckl++) ck[ckl] = kpd[6 + ckl];
+        for (int p = 0; p < nij; p++) {
+            const double* e = tab + p * kBraFields;
+            const double aij = e[0];
+            const double Pc[3] = {e[1], e[2], e[3]};
+            const double ai2 = e[5], hij = e[6];
+            const double PQ[3] = {Pc[0] - Qc[0], Pc[1] - Qc[1], Pc[2] - Qc[2]};
+            const double PA[3] = {Pc[0] - RA[0], Pc[1] - RA[1], Pc[2] - RA[2]};
+            const double rr = PQ[0] * PQ[0] + PQ[1] * PQ[1] + PQ[2] * PQ[2];
+            const double inv = 1.0 / (aij + akl);
+            const double T = aij * akl * inv * rr;
+            double pref = e[4] * KcdN * sqrt(inv);
+            if constexpr (MIJ * MKL == 1) pref *= e[7] * ck[0];
+            double cc[MIJ][MKL];
+#pragma unroll
+            for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+                for (int ckl = 0; ckl < MKL; ckl++) cc[cij][ckl] = (MIJ * MKL == 1) ? 1.0 : e[7 + cij] * ck[ckl];
+            double rt[NROOTS], wt[NROOTS];
+            ccsd::rys_roots_dev(NROOTS, T, rt, wt);
+#pragma unroll
+            for (int r = 0; r < NROOTS; r++) {
+                const double a2 = rt[r] * inv;
+                const double rt_aij = a2 * akl, rt_akl = a2 * aij;
+                const double b00 = 0.5 * a2;
+                const double b10 = hij * (1.0 - rt_aij);
+                const double b01 = hkl * (1.0 - rt_akl);
+                double Gx[LI + 2][LJ + 1][LK + 1][LL + 1];
+                double Gy[LI + 2][LJ + 1][LK + 1][LL + 1];
+                double Gz[LI + 2][LJ + 1][LK + 1][LL + 1];
+                factor1d<LI, LJ, LK, LL>(Gx, 1.0, PA[0] - rt_aij * PQ[0], QC[0] + rt_akl * PQ[0], b00, b10, b01,
+                                         AB[0], CD[0]);
+                factor1d<LI, LJ, LK, LL>(Gy, 1.0, PA[1] - rt_aij * PQ[1], QC[1] + rt_akl * PQ[1], b00, b10, b01,
+                                         AB[1], CD[1]);
+                factor1d<LI, LJ, LK, LL>(Gz, wt[r] * pref, PA[2] - rt_aij * PQ[2], QC[2] + rt_akl * PQ[2], b00,
+                                         b10, b01, AB[2], CD[2]);
+#pragma unroll
+                for (int t = 0; t < NFP; t++) {
+                    const int f = F0 + t;
+                    const int fi = f % NFI, r1 = f / NFI;
+                    const int fj = r1 % NFJ, r2 = r1 / NFJ;
+                    const int fk = r2 % NFK, fl = r2 / NFK;
+                    const int ix = ci_t.x[fi], iy = ci_t.y[fi], iz = ci_t.z[fi];
+                    const int jx = cj_t.x[fj], jy = cj_t.y[fj], jz = cj_t.z[fj];
+                    const int kx = ck_t.x[fk], ky = ck_t.y[fk], kz = ck_t.z[fk];
+                    const int lx = cl_t.x[fl], ly = cl_t.y[fl], lz = cl_t.z[fl];
+                    const double gx = Gx[ix][jx][kx][lx];
+                    const double gy = Gy[iy][jy][ky][ly];
+                    const double gz = Gz[iz][jz][kz][lz];
+                    double v[NC];
+                    if constexpr (DERIV) {
+                        double dx = -ai2 * Gx[ix + 1][jx][kx][lx];
+                        double dy = -ai2 * Gy[iy + 1][jy][ky][ly];
+                        double dz = -ai2 * Gz[iz + 1][jz][kz][lz];
+                        if (ix > 0) dx += ix * Gx[ix > 0 ? ix - 1 : 0][jx][kx][lx];
+                        if (iy > 0) dy += iy * Gy[iy > 0 ? iy - 1 : 0][jy][ky][ly];
+                        if (iz > 0) dz += iz * Gz[iz > 0 ? iz - 1 : 0][jz][kz][lz];
+                        v[0] = dx * gy * gz;
+                        v[NC > 1 ? 1 : 0] = gx * dy * gz;
+                        v[NC > 2 ? 2 : 0] = gx * gy * dz;
+                    } else {
+                        v[0] = gx * gy * gz;
+                    }
+#pragma unroll
+                    for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+                        for (int ckl = 0; ckl < MKL; ckl++)
+#pragma unroll
+                            for (int c = 0; c < NC; c++) acc[cij][ckl][t][c] += cc[cij][ckl] * v[c];
+                }
+            }
+        }
+    }
+    // write every (column combination, function) of the part; when ksh == lsh only the
+    // k >= l half of the shell pair belongs to the packed layout
+    const bool same_kl = kp.x == kp.y;
+#pragma unroll
+    for (int cij = 0; cij < MIJ; cij++)
+#pragma unroll
+        for (int ckl = 0; ckl < MKL; ckl++) {
+            if (cij >= ncij || ckl >= nckl) continue;
+            const int c_i = cij / sj.nctr, c_j = cij - c_i * sj.nctr;
+            const int c_k = ckl / sl.nctr, c_l = ckl - c_k * sl.nctr;
+#pragma unroll
+            for (int t = 0; t < NFP; t++) {
+                const int f = F0 + t;
+                const int fi = f % NFI, r1 = f / NFI;
+                const int fj = r1 % NFJ, r2 = r1 / NFJ;
+                const int fk = r2 % NFK, fl = r2 / NFK;
+                const int i = si.ao_cart + c_i * NFI + fi;
+                const int j = sj.ao_cart + c_j * NFJ + fj;
+                const int k = sk.ao_cart + c_k * NFK + fk;
+                const int l = sl.ao_cart + c_l * NFL + fl;
+                if (same_kl && l > k) continue;
+                const long long idx =
+                    ((long long)(i - a.i_base) * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l;
+#pragma unroll
+                for (int c = 0; c < NC; c++) a.out[c * a.comp_stride + idx] = acc[cij][ckl][t][c];
+            }
+        }
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int P, int DERIV, int... Q>
+__device__ __forceinline__ void run_store_parts(const EriClassArgs& a, const int4 ch, int lane, int part,
+                                                const double* tab, int nij, std::integer_sequence<int, Q...>)
+{
+    ((part == Q ? quartet_store<LI, LJ, LK, LL, GC, P, Q, DERIV>(a, ch, lane, tab, nij) : void()), ...);
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int P, int DERIV>
+__global__ void __launch_bounds__(256) store_reg_kernel(const EriClassArgs a)
+{
+    constexpr int NC = DERIV ? 3 : 1;
+    constexpr int P0 = min_parts_store(LI, LJ, LK, LL, GC, NC);
+    if constexpr (P0 == 0 || P < P0) {
+        return;
+    } else {
+        extern __shared__ double lds_tab[];
+        constexpr int MIJ = GC ? mcol(LI) * mcol(LJ) : 1;
+        const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+        const int gw = blockIdx.x * (blockDim.x >> 6) + wave;
+        const int c = gw / P;
+        const bool active = c < a.nchunks;   // every wave reaches the barrier below
+        const int part = gw - c * P;
+        const int4 ch = active ? a.chunks[c] : make_int4(0, 0, 0, 0);
+        double* tab = lds_tab + (size_t)wave * a.max_nij * kBraFields;
+        const int2 bp = a.bra[ch.x];
+        const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+        int nij = 0;
+        if (active) {
+            const double* RA = a.coords + 3 * si.atom;
+            const double* RB = a.coords + 3 * sj.atom;
+            const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
+            const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
+                                shell_norm_dev(LL);
+            nij = fill_bra<MIJ>(a, si, sj, RA, AB, norm, lane, tab);
+        }
+        __syncthreads();
+        if (!active) return;
+        run_store_parts<LI, LJ, LK, LL, GC, P, DERIV>(a, ch, lane, part, tab, nij,
+                                                      std::make_integer_sequence<int, P>{});
+    }
+}
+
+template <int LI, int LJ, int LK, int LL, int GC, int DERIV>
+hipError_t launch_store(const EriClassArgs& a, int wpb, hipStream_t stream)
+{
+    constexpr int G = (GC && has_s(LI, LJ, LK, LL)) ? 1 : 0;
+    constexpr int P0 = min_parts_store(LI, LJ, LK, LL, G, DERIV ? 3 : 1);
+    constexpr int P = P0 ? P0 : 1;
+    if (P0 == 0) return hipErrorInvalidValue;
+    const long long waves = (long long)a.nchunks * P;
+    const long long blocks = (waves + wpb - 1) / wpb;
+    if (blocks > 2147483647LL) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)wpb * a.max_nij * kBraFields;
+    if (lds > 65536) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((store_reg_kernel<LI, LJ, LK, LL, G, P, DERIV>), dim3((unsigned)blocks), dim3(64 * wpb), lds,
+                       stream, a);
+    return hipGetLastError();
+}
+
 }  // namespace ccsd_reg
 
 // Launcher of the classes with angular momenta (LI, LJ, LK, *); each triple is its own
@@ -465,6 +710,28 @@ hipError_t launch(const EriClassArgs& a, int wpb, hipStream_t stream)
             CCSD_REG_CASE(I, J, K, 0)                                                                        \
             CCSD_REG_CASE(I, J, K, 1)                                                                        \
             CCSD_REG_CASE(I, J, K, 2)                                                                        \
+            default:                                                                                         \
+                return hipErrorInvalidValue;                                                                 \
+        }                                                                                                    \
+    }
+
+// Store-mode launcher of the classes (LI, LJ, LK, *), int2e (deriv 0) or int2e_ip1 (deriv 1).
+#define CCSD_STORE_CASE(I, J, K, L)                                                                          \
+    case L:                                                                                                  \
+        if (a.deriv)                                                                                         \
+            return gc ? ccsd_reg::launch_store<I, J, K, L, 1, 1>(a, wpb, stream)                             \
+                      : ccsd_reg::launch_store<I, J, K, L, 0, 1>(a, wpb, stream);                            \
+        return gc ? ccsd_reg::launch_store<I, J, K, L, 1, 0>(a, wpb, stream)                                 \
+                  : ccsd_reg::launch_store<I, J, K, L, 0, 0>(a, wpb, stream);
+#define CCSD_STORE_TU(I, J, K, NAME)                                                                         \
+    hipError_t NAME(const EriClassArgs& a, int gc, int wpb, hipStream_t stream)                              \
+    {                                                                                                        \
+        if (a.nchunks <= 0) return hipSuccess;                                                               \
+        if (wpb < 1 || wpb > 4) return hipErrorInvalidValue;                                                 \
+        switch (a.ll) {                                                                                      \
+            CCSD_STORE_CASE(I, J, K, 0)                                                                      \
+            CCSD_STORE_CASE(I, J, K, 1)                                                                      \
+            CCSD_STORE_CASE(I, J, K, 2)                                                                      \
             default:                                                                                         \
                 return hipErrorInvalidValue;                                                                 \
         }                                                                                                    \

@@ -570,28 +570,48 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
 
 
 def _grad2e_unfused(mol, gp, Dc, Pc, de2):
-    """Reference structure on the GPU: per shell block, int2e_ip1 -> HBM, GEMV with
-    the packed 2-RDM rows, and the HF vhf1 J/K pieces contracted with dm1p
-    (ccsd_gpu/gpu_ccsd_grad.py:423-496), in the Cartesian AO basis."""
+    """Reference structure on the GPU (ccsd_gpu/gpu_ccsd_grad.py:423-496), in the
+    Cartesian AO basis: per atom, int2e_ip1 of its shell block into HBM (the
+    register-resident store kernels), then
+      * the 2-RDM term  de_A -= 2 sum_{i in A, j, k>=l} ip1 Gamma'  (one GEMV),
+      * the HF vhf1 pieces contracted with dm1p:
+          C1  sum ip1_ijkl D_ij P_kl      C3  sum ip1_ijkl P_ij D_kl
+          X2  sum ip1_ijkl D_il P_kj      X4  sum ip1_ijkl D_jk P_il
+        de_A -= C1 + C3 - (X2 + X4) / 2, from the block unpacked over kl: C1 / C3 are a
+        matrix product with the flattened densities over (k, l), X2 / X4 one over
+        (j, k) (every operand contiguous, no permuted copy).
+    D = the HF density, P = dm1p (both symmetric)."""
     mol_c = mol.copy()
     mol_c.cart = True
     loc = mol_c.ao_loc_nr()
     nc = mol_c.nao_nr()
-    k, l = torch.tril_indices(nc, nc, device=DEV)
-    for ib in range(mol.nbas):
-        i0, i1 = int(loc[ib]), int(loc[ib + 1])
-        ip1 = compute_int2e_ip1_gpu(mol_c, ib, ib + 1)                  # (3, nf, nc, npair)
-        a = mol.bas_atom(ib)
+    npair = nc * (nc + 1) // 2
+    kk, ll = torch.tril_indices(nc, nc, device=DEV)
+    pidx = torch.empty((nc, nc), dtype=torch.int64, device=DEV)
+    pidx[kk, ll] = torch.arange(npair, device=DEV)
+    pidx[ll, kk] = pidx[kk, ll]
+    dens = torch.stack([Pc.reshape(-1), Dc.reshape(-1)])           # (2, nc^2): P, D
+    offs = mol.offset_nr_by_atom()
+    for a in range(mol.natm):
+        s0, s1 = int(offs[a, 0]), int(offs[a, 1])
+        if s1 <= s0:
+            continue
+        i0, i1 = int(loc[s0]), int(loc[s1])
+        nf = i1 - i0
+        ip1 = compute_int2e_ip1_gpu(mol_c, s0, s1)                  # (3, nf, nc, npair)
         de2[a] -= 2.0 * (ip1.reshape(3, -1) @ gp[i0:i1].reshape(-1))
-        # HF part: -sum_x vhf1[x] . dm1p with i restricted to this block
-        full = torch.empty((3, i1 - i0, nc, nc, nc), dtype=torch.float64, device=DEV)
-        full[..., k, l] = ip1
-        full[..., l, k] = ip1
-        D, P = Dc, Pc
-        v = E("xijkl,ij,kl->x", full, D[i0:i1], P) + E("xijkl,ij,kl->x", full, P[i0:i1], D)
-        v -= 0.5 * E("xijkl,il,kj->x", full, D[i0:i1], P) + 0.5 * E("xijkl,il,jk->x", full, P[i0:i1], D)
-        de2[a] -= v
-        del full, ip1
+        full = ip1[..., pidx]                                        # (3, nf, nc, nc, nc)
+        del ip1
+        # C1 / C3: sum_kl e[x,i,j,k,l] {P, D}[k,l], then with {D, P}[i,j]
+        u = full.reshape(3 * nf * nc, nc * nc) @ dens.T              # (3 nf nc, 2)
+        u = u.reshape(3, nf, nc, 2)
+        c13 = (u[..., 0] * Dc[i0:i1]).sum((1, 2)) + (u[..., 1] * Pc[i0:i1]).sum((1, 2))
+        # X2 / X4: sum_jk e[x,i,j,k,l] {P, D}[j,k], then with {D, P}[i,l]
+        w = torch.matmul(dens, full.reshape(3 * nf, nc * nc, nc))   # (3 nf, 2, nc)
+        w = w.reshape(3, nf, 2, nc)
+        x24 = (w[:, :, 0] * Dc[i0:i1]).sum((1, 2)) + (w[:, :, 1] * Pc[i0:i1]).sum((1, 2))
+        de2[a] -= c13 - 0.5 * x24
+        del full, u, w
 
 
 def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None):

@@ -95,3 +95,38 @@ def test_rys_roots_gpu_matches_golden(gpu):
         np.testing.assert_allclose(r.cpu().numpy(), d["mp_r"][m][:, :n], rtol=1e-13, atol=0)
         np.testing.assert_allclose(w.cpu().numpy(), d["mp_w"][m][:, :n], rtol=1e-13, atol=0)
     del ctypes
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deriv", [0, 1])
+@pytest.mark.parametrize("make,cart", [(molecules.h2o_ccpvdz, True), (molecules.ch4_ccpvdz, False)])
+def test_register_store_kernels_equal_lds_kernel(gpu, make, cart, deriv):
+    """The register-resident store kernels (classes with l <= 2, general contractions
+    included) write the same s2kl tensor as the generic LDS kernel, every element
+    (zeros of cut quartets included: a Cartesian output starting as NaN is fully set)."""
+    from ccsd_gpu_amd import _lib
+    from ccsd_gpu_amd.hip import compute_int2e_gpu, compute_int2e_ip1_gpu
+    from ccsd_gpu_amd.hip.grad2e import last_launches
+    mol = make(cart=cart)
+    fn = compute_int2e_ip1_gpu if deriv else compute_int2e_gpu
+    plan = _lib.get_plan(mol)
+    lib = _lib.load()
+    try:
+        _lib.check(lib.ccsd_plan_set_option(plan, 0, 0), "set_option")
+        ref = fn(mol, 0, mol.nbas).cpu().numpy()
+        assert all(r["kernel"] == 0 for r in last_launches(mol))
+    finally:
+        _lib.check(lib.ccsd_plan_set_option(plan, 0, 1), "set_option")
+    got = fn(mol, 0, mol.nbas)
+    # with a Cartesian molecule the kernels write straight into `out`: every element must be set
+    out = fn(mol, 0, mol.nbas, out=torch.full_like(got, float("nan"))) if cart else None
+    launches = last_launches(mol)
+    assert any(r["kernel"] == 1 for r in launches)
+    got = got.cpu().numpy()
+    err = np.abs(got - ref).max()
+    print(make.__name__, "deriv", deriv, "register store vs LDS", err,
+          sum(r["kernel"] == 1 for r in launches), "of", len(launches), "launches on register kernels", flush=True)
+    assert err < 1e-12 * max(1.0, np.abs(ref).max())
+    if out is not None:
+        assert not torch.isnan(out).any()
+        assert np.abs(out.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
