@@ -494,6 +494,7 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
     Xvo = mo[:, nocc:].T @ vhf @ mo[:, :nocc]
     Xvo = Xvo + Imo[:nocc, nocc:].T - Imo[nocc:, :nocc]
     dm1mo = dm1mo + _response_dm1_gpu(mycc, Xvo, ints, mo_energy=mo_energy, mo_coeff=mo)
+    tm["zvector_iterations"] = getattr(mycc, "_zvector_iterations", None)
     Imo[nocc:, :nocc] = Imo[:nocc, nocc:].T
     im1 = mo @ Imo @ mo.T
     t0 = mark("zvector", t0)

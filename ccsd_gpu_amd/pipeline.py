@@ -23,10 +23,16 @@ from .gpu_ccsd_lambda import solve_lambda_gpu
 
 
 def setup(mol, conv_tol_normt=1e-8, log=None):
-    """RHF and CCSD on the GPU; `log` (callable) receives per-phase wall times."""
+    """RHF and CCSD on the GPU; `log` (callable) receives per-phase wall times, which
+    are also kept in mf._setup_times (the AO integrals -- Cartesian ERIs computed once
+    and reused by the SCF, the MO integrals and the gradient's Imat / J / K --, RHF,
+    MO integrals, CCSD)."""
+    times = {}
+
     def tick(name, t0):
         torch.cuda.synchronize()
         t1 = time.perf_counter()
+        times[name] = t1 - t0
         if log:
             log(f"setup {name}: {t1 - t0:.2f} s")
         return t1
@@ -42,7 +48,9 @@ def setup(mol, conv_tol_normt=1e-8, log=None):
     eris = mycc.ao2mo()
     t0 = tick("MO integrals", t0)
     mycc.kernel(eris)
-    tick(f"CCSD ({getattr(mycc, '_iterations', '?')} iterations)", t0)
+    tick("CCSD", t0)
+    times["CCSD iterations"] = getattr(mycc, "_iterations", None)
+    mf._setup_times = times
     return mf, mycc
 
 
@@ -59,11 +67,13 @@ def ccsd_gradient(mf, mycc, mode="mixed", precision="bf16x3", lambda_tol=1e-8):
                                     precision=prec, timings=timings)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    zit = timings.pop("zvector_iterations", None)
     info = {"lambda_s": t1 - t0, "grad_s": t2 - t1, "total_s": t2 - t0,
             "lambda_iterations": getattr(mycc, "_lambda_iterations", None),
             "lambda_fp64_updates": getattr(mycc, "_lambda_fp64_updates", None),
             "lambda_converged": bool(mycc.converged_lambda),
             "lambda_graph_replays": getattr(mycc, "_lambda_graph_replays", 0) if mode != "plain" else 0,
             "lambda_phases_s": {k: round(v, 4) for k, v in getattr(mycc, "_lambda_phases", {}).items()},
+            "zvector_iterations": zit,
             "phases_s": timings}
     return np.asarray(de), info

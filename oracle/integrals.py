@@ -34,6 +34,8 @@ def lib():
         L.oracle_rys_roots.restype = None
         L.oracle_rys_roots.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_num_threads.restype = None
+        L.oracle_set_num_threads.argtypes = [ctypes.c_int]
         L.oracle_int1e_cart.restype = ctypes.c_int
         L.oracle_int1e_cart.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
                                         ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
@@ -50,6 +52,10 @@ def rys_roots(n, T):
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def set_num_threads(n):
+    lib().oracle_set_num_threads(int(n))
 
 
 def int2e_s2kl_cart(mol, b0, b1, deriv):

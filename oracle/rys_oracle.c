@@ -348,3 +348,13 @@ int oracle_num_threads(void)
     return 1;
 #endif
 }
+
+/* Threads of the following oracle calls (the CPU baseline's single-thread sample). */
+void oracle_set_num_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
