@@ -572,15 +572,16 @@ def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose
 
 def _grad2e_unfused(mol, gp, Dc, Pc, de2):
     """Reference structure on the GPU (ccsd_gpu/gpu_ccsd_grad.py:423-496), in the
-    Cartesian AO basis: per atom, int2e_ip1 of its shell block into HBM (the
-    register-resident store kernels), then
+    Cartesian AO basis: int2e_ip1 into HBM (the register-resident store kernels; with
+    288 GB the block is the whole molecule, the reference's memory-sized shell blocks
+    collapse into one), then per atom A:
       * the 2-RDM term  de_A -= 2 sum_{i in A, j, k>=l} ip1 Gamma'  (one GEMV),
       * the HF vhf1 pieces contracted with dm1p:
           C1  sum ip1_ijkl D_ij P_kl      C3  sum ip1_ijkl P_ij D_kl
           X2  sum ip1_ijkl D_il P_kj      X4  sum ip1_ijkl D_jk P_il
-        de_A -= C1 + C3 - (X2 + X4) / 2, from the block unpacked over kl: C1 / C3 are a
-        matrix product with the flattened densities over (k, l), X2 / X4 one over
-        (j, k) (every operand contiguous, no permuted copy).
+        de_A -= C1 + C3 - (X2 + X4) / 2. C1 / C3 are GEMVs over the packed k >= l pairs
+        with the pair-weighted densities; X2 / X4 GEMVs over (j, k) of the block
+        gathered as [x][i][l][j][k].
     D = the HF density, P = dm1p (both symmetric)."""
     mol_c = mol.copy()
     mol_c.cart = True
@@ -588,10 +589,16 @@ def _grad2e_unfused(mol, gp, Dc, Pc, de2):
     nc = mol_c.nao_nr()
     npair = nc * (nc + 1) // 2
     kk, ll = torch.tril_indices(nc, nc, device=DEV)
+    wdiag = torch.where(kk == ll, 1.0, 2.0).to(torch.float64)
+    Pw, Dw = Pc[kk, ll] * wdiag, Dc[kk, ll] * wdiag                 # sum over all kl of symmetric pairs
     pidx = torch.empty((nc, nc), dtype=torch.int64, device=DEV)
     pidx[kk, ll] = torch.arange(npair, device=DEV)
     pidx[ll, kk] = pidx[kk, ll]
-    dens = torch.stack([Pc.reshape(-1), Dc.reshape(-1)])           # (2, nc^2): P, D
+    ar = torch.arange(nc, device=DEV)
+    J = ar[None, :, None].expand(nc, nc, nc)                         # [l][j][k] -> j
+    Q = pidx.T[:, None, :].expand(nc, nc, nc)                        # [l][j][k] -> pair(k, l)
+    Pf, Df = Pc.reshape(-1), Dc.reshape(-1)
+    ip1 = compute_int2e_ip1_gpu(mol_c, 0, mol.nbas)                  # (3, nc, nc, npair)
     offs = mol.offset_nr_by_atom()
     for a in range(mol.natm):
         s0, s1 = int(offs[a, 0]), int(offs[a, 1])
@@ -599,20 +606,24 @@ def _grad2e_unfused(mol, gp, Dc, Pc, de2):
             continue
         i0, i1 = int(loc[s0]), int(loc[s1])
         nf = i1 - i0
-        ip1 = compute_int2e_ip1_gpu(mol_c, s0, s1)                  # (3, nf, nc, npair)
-        de2[a] -= 2.0 * (ip1.reshape(3, -1) @ gp[i0:i1].reshape(-1))
-        full = ip1[..., pidx]                                        # (3, nf, nc, nc, nc)
-        del ip1
-        # C1 / C3: sum_kl e[x,i,j,k,l] {P, D}[k,l], then with {D, P}[i,j]
-        u = full.reshape(3 * nf * nc, nc * nc) @ dens.T              # (3 nf nc, 2)
-        u = u.reshape(3, nf, nc, 2)
-        c13 = (u[..., 0] * Dc[i0:i1]).sum((1, 2)) + (u[..., 1] * Pc[i0:i1]).sum((1, 2))
-        # X2 / X4: sum_jk e[x,i,j,k,l] {P, D}[j,k], then with {D, P}[i,l]
-        w = torch.matmul(dens, full.reshape(3 * nf, nc * nc, nc))   # (3 nf, 2, nc)
-        w = w.reshape(3, nf, 2, nc)
-        x24 = (w[:, :, 0] * Dc[i0:i1]).sum((1, 2)) + (w[:, :, 1] * Pc[i0:i1]).sum((1, 2))
+        blk = ip1[:, i0:i1]                                            # (3, nf, nc, npair); blk[x] contiguous
+        g = gp[i0:i1].reshape(-1)
+        u0 = torch.empty((3, nf * nc), dtype=torch.float64, device=DEV)
+        u1 = torch.empty_like(u0)
+        for x in range(3):
+            rows = blk[x].reshape(nf * nc, npair)
+            de2[a, x] -= 2.0 * torch.dot(rows.reshape(-1), g)
+            mfma.gemv(rows, Pw, u0[x])
+            mfma.gemv(rows, Dw, u1[x])
+        u0, u1 = u0.view(3, nf, nc), u1.view(3, nf, nc)
+        c13 = (u0 * Dc[i0:i1]).sum((1, 2)) + (u1 * Pc[i0:i1]).sum((1, 2))
+        f2 = blk[:, :, J, Q].reshape(3 * nf * nc, nc * nc)             # [x][i][l][(j, k)]
+        w0 = mfma.gemv(f2, Pf, torch.empty(3 * nf * nc, dtype=torch.float64, device=DEV)).view(3, nf, nc)
+        w1 = mfma.gemv(f2, Df, torch.empty(3 * nf * nc, dtype=torch.float64, device=DEV)).view(3, nf, nc)
+        x24 = (w0 * Dc[i0:i1]).sum((1, 2)) + (w1 * Pc[i0:i1]).sum((1, 2))
         de2[a] -= c13 - 0.5 * x24
-        del full, u, w
+        del f2
+    del ip1
 
 
 def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None):
