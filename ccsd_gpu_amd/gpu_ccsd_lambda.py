@@ -31,7 +31,7 @@ import torch
 from . import mfma
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
 from .pairops import l1_terms, pair_combine
-from .precision import contract, contract_acc, split_memo, static_operands, vvvv_term
+from .precision import contract, contract_acc, refresh_static, split_memo, static_operands, vvvv_term
 from .scf import DEV
 
 
@@ -299,15 +299,6 @@ def _wvvov_l2_fp64(imds, t2, l2, ovvv, l2tau):
     return out + C("lakm,mikl->ia", g2ovoo, l2tau)
 
 
-# Largest doubles amplitude (o^2 v^2 elements) for which the mixed solve captures its update
-# as a HIP graph by default. Measured on MI355X (one capture, ~15 replays per solve): the
-# capture costs about one update more than the replays save from (H2O)4 (o^2 v^2 = 2.3e6)
-# up, where the eager update is kernel-bound (0.073 vs 0.078 s per gradient; (H2O)6 0.300
-# vs 0.315; (H2O)8 0.695 vs 0.721); smaller updates are launch-bound ((H2O)2, 1.4e5:
-# 0.056 s with the graph, 0.061 s eager).
-GRAPH_MAX_DOUBLES = 1e6
-
-
 class GraphedUpdate:
     """l -> F(l) (or A l when homogeneous) captured once as a HIP graph and replayed.
 
@@ -354,6 +345,54 @@ class GraphedUpdate:
         return self
 
 
+def _graph_cache(mycc):
+    """The CCSD object's cache of captured lambda updates (None if it takes no attributes)."""
+    cache = getattr(mycc, "_lambda_graph_cache", None)
+    if cache is None:
+        cache = {}
+        try:
+            mycc._lambda_graph_cache = cache
+        except AttributeError:
+            return None
+    return cache
+
+
+class _GraphEntry:
+    """One captured low-precision lambda update and every buffer it reads.
+
+    The capture records l -> F(l) with the weight `inh` of the l-independent terms as a
+    device scalar (1: F(l), 0: A l), so one graph serves the sweeps and the correction
+    solves. It reads the amplitudes t1 / t2, the intermediates and the cached
+    low-precision copies of both (static_operands state) of the solve that captured it;
+    a later solve writes its own values into exactly those buffers (refresh) and
+    replays."""
+
+    def __init__(self, mycc, t1, t2, eris, imds, use_cpu_vvvv, prec, l1, l2, homogeneous, static_state):
+        # (holding eris keeps its id, part of the cache key, from being reused)
+        self.t1, self.t2, self.eris, self.imds, self.static = t1, t2, eris, imds, static_state
+        self.inh = torch.full((), 0.0 if homogeneous else 1.0, dtype=torch.float64, device=t1.device)
+        self.graph = GraphedUpdate(
+            lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec, inh=self.inh),
+            l1, l2).bind_first(l1, l2)
+
+    def refresh(self, t1, t2, imds):
+        """Write this solve's amplitudes and intermediates into the captured buffers and
+        re-prepare the cached low-precision copies in place."""
+        if t1 is not self.t1:
+            self.t1.copy_(t1)
+        if t2 is not self.t2:
+            self.t2.copy_(t2)
+        for name, buf in vars(self.imds).items():
+            new = getattr(imds, name, None)
+            if torch.is_tensor(buf) and torch.is_tensor(new) and new is not buf:
+                buf.copy_(new)
+        refresh_static(self.static)
+
+    def __call__(self, l1, l2, homogeneous):
+        self.inh.fill_(0.0 if homogeneous else 1.0)
+        return self.graph(l1, l2)
+
+
 def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=None, tol=None, verbose=0,
                use_cpu_vvvv=True, precision="fp64", use_graph=None):
     _require_gpu()
@@ -375,28 +414,37 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
     tick("intermediates")
     diis_space = getattr(mycc, "diis_space", 6)
     use_diis = bool(getattr(mycc, "diis", True))
-    # the mixed-precision sweeps replay a captured HIP graph of the update when its
-    # launches, not its kernels, would set the pace (mycc.lambda_graph: None = automatic)
+    # the mixed-precision sweeps replay a captured HIP graph of the low-precision update
+    # (mycc.lambda_graph: None = automatic, True / False forces it). The graph is captured
+    # once per (CCSD object, integrals, precision, sizes) and replayed by every later solve
+    # after this solve's amplitudes, intermediates and static operand copies are written
+    # into the buffers it reads (_GraphEntry.refresh); without a CCSD object that takes
+    # attributes it lives for this solve only.
     if use_graph is None:
         forced = getattr(mycc, "lambda_graph", None)
-        use_graph = precision != "fp64" and (nocc * nocc * nvir * nvir <= GRAPH_MAX_DOUBLES if forced is None
-                                             else bool(forced))
-    graphs = {}
-    inh_t = torch.ones((), dtype=torch.float64, device=t1.device)     # 1: F(l), 0: A l (graph input)
+        use_graph = precision != "fp64" and (forced is None or bool(forced))
+    cache = _graph_cache(mycc) if use_graph else None
+    gkey = (precision, id(eris), bool(use_cpu_vvvv), nocc, nvir, t1.device.index)
+    entry = cache.get(gkey) if cache is not None else None
+    if entry is not None:
+        entry.refresh(t1, t2, imds)
+        tick("graph buffers refreshed")
+    replays0 = entry.graph.replays if entry is not None else 0
     history = []
     mycc._lambda_history = history
 
     def update(l1, l2, prec, homogeneous):
+        nonlocal entry
         if not use_graph or prec == "fp64":
             return update_lambda_gpu(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, prec, homogeneous=homogeneous)
-        inh_t.fill_(0.0 if homogeneous else 1.0)
-        if prec not in graphs:
+        if entry is None:
             tick(None)
-            graphs[prec] = GraphedUpdate(
-                lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec, inh=inh_t),
-                l1, l2).bind_first(l1, l2)
+            entry = _GraphEntry(mycc, t1, t2, eris, imds, use_cpu_vvvv, prec, l1, l2, homogeneous, static_state)
+            if cache is not None:
+                cache.clear()              # one captured update per CCSD object (its buffers are large)
+                cache[gkey] = entry
             tick("graph capture (incl. first update)")
-        return graphs[prec](l1, l2)
+        return entry(l1, l2, homogeneous)
 
     def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
         """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
@@ -433,7 +481,7 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
 
     with static_operands(t1, t2, eris.ovvv, eris.vvvv, eris.ovoo, eris.ovvo, eris.oovv, eris.oooo, imds.w1,
                          imds.w2, imds.w3, imds.w4, imds.wvvov, imds.wvooo, imds.wVOov, imds.wvOOv, imds.woooo,
-                         imds.theta, imds.tau, imds.tmp_voov, imds.g2ovvv, imds.g2voov_b):
+                         imds.theta, imds.tau, imds.tmp_voov, imds.g2ovvv, imds.g2voov_b) as static_state:
         if precision == "fp64":
             conv, l1, l2, normt, iters = fixed_point(l1, l2, "fp64", tol, max_cycle)
             tick("fp64 updates")
@@ -442,10 +490,11 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
             mycc._lambda_normt = normt
             return conv, l1, l2
         # mixed precision: low-precision sweeps, then FP64-residual iterative refinement
-        out = _mixed_solve(mycc, fixed_point, update_lambda_gpu, t1, t2, l1, l2, eris, imds, use_cpu_vvvv,
-                           precision, tol, max_cycle, tick)
-        mycc._lambda_graph_replays = sum(g.replays for g in graphs.values())
-        graphs.clear()
+        out = _mixed_solve(mycc, fixed_point,
+                           lambda _cc, _t1, _t2, a, b, _e, _i, _u, prec, homogeneous=False: update(a, b, prec,
+                                                                                                  homogeneous),
+                           t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, tol, max_cycle, tick)
+        mycc._lambda_graph_replays = entry.graph.replays - replays0 if entry is not None else 0
         return out
 
 

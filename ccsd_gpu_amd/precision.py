@@ -34,13 +34,39 @@ LOWP_PRODUCTS = {"bf16x3": 0, "fp32": 0}
 
 
 @contextmanager
-def static_operands(*tensors):
-    """Within the block, prepared copies of these tensors are cached (they must not change)."""
+def static_operands(*tensors, state=None):
+    """Within the block, prepared copies of these tensors are cached (they must not change).
+
+    Yields the cache state; passing a state back (`state=`) re-enters that cache (its
+    registered tensors and prepared copies), e.g. to refresh it in place with
+    refresh_static after the registered tensors were overwritten."""
     global _STATIC
     prev = _STATIC
-    _STATIC = {"ids": {id(t) for t in tensors if t is not None}, "keep": list(tensors), "cache": {}}
+    if state is None:
+        state = {"ids": {id(t) for t in tensors if t is not None}, "keep": list(tensors), "cache": {},
+                 "makers": {}}
+    _STATIC = state
     try:
-        yield
+        yield state
+    finally:
+        _STATIC = prev
+
+
+def refresh_static(state):
+    """Re-prepare every cached copy of a static_operands state from its (overwritten)
+    source tensors and write it into the cached storage in place, so that device
+    pointers taken from the cache (a captured HIP graph) stay valid."""
+    global _STATIC
+    prev = _STATIC
+    _STATIC = None                      # makers run uncached
+    try:
+        for k, make in state["makers"].items():
+            new, old = make(), state["cache"][k]
+            if isinstance(old, torch.Tensor):
+                old.copy_(new)
+            else:                       # a Split of bf16 planes
+                old.hi.copy_(new.hi)
+                old.lo.copy_(new.lo)
     finally:
         _STATIC = prev
 
@@ -66,6 +92,7 @@ def _cached(X, key, make):
             if v is None:
                 v = make()
                 _STATIC["cache"][k] = v
+                _STATIC["makers"][k] = make
             return v
     return make()
 

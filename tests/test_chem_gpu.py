@@ -459,3 +459,32 @@ def test_fp64_update_with_factored_wvvov_water_tetramer(gpu):
     print("(H2O)4 factored vs formed: l1", d1, "l2", d2, "bf16x3 wvvov", dw, flush=True)
     assert d1 < 1e-12 and d2 < 1e-12
     assert 0 < dw < 1e-4
+
+
+@pytest.mark.gpu
+def test_lambda_graph_captured_once_and_replayed_across_solves(gpu, h2o_pair):
+    """The low-precision update is captured by the first mixed solve and replayed by the
+    next ones: a second solve on the same CCSD object captures nothing new and gives the
+    same lambda; a third one with changed amplitudes (its values written into the
+    captured buffers) equals an eager solve with those amplitudes."""
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    mycc.conv_tol_normt = 1e-10
+    a1, a2 = solve_lambda_gpu(mycc, precision="bf16x3", return_tensors=True)
+    a1, a2 = a1.clone(), a2.clone()
+    entry = next(iter(mycc._lambda_graph_cache.values()))
+    b1, b2 = solve_lambda_gpu(mycc, precision="bf16x3", return_tensors=True)
+    assert next(iter(mycc._lambda_graph_cache.values())) is entry and mycc._lambda_graph_replays > 0
+    assert float((a1 - b1).abs().max()) < 1e-12 and float((a2 - b2).abs().max()) < 1e-12
+    assert float(np.abs(b1.cpu().numpy() - l1).max()) < 1e-8
+    # changed amplitudes: a t2 scaled by 1.01 (a different, still contractive problem)
+    t1s, t2s = mycc.t1 * 0.99, mycc.t2 * 1.01
+    g1, g2 = solve_lambda_gpu(mycc, t1=t1s, t2=t2s, precision="bf16x3", return_tensors=True)
+    g1, g2 = g1.clone(), g2.clone()
+    assert mycc._lambda_graph_replays > 0 and next(iter(mycc._lambda_graph_cache.values())) is entry
+    mycc.lambda_graph = False
+    e1, e2 = solve_lambda_gpu(mycc, t1=t1s, t2=t2s, precision="bf16x3", return_tensors=True)
+    print("graph reuse: changed amplitudes, graph vs eager", float((g1 - e1).abs().max()),
+          float((g2 - e2).abs().max()), flush=True)
+    assert float((g1 - e1).abs().max()) < 1e-9 and float((g2 - e2).abs().max()) < 1e-9
