@@ -368,25 +368,38 @@ class _GraphEntry:
     replays."""
 
     def __init__(self, mycc, t1, t2, eris, imds, use_cpu_vvvv, prec, l1, l2, homogeneous, static_state):
-        # (holding eris keeps its id, part of the cache key, from being reused)
-        self.t1, self.t2, self.eris, self.imds, self.static = t1, t2, eris, imds, static_state
+        # own copies of the amplitudes (a later solve's are written into them, never into the
+        # caller's), registered as static operands of the capture; holding eris keeps its id,
+        # part of the cache key, from being reused
+        self.t1, self.t2 = t1.clone(), t2.clone()
+        static_state["ids"].update((id(self.t1), id(self.t2)))
+        static_state["keep"] += [self.t1, self.t2]
+        self.eris, self.imds, self.static = eris, imds, static_state
         self.inh = torch.full((), 0.0 if homogeneous else 1.0, dtype=torch.float64, device=t1.device)
-        self.graph = GraphedUpdate(
-            lambda a, b: update_lambda_gpu(mycc, t1, t2, a, b, eris, imds, use_cpu_vvvv, prec, inh=self.inh),
-            l1, l2).bind_first(l1, l2)
+        # the cached low-precision copies the update reads (the refresh re-makes only these)
+        static_state["trace"] = set()
+        try:
+            self.graph = GraphedUpdate(
+                lambda a, b: update_lambda_gpu(mycc, self.t1, self.t2, a, b, eris, imds, use_cpu_vvvv, prec,
+                                               inh=self.inh),
+                l1, l2).bind_first(l1, l2)
+        finally:
+            self.keys = static_state.pop("trace")
+
+    # the intermediates a low-precision update reads (update_lambda_gpu)
+    IMDS_READ = ("w1", "w2", "w3", "w4", "woooo", "wvooo", "wvvov", "wvOOv", "theta", "tau", "tmp_voov", "l2_const",
+                 "g2ovvv", "g2voov_b", "eia", "inv_d2")
 
     def refresh(self, t1, t2, imds):
         """Write this solve's amplitudes and intermediates into the captured buffers and
         re-prepare the cached low-precision copies in place."""
-        if t1 is not self.t1:
-            self.t1.copy_(t1)
-        if t2 is not self.t2:
-            self.t2.copy_(t2)
-        for name, buf in vars(self.imds).items():
-            new = getattr(imds, name, None)
-            if torch.is_tensor(buf) and torch.is_tensor(new) and new is not buf:
+        self.t1.copy_(t1)
+        self.t2.copy_(t2)
+        for name in self.IMDS_READ:
+            buf, new = getattr(self.imds, name), getattr(imds, name)
+            if new is not buf:
                 buf.copy_(new)
-        refresh_static(self.static)
+        refresh_static(self.static, self.keys)
 
     def __call__(self, l1, l2, homogeneous):
         self.inh.fill_(0.0 if homogeneous else 1.0)

@@ -52,15 +52,17 @@ def static_operands(*tensors, state=None):
         _STATIC = prev
 
 
-def refresh_static(state):
-    """Re-prepare every cached copy of a static_operands state from its (overwritten)
-    source tensors and write it into the cached storage in place, so that device
-    pointers taken from the cache (a captured HIP graph) stay valid."""
+def refresh_static(state, keys=None):
+    """Re-prepare the cached copies of a static_operands state (those of `keys`, default
+    all) from their (overwritten) source tensors and write them into the cached storage
+    in place, so that device pointers taken from the cache (a captured HIP graph) stay
+    valid. A state with state["trace"] set to a set records the keys read meanwhile."""
     global _STATIC
     prev = _STATIC
     _STATIC = None                      # makers run uncached
     try:
-        for k, make in state["makers"].items():
+        for k in (state["makers"] if keys is None else keys):
+            make = state["makers"][k]
             new, old = make(), state["cache"][k]
             if isinstance(old, torch.Tensor):
                 old.copy_(new)
@@ -93,6 +95,9 @@ def _cached(X, key, make):
                 v = make()
                 _STATIC["cache"][k] = v
                 _STATIC["makers"][k] = make
+            trace = _STATIC.get("trace")
+            if trace is not None:
+                trace.add(k)
             return v
     return make()
 
