@@ -4,112 +4,23 @@ Shells are listed per element as (l, exponents, coefficient columns). A shell
 with several coefficient columns is a general contraction (nctr > 1), stored
 the way PySCF stores it: one _bas row with NCTR_OF = ncolumns.
 
-Values are the published Dunning cc-pVDZ / cc-pVTZ and Pople STO-3G sets (EMSL
-Basis Set Exchange). PySCF's bundled copies of these sets are not available in
-this environment, so whether they agree digit-for-digit with PySCF 2.12.1 is
+The data live in NWChem-format files under gto/basis/ (read by basis_io):
+sto-3g (H, C, N, O, F), cc-pvdz (H, C, N, O), cc-pvtz (H, O), cc-pvqz (H, O),
+def2-svp (H, C, N, O). A path to any NWChem-format file is accepted as a basis
+name too. PySCF's bundled copies of these sets are not available in this
+environment, so whether they agree digit-for-digit with PySCF 2.12.1 is
 unverified; every test here uses this data on both sides of a comparison.
 """
+import os
 
-# cc-pVDZ ------------------------------------------------------------------
-_CCPVDZ = {
-    "H": [
-        (0, [13.01, 1.962, 0.4446], [[0.019685, 0.137977, 0.478148]]),
-        (0, [0.122], [[1.0]]),
-        (1, [0.727], [[1.0]]),
-    ],
-    "C": [
-        (0, [6665.0, 1000.0, 228.0, 64.71, 21.06, 7.495, 2.797, 0.5215],
-         [[0.000692, 0.005329, 0.027077, 0.101718, 0.274740, 0.448564, 0.285074, 0.015204],
-          [-0.000146, -0.001154, -0.005725, -0.023312, -0.063955, -0.149981, -0.127262, 0.544529]]),
-        (0, [0.1596], [[1.0]]),
-        (1, [9.439, 2.002, 0.5456], [[0.038109, 0.209480, 0.508557]]),
-        (1, [0.1517], [[1.0]]),
-        (2, [0.55], [[1.0]]),
-    ],
-    "N": [
-        (0, [9046.0, 1357.0, 309.3, 87.73, 28.56, 10.21, 3.838, 0.7466],
-         [[0.000700, 0.005389, 0.027406, 0.103207, 0.278723, 0.448540, 0.278238, 0.015440],
-          [-0.000153, -0.001208, -0.005992, -0.024544, -0.067459, -0.158078, -0.121831, 0.549003]]),
-        (0, [0.2248], [[1.0]]),
-        (1, [13.55, 2.917, 0.7973], [[0.039919, 0.217169, 0.510319]]),
-        (1, [0.2185], [[1.0]]),
-        (2, [0.817], [[1.0]]),
-    ],
-    "O": [
-        (0, [11720.0, 1759.0, 400.8, 113.7, 37.03, 13.27, 5.025, 1.013],
-         [[0.000710, 0.005470, 0.027837, 0.104800, 0.283062, 0.448719, 0.270952, 0.015458],
-          [-0.000160, -0.001263, -0.006267, -0.025716, -0.070924, -0.165411, -0.116955, 0.557368]]),
-        (0, [0.3023], [[1.0]]),
-        (1, [17.70, 3.854, 1.046], [[0.043018, 0.228913, 0.508728]]),
-        (1, [0.2753], [[1.0]]),
-        (2, [1.185], [[1.0]]),
-    ],
-}
-
-# cc-pVTZ (H, O only: used for f-shell coverage) -----------------------------
-_CCPVTZ = {
-    "H": [
-        (0, [33.87, 5.095, 1.159], [[0.006068, 0.045308, 0.202822]]),
-        (0, [0.3258], [[1.0]]),
-        (0, [0.1027], [[1.0]]),
-        (1, [1.407], [[1.0]]),
-        (1, [0.388], [[1.0]]),
-        (2, [1.057], [[1.0]]),
-    ],
-    "O": [
-        (0, [15330.0, 2299.0, 522.4, 147.3, 47.55, 16.76, 6.207, 0.6882],
-         [[0.000508, 0.003929, 0.020243, 0.079181, 0.230687, 0.433118, 0.350260, -0.008154],
-          [-0.000115, -0.000895, -0.004636, -0.018724, -0.058463, -0.136463, -0.175740, 0.603418]]),
-        (0, [1.752], [[1.0]]),
-        (0, [0.2384], [[1.0]]),
-        (1, [34.46, 7.749, 2.280], [[0.015928, 0.099740, 0.310492]]),
-        (1, [0.7156], [[1.0]]),
-        (1, [0.2140], [[1.0]]),
-        (2, [2.314], [[1.0]]),
-        (2, [0.645], [[1.0]]),
-        (3, [1.428], [[1.0]]),
-    ],
-}
-
-# STO-3G ---------------------------------------------------------------------
-_STO3G_1S = [0.15432897, 0.53532814, 0.44463454]
-_STO3G_2S = [-0.09996723, 0.39951283, 0.70011547]
-_STO3G_2P = [0.15591627, 0.60768372, 0.39195739]
-
-
-def _sto3g_row2(e1s, e2sp):
-    return [
-        (0, e1s, [_STO3G_1S]),
-        (0, e2sp, [_STO3G_2S]),
-        (1, e2sp, [_STO3G_2P]),
-    ]
-
-
-_STO3G = {
-    "H": [(0, [3.42525091, 0.62391373, 0.16885540], [_STO3G_1S])],
-    "C": _sto3g_row2([71.6168370, 13.0450960, 3.5305122], [2.9412494, 0.6834831, 0.2222899]),
-    "N": _sto3g_row2([99.1061690, 18.0523120, 4.8856602], [3.7804559, 0.8784966, 0.2857144]),
-    "O": _sto3g_row2([130.7093200, 23.8088610, 6.4436083], [5.0331513, 1.1695961, 0.3803890]),
-    "F": _sto3g_row2([166.6791300, 30.3608120, 8.2168207], [6.4648032, 1.5022812, 0.4885885]),
-}
-
-BASIS_SETS = {
-    "cc-pvdz": _CCPVDZ,
-    "ccpvdz": _CCPVDZ,
-    "cc-pvtz": _CCPVTZ,
-    "ccpvtz": _CCPVTZ,
-    "sto-3g": _STO3G,
-    "sto3g": _STO3G,
-}
+from . import basis_io
 
 NUC_CHARGE = {"H": 1, "He": 2, "C": 6, "N": 7, "O": 8, "F": 9}
 
 
 def get_basis(name, element):
-    key = name.lower().replace("_", "-")
-    if key not in BASIS_SETS:
-        raise KeyError(f"basis set {name!r} is not available (have: sto-3g, cc-pvdz, cc-pvtz)")
-    table = BASIS_SETS[key]
+    """Shells of `element` in the basis set `name` (a bundled set or an NWChem file)."""
+    table = basis_io.load_file(name) if os.path.exists(str(name)) else basis_io.load(name)
     if element not in table:
         raise KeyError(f"basis set {name!r} has no data for element {element!r}")
     return table[element]

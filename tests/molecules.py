@@ -32,6 +32,15 @@ def h2o_ccpvtz(cart=False):
     return M(H2O_REF, basis="cc-pvtz", cart=cart)
 
 
+def h2o_def2svp(cart=False):
+    return M(H2O_REF, basis="def2-svp", cart=cart)
+
+
+def h2o_ccpvqz(cart=False):
+    """O carries a g shell: int2e_ip1 with nine Rys roots."""
+    return M(H2O_REF, basis="cc-pvqz", cart=cart)
+
+
 def hf_sto3g(cart=False):
     return M("H 0 0 0; F 0 0 1.1", basis="sto-3g", cart=cart)
 

@@ -22,6 +22,7 @@ CASES = [
     ("H2O/cc-pVDZ sph", molecules.h2o_ccpvdz, False),
     ("CH4/cc-pVDZ sph", molecules.ch4_ccpvdz, False),
     ("H2O/cc-pVTZ sph", molecules.h2o_ccpvtz, False),
+    ("H2O/def2-SVP sph", molecules.h2o_def2svp, False),
     ("g-shell sph", molecules.g_shell_mol, False),
     ("g-shell cart", molecules.g_shell_mol, True),
 ]
@@ -130,3 +131,25 @@ def test_register_store_kernels_equal_lds_kernel(gpu, make, cart, deriv):
     if out is not None:
         assert not torch.isnan(out).any()
         assert np.abs(out.cpu().numpy() - ref).max() < 1e-12 * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("deriv", [0, 1])
+def test_water_ccpvqz_g_shell_blocks_match_oracle(gpu, deriv):
+    """H2O/cc-pVQZ (ccsd_gpu tests/test_int2e_gpu.py:89-96, test_int2e_ip1_gpu.py:99-106):
+    the i-shell blocks of O's g shell, its f shells and the first H s shell, spherical,
+    against the oracle at 1e-10 (nine Rys roots for int2e_ip1 of the g block)."""
+    from ccsd_gpu_amd.hip import compute_int2e_gpu, compute_int2e_ip1_gpu
+    mol = molecules.h2o_ccpvqz(cart=False)
+    ls = mol._bas[:, 1]
+    blocks = [int(np.nonzero(ls == 4)[0][0])] + [int(i) for i in np.nonzero(ls == 3)[0][:2]] + \
+             [int(np.nonzero(mol._bas[:, 0] == 1)[0][0])]
+    fn = compute_int2e_ip1_gpu if deriv else compute_int2e_gpu
+    worst = 0.0
+    for b in blocks:
+        got = fn(mol, b, b + 1).cpu().numpy()
+        ref = oi._intor(mol, b, b + 1, deriv)
+        ref = ref if deriv else ref[0]
+        worst = max(worst, float(np.abs(got - ref).max()))
+    print("H2O/cc-pVQZ blocks", blocks, "deriv", deriv, "max|err|", worst, flush=True)
+    assert worst < ATOL
