@@ -135,14 +135,7 @@ def bench_eri(args):
         set_persistent(mol, False)
     log(f"one persistent launch {pers_ms:.1f} ms vs per-class launches {kern_ms:.1f} ms (max |diff| {pers_diff:.1e})")
     if args.copy_calibration:
-        # a plain 256 MiB device copy (256 MiB read + 256 MiB written) in the same run: the
-        # reference that scales the FETCH_SIZE / WRITE_SIZE counters of the fused pass
-        x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
-        y = torch.empty_like(x)
-        torch.cuda.synchronize()
-        y.copy_(x)
-        torch.cuda.synchronize()
-        del x, y
+        copy_calibration()
     cpu = None
     if not args.no_cpu:
         rate, dt, nth = cpu_eri_rate(mol)
@@ -161,6 +154,18 @@ def bench_eri(args):
                    "persistent_single_launch": {"persistent_ms": pers_ms, "per_class_ms": kern_ms,
                                                 "max_abs_diff": pers_diff}},
         "roofline": roof, "cpu_baseline": cpu}
+
+
+def copy_calibration():
+    """A plain 256 MiB device copy (256 MiB read + 256 MiB written): the reference that
+    scales the FETCH_SIZE / WRITE_SIZE counters of a profiled run to bytes."""
+    import torch
+    x = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    del x, y
 
 
 def _sym_density(n, rng):
@@ -605,7 +610,7 @@ def main():
     ap.add_argument("--waters", type=int, default=0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="eri config: end with a plain 256 MiB device copy (HBM counter calibration)")
+                    help="end with a plain 256 MiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -618,6 +623,8 @@ def main():
     _lib.load()
     line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision,
             "gemm": bench_gemm}[args.config](args)
+    if args.copy_calibration and args.config != "eri":
+        copy_calibration()
     print(json.dumps(line), flush=True)
     if line.get("config", {}).get("accuracy_gate_1e-6_ok") is False:
         print("bench.py: mixed and plain FP64 gradients differ by more than 1e-6 Eh/Bohr or the net force is "
