@@ -456,15 +456,16 @@ def bench_grad(args):
     torch.cuda.synchronize()
     log(f"setup (RHF {mf.e_tot:.10f}, CCSD corr {mycc.e_corr:.10f}) {time.perf_counter() - t0:.1f} s")
     res = []
-    # two small fills (12345 float32 elements) bracket each timed step outside its timed
-    # region, so a kernel trace of this run can cut out one mixed gradient
-    marker = torch.empty(12345, dtype=torch.float32, device="cuda")
+    # two small fills (12345 int16 elements, a kernel nothing else launches) bracket each
+    # timed step outside its timed region, so a kernel trace of this run can cut out one
+    # mixed gradient
+    marker = torch.empty(12345, dtype=torch.int16, device="cuda")
     for s in range(args.warmup + args.steps):
         if s >= args.warmup:
-            marker.fill_(1.0)
+            marker.fill_(1)
         de, info = pipeline.ccsd_gradient(mf, mycc, "mixed", args.precision)
         if s >= args.warmup:
-            marker.fill_(2.0)
+            marker.fill_(2)
         log(f"mixed step {s}: {json.dumps(info)}")
         hist = getattr(mycc, "_lambda_history", [])
         log("lambda history: " + " ".join(f"{p[:2]}{'c' if c else ''}:{n:.1e}" for p, c, n in hist))

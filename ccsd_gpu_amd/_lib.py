@@ -61,6 +61,7 @@ _SIGNATURES = {
                                   c_void_p, c_void_p]),
     "ccsd_pair_combine": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double, c_double, c_double,
                                   c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "ccsd_gemm_set_variant": (c_int, [c_int]),
     "ccsd_gemm_bf16x3_nt_scatter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                             ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                             c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),

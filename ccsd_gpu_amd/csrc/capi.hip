@@ -34,6 +34,7 @@ int ccsd_store_reg_parts(int li, int lj, int lk, int ll, int max_ncij, int max_n
 hipError_t ccsd_launch_store_reg(const EriClassArgs& a, int wpb, hipStream_t stream);
 size_t ccsd_fused_reg_lds(int max_nij, int wpb);
 hipError_t ccsd_launch_fused_reg(const EriClassArgs& a, int waves_per_block, hipStream_t stream);
+extern int g_gemm_variant;
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
                                       long long sC, long long ldc, int batch, int beta, double alpha,
@@ -1622,6 +1623,13 @@ int ccsd_gemm_bf16x3_nt_scatter(const void* Ahi, const void* Alo, const void* Bh
     hipError_t e = ccsd_launch_gemm_bf16x3_nt(Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, strideA, strideB, 0, 0, batch,
                                               beta, alpha, off_b, off_r, off_c, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3_nt_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_gemm_set_variant(int v)
+{
+    if (v < 0 || v > 2) return fail(CCSD_ERR_SHAPE, "GEMM variant must be 0, 1 or 2");
+    g_gemm_variant = v;
     return CCSD_OK;
 }
 

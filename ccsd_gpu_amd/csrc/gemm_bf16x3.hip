@@ -34,28 +34,41 @@ __device__ __forceinline__ uint16_t f2bf(float f)
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-constexpr int BM = 128, BN = 128;
 constexpr int NXCD = 8, GROUP_M = 8;
 
-// BK: K elements staged per LDS tile (64 for long K; 32 when K is short, where the
-// deeper tile only adds zero padding and LDS)
-template <int BK>
-__global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
+// One block computes a (64 WM) x (64 WN) tile of C with WM x WN waves, each wave
+// 64 x 64 = 2 x 2 MFMA 32x32 tiles. BK: K elements staged per LDS tile (64 for long K;
+// 32 when K is short, where the deeper tile only adds zero padding and LDS). NS: LDS
+// stages -- 1: one staging buffer, two barriers per K tile; 2: the next tile is stored
+// into the other buffer after the current one is multiplied, one barrier per K tile.
+// The next K tile is always prefetched into registers while the current one is
+// multiplied.
+// AL: every row start is 16-byte aligned and K is a multiple of 8 (checked by the
+// launcher), so a staged 16-byte chunk lies wholly inside or wholly outside K: full K
+// tiles load through precomputed 32-bit row offsets with no per-element test, rows past
+// M or N are read from the last valid row instead (they only feed outputs that are
+// never stored), and only the last, partial K tile selects zeros.
+template <int BK, int WM, int WN, int NS, bool AL>
+__device__ __forceinline__ void gemm_bf16x3_nt_body(
     const __bf16* __restrict__ Ahi, const __bf16* __restrict__ Alo, const __bf16* __restrict__ Bhi,
     const __bf16* __restrict__ Blo, double* __restrict__ C, int M, int N, int K, long long lda, long long ldb,
     long long sA, long long sB, long long sC, long long ldc, int beta, double alpha, const long long* __restrict__ off_b,
     const long long* __restrict__ off_r, const long long* __restrict__ off_c)
 {
+    constexpr int NT = 64 * WM * WN;            // threads
+    constexpr int BMT = 64 * WM, BNT = 64 * WN; // block tile
     constexpr int LDK = BK + 8;                 // bf16 elements per LDS row (BK + 8: conflict-free b128 reads)
-    constexpr int TILE = BM * LDK;              // bf16 elements of one operand plane tile
+    constexpr int TA = BMT * LDK, TB = BNT * LDK;   // bf16 elements of one A / B plane tile
+    constexpr int STAGE = 2 * TA + 2 * TB;      // Ahi, Alo, Bhi, Blo
     constexpr int CPR = BK / 8;                 // 16-byte chunks per tile row
-    constexpr int HCH = BM * CPR / 256;         // chunks per thread per plane
-    __shared__ __attribute__((aligned(16))) __bf16 lds[4 * TILE];   // Ahi, Alo, Bhi, Blo
+    constexpr int HA = BMT * CPR / NT, HB = BNT * CPR / NT;   // chunks per thread per A / B plane
+    static_assert(HA * NT == BMT * CPR && HB * NT == BNT * CPR, "tile rows must split evenly over threads");
+    __shared__ __attribute__((aligned(16))) __bf16 lds[NS * STAGE];
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WN, wn = wave % WN;
     int tm = blockIdx.x, tn = blockIdx.y;
-    if (BK == 64) {
+    if (K > 256) {
         // long K: XCD-aware grouped order (short K keeps the plain M-fastest order,
         // which measured faster when the output write dominates)
         int lin = blockIdx.x + gridDim.x * blockIdx.y;
@@ -67,49 +80,112 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
         tm = first + (lin % (GROUP_M * gridDim.y)) % gsz;
         tn = (lin % (GROUP_M * gridDim.y)) / gsz;
     }
-    const int m0 = tm * BM, n0 = tn * BN;
+    const int m0 = tm * BMT, n0 = tn * BNT;
     const long long bz = blockIdx.z;
-    const __bf16* src[4] = {Ahi + bz * sA, Alo + bz * sA, Bhi + bz * sB, Blo + bz * sB};
-    const int rows[4] = {M, M, N, N};
-    const int base[4] = {m0, m0, n0, n0};
-    const long long ld[4] = {lda, lda, ldb, ldb};
+    const __bf16* pa[2] = {Ahi + bz * sA, Alo + bz * sA};
+    const __bf16* pb[2] = {Bhi + bz * sB, Blo + bz * sB};
 
-    // each thread moves HCH chunks of 8 bf16 (16 bytes) per plane per K tile:
-    // chunk c = tid + 256*h, row = c / CPR, kchunk = c % CPR
-    u32x4 stage[4][HCH];
+    // each thread moves 16-byte chunks (8 bf16): chunk c = tid + NT*h, row = c / CPR,
+    // kchunk = c % CPR
+    u32x4 sta[2][HA], stb[2][HB];
+    auto fetch = [&](const __bf16* src, long long ld, int rows, int base, int c, int k0) {
+        const int r = c / CPR, kc = (c % CPR) * 8;
+        const int gr = base + r, gk = k0 + kc;
+        u32x4 v = {0u, 0u, 0u, 0u};
+        if (gr < rows) {
+            const __bf16* sp = src + (long long)gr * ld + gk;
+            if (gk + 8 <= K && ((reinterpret_cast<uintptr_t>(sp) & 15) == 0)) {
+                v = *reinterpret_cast<const u32x4*>(sp);
+            } else {
+                const uint16_t* s16 = reinterpret_cast<const uint16_t*>(sp);
+                __attribute__((aligned(16))) uint16_t tmp[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) tmp[e] = (gk + e < K) ? s16[e] : (uint16_t)0;
+                v = *reinterpret_cast<const u32x4*>(tmp);
+            }
+        }
+        return v;
+    };
     auto load_tile = [&](int k0) {
 #pragma unroll
-        for (int p = 0; p < 4; p++)
+        for (int p = 0; p < 2; p++) {
 #pragma unroll
-            for (int h = 0; h < HCH; h++) {
-                const int c = tid + 256 * h;
-                const int r = c / CPR, kc = (c % CPR) * 8;
-                const int gr = base[p] + r, gk = k0 + kc;
-                u32x4 v = {0u, 0u, 0u, 0u};
-                if (gr < rows[p]) {
-                    const __bf16* s = src[p] + (long long)gr * ld[p] + gk;
-                    if (gk + 8 <= K && ((reinterpret_cast<uintptr_t>(s) & 15) == 0)) {
-                        v = *reinterpret_cast<const u32x4*>(s);
-                    } else {
-                        const uint16_t* s16 = reinterpret_cast<const uint16_t*>(s);
-                        __attribute__((aligned(16))) uint16_t tmp[8];
+            for (int h = 0; h < HA; h++) sta[p][h] = fetch(pa[p], lda, M, m0, tid + NT * h, k0);
 #pragma unroll
-                        for (int e = 0; e < 8; e++) tmp[e] = (gk + e < K) ? s16[e] : (uint16_t)0;
-                        v = *reinterpret_cast<const u32x4*>(tmp);
-                    }
-                }
-                stage[p][h] = v;
-            }
+            for (int h = 0; h < HB; h++) stb[p][h] = fetch(pb[p], ldb, N, n0, tid + NT * h, k0);
+        }
     };
-    auto store_tile = [&]() {
+    // aligned path: byte offset of the chunk (clamped row, column kc) from the plane
+    // start, shared by the hi and lo planes (same strides); the launcher guarantees a
+    // plane's last row ends below 2^31 bytes, so the offsets are 32-bit
+    unsigned oa[HA], ob[HB];
+    int kca[HA], kcb[HB];
+    if (AL) {
 #pragma unroll
-        for (int p = 0; p < 4; p++)
+        for (int h = 0; h < HA; h++) {
+            const int c = tid + NT * h, r = min(m0 + c / CPR, M - 1);
+            kca[h] = (c % CPR) * 8;
+            oa[h] = (unsigned)(((long long)r * lda + kca[h]) * 2);
+        }
 #pragma unroll
-            for (int h = 0; h < HCH; h++) {
-                const int c = tid + 256 * h;
-                const int r = c / CPR, kc = (c % CPR) * 8;
-                *reinterpret_cast<u32x4*>(lds + p * TILE + r * LDK + kc) = stage[p][h];
+        for (int h = 0; h < HB; h++) {
+            const int c = tid + NT * h, r = min(n0 + c / CPR, N - 1);
+            kcb[h] = (c % CPR) * 8;
+            ob[h] = (unsigned)(((long long)r * ldb + kcb[h]) * 2);
+        }
+    }
+    auto chunk = [](const __bf16* base, unsigned off) {
+        return *reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(base) + off);
+    };
+    auto load_full = [&](int k0) {
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const __bf16* a = pa[p] + k0;
+            const __bf16* b = pb[p] + k0;
+#pragma unroll
+            for (int h = 0; h < HA; h++) sta[p][h] = chunk(a, oa[h]);
+#pragma unroll
+            for (int h = 0; h < HB; h++) stb[p][h] = chunk(b, ob[h]);
+        }
+    };
+    auto load_last = [&](int k0) {   // chunks at or past K read their row start and become zeros
+        const u32x4 z = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+#pragma unroll
+            for (int h = 0; h < HA; h++) {
+                const bool in = k0 + kca[h] < K;
+                const u32x4 v = chunk(pa[p], in ? oa[h] + 2u * k0 : oa[h] - 2u * kca[h]);
+                sta[p][h] = in ? v : z;
             }
+#pragma unroll
+            for (int h = 0; h < HB; h++) {
+                const bool in = k0 + kcb[h] < K;
+                const u32x4 v = chunk(pb[p], in ? ob[h] + 2u * k0 : ob[h] - 2u * kcb[h]);
+                stb[p][h] = in ? v : z;
+            }
+        }
+    };
+    auto load_next = [&](int k0) {
+        if (!AL) load_tile(k0);
+        else if (k0 + BK <= K) load_full(k0);
+        else load_last(k0);
+    };
+    auto store_tile = [&](int stage) {
+        __bf16* st = lds + stage * STAGE;
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+#pragma unroll
+            for (int h = 0; h < HA; h++) {
+                const int c = tid + NT * h;
+                *reinterpret_cast<u32x4*>(st + p * TA + (c / CPR) * LDK + (c % CPR) * 8) = sta[p][h];
+            }
+#pragma unroll
+            for (int h = 0; h < HB; h++) {
+                const int c = tid + NT * h;
+                *reinterpret_cast<u32x4*>(st + 2 * TA + p * TB + (c / CPR) * LDK + (c % CPR) * 8) = stb[p][h];
+            }
+        }
     };
 
     v16f acc[2][2];
@@ -121,33 +197,76 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
             for (int e = 0; e < 16; e++) acc[i][j][e] = 0.0f;
 
     const int half = lane >> 5, l32 = lane & 31;
-    load_tile(0);
-    for (int k0 = 0; k0 < K; k0 += BK) {
-        __syncthreads();           // previous tile fully consumed
-        store_tile();
-        __syncthreads();
-        if (k0 + BK < K) load_tile(k0 + BK);
+    // fragments of one 16-deep K step: ah[0..1], al[0..1], bh[0..1], bl[0..1]
+    auto read_frags = [&](const __bf16* st, int kk, v8bf* f) {
+        const int ko = kk + half * 8;
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int r = wm * 64 + i * 32 + l32;
+            f[i] = *reinterpret_cast<const v8bf*>(st + r * LDK + ko);
+            f[2 + i] = *reinterpret_cast<const v8bf*>(st + TA + r * LDK + ko);
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int c = wn * 64 + j * 32 + l32;
+            f[4 + j] = *reinterpret_cast<const v8bf*>(st + 2 * TA + c * LDK + ko);
+            f[6 + j] = *reinterpret_cast<const v8bf*>(st + 2 * TA + TB + c * LDK + ko);
+        }
+    };
+    // each accumulator takes lo.hi, hi.lo, hi.hi in that order; consecutive
+    // instructions go to different accumulators (no back-to-back dependence)
+    auto mfmas = [&](const v8bf* f) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[2 + i], f[4 + j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[i], f[6 + j], acc[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[i], f[4 + j], acc[i][j], 0, 0, 0);
+    };
+    auto multiply = [&](int stage) {
+        const __bf16* st = lds + stage * STAGE;
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
-            const int ko = kk + half * 8;
-            v8bf ah[2], al[2], bh[2], bl[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const int r = wm * 64 + i * 32 + l32;
-                ah[i] = *reinterpret_cast<const v8bf*>(lds + 0 * TILE + r * LDK + ko);
-                al[i] = *reinterpret_cast<const v8bf*>(lds + 1 * TILE + r * LDK + ko);
-                const int c = wn * 64 + i * 32 + l32;
-                bh[i] = *reinterpret_cast<const v8bf*>(lds + 2 * TILE + c * LDK + ko);
-                bl[i] = *reinterpret_cast<const v8bf*>(lds + 3 * TILE + c * LDK + ko);
+            v8bf f[8];
+            read_frags(st, kk, f);
+            mfmas(f);
+        }
+    };
+
+    load_next(0);
+    if (NS == 1) {
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            __syncthreads();           // previous tile fully consumed
+            store_tile(0);
+            __syncthreads();
+            if (k0 + BK < K) load_next(k0 + BK);
+            multiply(0);
+        }
+    } else {
+        // stage s holds tile t with s = t % 2; tile t + 1 goes into the other stage once
+        // tile t has been multiplied (that stage was last read for tile t - 1, before
+        // the previous barrier)
+        store_tile(0);
+        if (BK < K) load_next(BK);
+        __syncthreads();
+        int stage = 0;
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            multiply(stage);
+            if (k0 + BK < K) {
+                store_tile(stage ^ 1);
+                if (k0 + 2 * BK < K) load_next(k0 + 2 * BK);
             }
-#pragma unroll
-            for (int i = 0; i < 2; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                }
+            __syncthreads();
+            stage ^= 1;
         }
     }
     // lane holds C[8*(e/4) + 4*half + e%4][l32] of each 32x32 tile; the element (row, col)
@@ -156,16 +275,8 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
     // The destination offsets of the lane's 32 rows and 2 columns are loaded first, and
     // a 32 x 32 tile's 16 previous values before any of its stores, so that the
     // epilogue's memory round trips overlap instead of running one element at a time
-    // (all four tiles at once would need more than 256 registers).
+    // (all tiles at once would need more than 256 registers).
     double* Cb = off_r ? C + off_b[bz] : C + bz * sC;
-    long long orow[2][16];
-#pragma unroll
-    for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int e = 0; e < 16; e++) {
-            const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
-            orow[i][e] = gr < M ? (off_r ? off_r[gr] : (long long)gr * ldc) : -1;
-        }
     long long ocol[2];
 #pragma unroll
     for (int j = 0; j < 2; j++) {
@@ -173,17 +284,37 @@ __global__ void __launch_bounds__(256) gemm_bf16x3_nt_kernel(
         ocol[j] = gc < N ? (off_r ? off_c[gc] : (long long)gc) : -1;
     }
 #pragma unroll
-    for (int i = 0; i < 2; i++)
+    for (int i = 0; i < 2; i++) {
+        long long orow[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int gr = m0 + wm * 64 + i * 32 + 8 * (e >> 2) + 4 * half + (e & 3);
+            orow[e] = gr < M ? (off_r ? off_r[gr] : (long long)gr * ldc) : -1;
+        }
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             if (ocol[j] < 0) continue;
             double prev[16];
 #pragma unroll
-            for (int e = 0; e < 16; e++) prev[e] = (beta && orow[i][e] >= 0) ? Cb[orow[i][e] + ocol[j]] : 0.0;
+            for (int e = 0; e < 16; e++) prev[e] = (beta && orow[e] >= 0) ? Cb[orow[e] + ocol[j]] : 0.0;
 #pragma unroll
             for (int e = 0; e < 16; e++)
-                if (orow[i][e] >= 0) Cb[orow[i][e] + ocol[j]] = prev[e] + alpha * (double)acc[i][j][e];
+                if (orow[e] >= 0) Cb[orow[e] + ocol[j]] = prev[e] + alpha * (double)acc[i][j][e];
         }
+    }
+}
+
+#define GEMM_BF16X3_ARGS                                                                                              \
+    const __bf16 *__restrict__ Ahi, const __bf16 *__restrict__ Alo, const __bf16 *__restrict__ Bhi,                  \
+        const __bf16 *__restrict__ Blo, double *__restrict__ C, int M, int N, int K, long long lda, long long ldb,    \
+        long long sA, long long sB, long long sC, long long ldc, int beta, double alpha,                              \
+        const long long *__restrict__ off_b, const long long *__restrict__ off_r, const long long *__restrict__ off_c
+#define GEMM_BF16X3_PASS Ahi, Alo, Bhi, Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta, alpha, off_b, off_r, off_c
+
+template <int BK, int WM, int WN, int NS, bool AL>
+__global__ void __launch_bounds__(64 * WM * WN) gemm_bf16x3_nt_kernel(GEMM_BF16X3_ARGS)
+{
+    gemm_bf16x3_nt_body<BK, WM, WN, NS, AL>(GEMM_BF16X3_PASS);
 }
 
 // x (float64, rows x cols, row stride ld) -> hi, lo (bfloat16, rows x cols contiguous)
@@ -331,6 +462,13 @@ hipError_t ccsd_launch_scatter_sum(const double* ws, int S, int nb, int M, int N
     return hipGetLastError();
 }
 
+// block shape and staging for long K on aligned operands (ccsd_gemm_set_variant):
+// 0: 128 x 128, BK 64, one LDS stage (the default); 1: 128 x 128, BK 32, two stages;
+// 2: 128 x 256 (8 waves), BK 32, two stages. On the lambda shapes (820 x 23104 x 23104,
+// 6080^3, 1600 x 23104 x 6080, 1600 x 23104 x 1600) 1 and 2 measured 0-15% slower
+// than 0, except 2 on the shortest K (4% faster).
+int g_gemm_variant = 0;
+
 hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const void* Bhi, const void* Blo, double* C,
                                       int M, int N, int K, long long lda, long long ldb, long long sA, long long sB,
                                       long long sC, long long ldc, int batch, int beta, double alpha,
@@ -338,15 +476,26 @@ hipError_t ccsd_launch_gemm_bf16x3_nt(const void* Ahi, const void* Alo, const vo
                                       hipStream_t stream)
 {
     if (M <= 0 || N <= 0 || batch <= 0) return hipSuccess;
-    dim3 grid((M + BM - 1) / BM, (N + BN - 1) / BN, batch);
-    if (K > 256)
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<64>, grid, dim3(256), 0, stream, (const __bf16*)Ahi,
-                           (const __bf16*)Alo, (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB,
-                           sC, ldc, beta, alpha, off_b, off_r, off_c);
-    else
-        hipLaunchKernelGGL(gemm_bf16x3_nt_kernel<32>, grid, dim3(256), 0, stream, (const __bf16*)Ahi,
-                           (const __bf16*)Alo, (const __bf16*)Bhi, (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB,
-                           sC, ldc, beta, alpha, off_b, off_r, off_c);
+    // aligned path: 16-byte rows and plane offsets below 2^31 bytes
+    const bool al = K % 8 == 0 && lda % 8 == 0 && ldb % 8 == 0 && sA % 8 == 0 && sB % 8 == 0 &&
+                    ((uintptr_t)Ahi | (uintptr_t)Alo | (uintptr_t)Bhi | (uintptr_t)Blo) % 16 == 0 &&
+                    ((long long)(M - 1) * lda + K) * 2 < (1LL << 31) && ((long long)(N - 1) * ldb + K) * 2 < (1LL << 31);
+    void (*kern)(GEMM_BF16X3_ARGS) = nullptr;
+    int bm = 128, bn = 128, nt = 256;
+    if (K > 256 && al) {
+        switch (g_gemm_variant) {
+        case 1: kern = gemm_bf16x3_nt_kernel<32, 2, 2, 2, true>; break;
+        case 2: kern = gemm_bf16x3_nt_kernel<32, 2, 4, 2, true>; bn = 256; nt = 512; break;
+        default: kern = gemm_bf16x3_nt_kernel<64, 2, 2, 1, true>;
+        }
+    } else if (K > 256) {
+        kern = gemm_bf16x3_nt_kernel<64, 2, 2, 1, false>;
+    } else {
+        kern = al ? gemm_bf16x3_nt_kernel<32, 2, 2, 1, true> : gemm_bf16x3_nt_kernel<32, 2, 2, 1, false>;
+    }
+    dim3 grid((M + bm - 1) / bm, (N + bn - 1) / bn, batch);
+    hipLaunchKernelGGL(kern, grid, dim3(nt), 0, stream, (const __bf16*)Ahi, (const __bf16*)Alo, (const __bf16*)Bhi,
+                       (const __bf16*)Blo, C, M, N, K, lda, ldb, sA, sB, sC, ldc, beta, alpha, off_b, off_r, off_c);
     return hipGetLastError();
 }
 

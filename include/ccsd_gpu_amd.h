@@ -296,6 +296,15 @@ int ccsd_gemm_bf16x3_nt_scatter(const void* Ahi, const void* Alo, const void* Bh
                                 const long long* off_b, const long long* off_r, const long long* off_c, int batch,
                                 double alpha, int beta, void* stream);
 
+/* Block shape of ccsd_gemm_bf16x3_nt(_scatter) for K > 256 on aligned operands
+ * (K and the row strides multiples of 8, 16-byte aligned planes): 0 (the default)
+ * 128 x 128 blocks staging K by 64 through one LDS buffer; 1 the same blocks staging
+ * K by 32 through two LDS buffers; 2 128 x 256 blocks of 8 waves, K by 32, two LDS
+ * buffers. Process-wide; every variant forms the same products in the same order per
+ * output element. Returns CCSD_ERR_SHAPE for another value. No counterpart in the
+ * reference: a tuning switch for timing and testing the block shapes on one problem. */
+int ccsd_gemm_set_variant(int v);
+
 /* ccsd_split_bf16 from an N-d strided view (ndim <= 8; sizes, element strides as
  * host arrays): the planes are contiguous in the view's row-major order, so an
  * operand is permuted and split in one pass. */

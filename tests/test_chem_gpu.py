@@ -84,6 +84,63 @@ def test_bf16x3_nt_split_k(gpu, nb, m, k, n, ksplit):
     assert float((C2 - 2 * ref).abs().max()) / k ** 0.5 < 4e-4
 
 
+@pytest.fixture
+def gemm_variant():
+    from ccsd_gpu_amd import _lib
+    lib = _lib.load()
+    yield lambda v: _lib.check(lib.ccsd_gemm_set_variant(v), "ccsd_gemm_set_variant")
+    lib.ccsd_gemm_set_variant(0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [1, 2])
+@pytest.mark.parametrize("nb,m,n,k", [(1, 1024, 1024, 1024), (1, 820, 2400, 4100), (2, 513, 2047, 1112),
+                                      (1, 130, 1030, 264)])
+def test_bf16x3_nt_block_variants_match_fp64_and_default(gpu, gemm_variant, variant, nb, m, n, k):
+    """Long-K block shapes and LDS staging (128 x 128 or 128 x 256 blocks, one or two
+    LDS stages) against FP64 and the default kernel: ragged M, N and K (K not a
+    multiple of the stage depth), batches, beta = 1."""
+    from ccsd_gpu_amd import mfma
+    gen = torch.Generator(device="cuda").manual_seed(m + n)
+    A = torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen)
+    B = torch.randn((nb, n, k), dtype=torch.float64, device="cuda", generator=gen)
+    As, Bs = mfma.split(A), mfma.split(B)
+    ref = A @ B.transpose(1, 2)
+    gemm_variant(variant)
+    got = mfma.gemm_nt(As, Bs, ksplit=1)
+    got2 = mfma.gemm_nt(As, Bs, out=got.clone(), beta=1, ksplit=1)
+    gemm_variant(0)
+    base = mfma.gemm_nt(As, Bs, ksplit=1)
+    err = float((got - ref).abs().max()) / k ** 0.5
+    err2 = float((got2 - 2 * ref).abs().max()) / k ** 0.5
+    d = float((got - base).abs().max()) / k ** 0.5
+    print("variant", variant, (nb, m, n, k), "vs fp64", err, "beta=1", err2, "vs default", d, flush=True)
+    # same MFMA sequence per output element (K steps of 16 in order, lo.hi, hi.lo, hi.hi)
+    assert err < 2e-4 and err2 < 4e-4 and d == 0.0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_bf16x3_nt_scatter_destination_variants(gpu, gemm_variant, variant):
+    """C[off_b + off_r[row] + off_c[col]] += alpha A B^T into a transposed destination."""
+    from ccsd_gpu_amd import mfma
+    m, n, k = 600, 1900, 1304
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    A = torch.randn((1, m, k), dtype=torch.float64, device="cuda", generator=gen)
+    B = torch.randn((1, n, k), dtype=torch.float64, device="cuda", generator=gen)
+    ref = (A @ B.transpose(1, 2))[0]
+    out = torch.randn((n, m), dtype=torch.float64, device="cuda", generator=gen)
+    want = out + 0.5 * ref.T
+    zero = torch.zeros(1, dtype=torch.int64, device="cuda")
+    rows = torch.arange(m, dtype=torch.int64, device="cuda")            # row r -> column r of out
+    cols = torch.arange(n, dtype=torch.int64, device="cuda") * m        # col c -> row c of out
+    gemm_variant(variant)
+    mfma.gemm_nt_scatter(mfma.split(A), mfma.split(B), out, zero, rows, cols, alpha=0.5, beta=1, ksplit=1)
+    err = float((out - want).abs().max()) / k ** 0.5
+    print("scatter variant", variant, err, flush=True)
+    assert err < 2e-4
+
+
 @pytest.mark.gpu
 def test_vvvv_term_symmetric_half_matches_full(gpu):
     from ccsd_gpu_amd.precision import static_operands, vvvv_term
