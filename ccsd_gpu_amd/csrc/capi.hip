@@ -54,6 +54,11 @@ hipError_t ccsd_launch_add_separable_packed(double* g, const double* D, const do
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream);
 size_t ccsd_l1_terms_lds(int v);
+hipError_t ccsd_launch_vvvv_pm_pack(const double* l2, int o, int v, const long long* iu, const long long* ju,
+                                    long long npair, const long long* tc, const long long* td, long long np,
+                                    long long ld, int split, void* out, void* lo, hipStream_t stream);
+hipError_t ccsd_launch_vvvv_pm_unpack(const double* X, int o, int v, long long ldx, double scale, double* out,
+                                      hipStream_t stream);
 hipError_t ccsd_launch_unpack_pairs(const double* X, const long long* iu, const long long* ju, int o, int v,
                                     long long npair, double s, double* out, hipStream_t stream);
 hipError_t ccsd_launch_permute(const double* src, int ndim, const long long* sizes, const long long* sstrides,
@@ -1459,6 +1464,29 @@ int ccsd_unpack_pairs(const double* x, const long long* iu, const long long* ju,
     if (o < 0 || v < 0 || npair < 0 || npair > (long long)o * (o + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
     hipError_t e = ccsd_launch_unpack_pairs(x, iu, ju, o, v, npair, scale, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "unpack_pairs_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_vvvv_pm_pack(const double* l2, int o, int v, const long long* iu, const long long* ju, long long npair,
+                      const long long* tc, const long long* td, long long np, long long ld, int split, void* out,
+                      void* lo, void* stream)
+{
+    if (!l2 || !iu || !ju || !tc || !td || !out || (split && !lo)) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0 || npair < 0 || npair > (long long)o * (o + 1) / 2 || np != (long long)v * (v + 1) / 2 ||
+        ld < np)
+        return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_vvvv_pm_pack(l2, o, v, iu, ju, npair, tc, td, np, ld, split, out, lo,
+                                            (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "vvvv_pm_pack_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_vvvv_pm_unpack(const double* x, int o, int v, long long ldx, double scale, double* out, void* stream)
+{
+    if (!x || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0 || ldx < (long long)v * (v + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_vvvv_pm_unpack(x, o, v, ldx, scale, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "vvvv_pm_unpack_kernel");
     return CCSD_OK;
 }
 

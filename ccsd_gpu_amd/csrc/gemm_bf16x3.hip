@@ -17,6 +17,8 @@
 // in groups of 8 M-tiles (N-tile by N-tile inside a group) so that the blocks
 // running together on one XCD share A and B tiles in that XCD's L2.
 #include <hip/hip_runtime.h>
+
+#include "bf16_split.h"
 #include <stdint.h>
 
 namespace {
@@ -25,14 +27,6 @@ typedef float v16f __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// float -> bfloat16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 constexpr int NXCD = 8, GROUP_M = 8;
 
@@ -325,9 +319,9 @@ __global__ void split_bf16_kernel(const double* __restrict__ x, uint16_t* __rest
     for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
         const long long r = t / cols, c = t - r * cols;
         const float f = (float)x[r * ld + c];
-        const uint16_t h = f2bf(f);
+        const uint16_t h = ccsd_f2bf(f);
         hi[t] = h;
-        lo[t] = f2bf(f - bf2f(h));
+        lo[t] = ccsd_f2bf(f - ccsd_bf2f(h));
     }
 }
 
@@ -351,9 +345,9 @@ __global__ void split_bf16_strided_kernel(const double* __restrict__ x, uint16_t
             rem = q;
         }
         const float f = (float)x[off];
-        const uint16_t h = f2bf(f);
+        const uint16_t h = ccsd_f2bf(f);
         hi[t] = h;
-        lo[t] = f2bf(f - bf2f(h));
+        lo[t] = ccsd_f2bf(f - ccsd_bf2f(h));
     }
 }
 

@@ -18,7 +18,16 @@
 // write runs along b (coalesced when X's and out's b strides are 1); X and out
 // are any strided 4-index views (a permuted layout such as [i][a][j][b]), D
 // optional and contiguous.
+//
+// vvvv_pm_pack_kernel / vvvv_pm_unpack_kernel: the l2 (ac|bd) term over index pairs.
+// With P = (c, d), c >= d, and the i <= j pair blocks A = l2[i,j],
+//   A+[P] = (A[c,d] + A[d,c]) (1/2 when c = d),  A-[P] = A[c,d] - A[d,c],
+// the products X+- = A+- W+-^T with W+-[Q, P] = (ac|bd) +- (ad|bc) give
+//   X[a,b] = (X+[Q] + s X-[Q]) / 2,  Q = (max(a,b), min(a,b)), s = +1 (a >= b) or -1,
+// which the unpack writes to out[i,j] and, transposed, out[j,i] (times a scale).
 #include <hip/hip_runtime.h>
+
+#include "bf16_split.h"
 
 namespace {
 
@@ -261,7 +270,100 @@ __global__ void __launch_bounds__(256) unpack_pairs_kernel(const double* __restr
     }
 }
 
+// Row r of the output is the pair (iu[r], ju[r]), column P < np the index pair
+// (tc[P], td[P]), tc >= td; columns np..ld-1 are zero (K padding of a GEMM operand).
+// SPLIT: write bf16 hi / lo planes (out, lo) instead of float64 (out).
+template <bool SPLIT>
+__global__ void __launch_bounds__(256) vvvv_pm_pack_kernel(const double* __restrict__ l2, int o, int v,
+                                                            const long long* __restrict__ iu,
+                                                            const long long* __restrict__ ju, long long npair,
+                                                            const long long* __restrict__ tc,
+                                                            const long long* __restrict__ td, long long np,
+                                                            long long ld, void* __restrict__ out,
+                                                            uint16_t* __restrict__ lo)
+{
+    const long long n = npair * ld;
+    const long long plane = npair * ld;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long r = t / ld, P = t - r * ld;
+        double plus = 0.0, minus = 0.0;
+        if (P < np) {
+            const int c = (int)tc[P], d = (int)td[P];
+            const double* A = l2 + (iu[r] * o + ju[r]) * (long long)v * v;
+            const double x = A[(long long)c * v + d], y = A[(long long)d * v + c];
+            plus = (c == d ? 0.5 : 1.0) * (x + y);
+            minus = x - y;
+        }
+        if (SPLIT) {
+            uint16_t* hi = static_cast<uint16_t*>(out);
+            ccsd_split_bf16(plus, hi[t], lo[t]);
+            ccsd_split_bf16(minus, hi[plane + t], lo[plane + t]);
+        } else {
+            double* o64 = static_cast<double*>(out);
+            o64[t] = plus;
+            o64[plane + t] = minus;
+        }
+    }
+}
+
+// out[i,j,a,b] = scale X[a,b] of the pair (min(i,j), max(i,j)) (transposed when i > j)
+// from X+- (2, npair, ldx); on i = j the symmetrised block, scale X+[Q] / 2.
+__global__ void __launch_bounds__(256) vvvv_pm_unpack_kernel(const double* __restrict__ X, int o, int v,
+                                                              long long ldx, double scale, double* __restrict__ out)
+{
+    const long long vv = (long long)v * v;
+    const long long n = (long long)o * o * vv;
+    const long long plane = (long long)o * (o + 1) / 2 * ldx;
+    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+        const long long ij = t / vv, ab = t - ij * vv;
+        const int i = (int)(ij / o), j = (int)(ij - (long long)i * o);
+        const int a = (int)(ab / v), b = (int)(ab - (long long)a * v);
+        const int pi = min(i, j), pj = max(i, j);
+        const long long r = (long long)pi * o - (long long)pi * (pi - 1) / 2 + (pj - pi);   // triu order
+        const int p = i <= j ? a : b, q = i <= j ? b : a;                                   // X[p, q]
+        const int hi = max(p, q), lw = min(p, q);
+        const long long Q = (long long)hi * (hi + 1) / 2 + lw;
+        const double xp = X[r * ldx + Q];
+        double val;
+        if (i == j) {
+            val = 0.5 * xp;
+        } else {
+            const double xm = X[plane + r * ldx + Q];
+            val = 0.5 * (p >= q ? xp + xm : xp - xm);
+        }
+        out[t] = scale * val;
+    }
+}
+
 }  // namespace
+
+hipError_t ccsd_launch_vvvv_pm_pack(const double* l2, int o, int v, const long long* iu, const long long* ju,
+                                    long long npair, const long long* tc, const long long* td, long long np,
+                                    long long ld, int split, void* out, void* lo, hipStream_t stream)
+{
+    const long long n = npair * ld;
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    if (split)
+        hipLaunchKernelGGL(vvvv_pm_pack_kernel<true>, dim3((unsigned)nb), dim3(256), 0, stream, l2, o, v, iu, ju, npair,
+                           tc, td, np, ld, out, (uint16_t*)lo);
+    else
+        hipLaunchKernelGGL(vvvv_pm_pack_kernel<false>, dim3((unsigned)nb), dim3(256), 0, stream, l2, o, v, iu, ju,
+                           npair, tc, td, np, ld, out, (uint16_t*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_vvvv_pm_unpack(const double* X, int o, int v, long long ldx, double scale, double* out,
+                                      hipStream_t stream)
+{
+    const long long n = (long long)o * o * v * v;
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(vvvv_pm_unpack_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, o, v, ldx, scale, out);
+    return hipGetLastError();
+}
 
 hipError_t ccsd_launch_unpack_pairs(const double* X, const long long* iu, const long long* ju, int o, int v,
                                     long long npair, double s, double* out, hipStream_t stream)

@@ -20,6 +20,8 @@
 // along n. With several K slices the partial sums go to a workspace that the
 // slice-sum kernel adds in a fixed order.
 #include <hip/hip_runtime.h>
+
+#include "bf16_split.h"
 #include <stdint.h>
 
 namespace {
@@ -33,13 +35,6 @@ struct PermDesc {
     long long dstride[MAXD];   // destination (contiguous) strides
 };
 
-__device__ __forceinline__ uint16_t f2bf(float f)
-{
-    uint32_t u = __float_as_uint(f);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
 __device__ __forceinline__ void store(double v, long long e, double* dst, uint16_t* hi, uint16_t* lo)
 {
@@ -47,9 +42,9 @@ __device__ __forceinline__ void store(double v, long long e, double* dst, uint16
         dst[e] = v;
     } else {
         const float f = (float)v;
-        const uint16_t h = f2bf(f);
+        const uint16_t h = ccsd_f2bf(f);
         hi[e] = h;
-        lo[e] = f2bf(f - bf2f(h));
+        lo[e] = ccsd_f2bf(f - ccsd_bf2f(h));
     }
 }
 

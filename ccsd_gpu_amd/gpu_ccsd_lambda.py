@@ -384,7 +384,12 @@ class _GraphEntry:
                                                inh=self.inh),
                 l1, l2).bind_first(l1, l2)
         finally:
-            self.keys = static_state.pop("trace")
+            traced = static_state.pop("trace")
+        # copies made from the integrals stay valid: an entry only serves solves on the
+        # ERIs object it was captured with (part of its cache key), whose blocks are fixed
+        fixed = {id(getattr(eris, n)) for n in ("ovvv", "vvvv", "ovoo", "ovvo", "oovv", "oooo")
+                 if getattr(eris, n, None) is not None}
+        self.keys = {k for k in traced if k[0] not in fixed}
 
     # the intermediates a low-precision update reads (update_lambda_gpu)
     IMDS_READ = ("w1", "w2", "w3", "w4", "woooo", "wvooo", "wvvov", "wvOOv", "theta", "tau", "tmp_voov", "l2_const",

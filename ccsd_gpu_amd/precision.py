@@ -23,6 +23,7 @@ from contextlib import contextmanager
 from math import prod
 
 import torch
+from torch.utils.weak import WeakIdKeyDictionary
 
 from . import mfma
 
@@ -438,33 +439,128 @@ def _fp64_matmul_ksplit(A, B):
     return run(ks)
 
 
-def vvvv_term(l2, vvvv, precision, scale=1.0):
-    """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update).
+# (W+, W-) of each vvvv tensor in use, per precision: built once and kept while the
+# tensor lives (the integrals of one ERIs object do not change)
+_VVVV_PM = WeakIdKeyDictionary()
 
-    X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so only i <= j is
-    evaluated (half the o^2 v^4 work) and the rest mirrored (times `scale`), at
-    every precision."""
+
+_PAIR_MAPS = {}
+
+
+def _pair_maps(v, device):
+    """Lower-triangle pairs P = (c, d), c >= d, in torch.tril_indices order: the flat
+    indices c v + d and d v + c, the weight 1/2 of the diagonal pairs (1 elsewhere),
+    and for every (a, b) the pair index of (max, min) with the sign +1 (a >= b) or
+    -1 (a < b). Made once per (v, device)."""
+    key = (v, str(device))
+    maps = _PAIR_MAPS.get(key)
+    if maps is None:
+        tc, td = torch.tril_indices(v, v, device=device)
+        cd, dc = tc * v + td, td * v + tc
+        wd = torch.where(tc == td, 0.5, 1.0).to(torch.float64)
+        a = torch.arange(v, device=device)
+        hi, lo = torch.maximum(a[:, None], a[None, :]), torch.minimum(a[:, None], a[None, :])
+        pidx = (hi * (hi + 1) // 2 + lo).flatten()
+        sign = torch.where(a[:, None] >= a[None, :], 1.0, -1.0).to(torch.float64).flatten()
+        maps = _PAIR_MAPS[key] = (tc, td, cd, dc, wd, pidx, sign)
+    return maps
+
+
+def _vvvv_pm(vvvv, precision):
+    """W[0] = W+, W[1] = W- over pairs Q = (a, b), a >= b and P = (c, d), c >= d:
+    W+-[Q, P] = (ac|bd) +- (ad|bc), symmetric matrices (n_pair x n_pair) in FP64 or FP32,
+    or as split-BF16 planes with P padded by zeros to a multiple of 8 (16-byte rows
+    for the GEMM's aligned loads); cached for the life of `vvvv`."""
+    ent = _VVVV_PM.get(vvvv)
+    if ent is None:
+        ent = {}
+        _VVVV_PM[vvvv] = ent
+    if precision in ent:
+        return ent[precision]
+    W = ent.get("fp64")
+    if W is None:
+        v = vvvv.shape[0]
+        tc, td, cd, dc, _, _, _ = _pair_maps(v, vvvv.device)
+        n = tc.numel()
+        W = torch.empty((2, n, n), dtype=torch.float64, device=vvvv.device)
+        for q0 in range(0, n, 2048):                      # (ac|bd) rows of 2048 (a, b) pairs at a time
+            q1 = min(n, q0 + 2048)
+            M = vvvv[tc[q0:q1], :, td[q0:q1], :].reshape(q1 - q0, v * v)   # M[q, c v + d] = (ac|bd)
+            x, y = M[:, cd], M[:, dc]
+            torch.add(x, y, out=W[0, q0:q1])
+            torch.sub(x, y, out=W[1, q0:q1])
+            del M, x, y
+        ent["fp64"] = W
+    if precision == "bf16x3":
+        n = W.shape[1]
+        Wp = torch.zeros((2, n, _pad8(n)), dtype=torch.float64, device=W.device)
+        Wp[:, :, :n] = W
+        ent[precision] = mfma.split(Wp)
+        del Wp
+    elif precision != "fp64":
+        ent[precision] = W.float()
+    return ent[precision]
+
+
+def _pad8(n):
+    return -(-n // 8) * 8
+
+
+def vvvv_term(l2, vvvv, precision, scale=1.0):
+    """X[i,j,a,b] = sum_cd l2[i,j,c,d] (ac|bd) (the l2.vvvv term of the lambda update),
+    times `scale`.
+
+    X[j,i,b,a] = X[i,j,a,b] for the 8-fold symmetric vvvv, so only i <= j is evaluated
+    and the rest mirrored. Over the (a, b) and (c, d) pairs the product splits into a
+    symmetric and an antisymmetric half,
+        X[ab] +- X[ba] = sum_{c >= d} (l2[cd] +- l2[dc]) ((ac|bd) +- (ad|bc))
+    (the c = d terms halved on the left, absent from the minus half), two products of
+    v(v+1)/2 x v(v+1)/2 pair matrices run as one batch: a quarter of the o^2 v^4 work of
+    the plain contraction, half that of its i <= j half. On the device one kernel packs
+    the pair rows (split into BF16 planes for bf16x3) and one writes the dense result."""
     o, _, v, _ = l2.shape
     iu, ju = torch.triu_indices(o, o, device=l2.device)
-    A = l2[iu, ju].reshape(1, -1, v * v)
+    tc, td, cd, dc, wd, pidx, sign = _pair_maps(v, l2.device)
+    npair, n = iu.numel(), tc.numel()
+    W = _vvvv_pm(vvvv, precision)
+    if l2.is_cuda and l2.dtype == torch.float64:
+        from . import _lib
+        lib = _lib.load()
+        l2c = l2.contiguous()
+        if precision == "bf16x3":
+            kp = _pad8(n)
+            planes = mfma.Split(torch.empty((2, npair, kp), dtype=torch.bfloat16, device=l2.device),
+                                torch.empty((2, npair, kp), dtype=torch.bfloat16, device=l2.device))
+            _lib.check(lib.ccsd_vvvv_pm_pack(_lib.ptr(l2c), o, v, _lib.ptr(iu), _lib.ptr(ju), npair, _lib.ptr(tc),
+                                             _lib.ptr(td), n, kp, 1, _lib.ptr(planes.hi), _lib.ptr(planes.lo),
+                                             _lib.current_stream_ptr()), "ccsd_vvvv_pm_pack")
+            Xpm = mfma.gemm_nt(planes, W, ksplit=mfma.tuned_ksplit(planes, W))
+            LOWP_PRODUCTS["bf16x3"] += 1
+        else:
+            Apm = torch.empty((2, npair, n), dtype=torch.float64, device=l2.device)
+            _lib.check(lib.ccsd_vvvv_pm_pack(_lib.ptr(l2c), o, v, _lib.ptr(iu), _lib.ptr(ju), npair, _lib.ptr(tc),
+                                             _lib.ptr(td), n, n, 0, _lib.ptr(Apm), None,
+                                             _lib.current_stream_ptr()), "ccsd_vvvv_pm_pack")
+            if precision == "fp64":
+                Xpm = torch.stack((_fp64_matmul_ksplit(Apm[0], W[0]), _fp64_matmul_ksplit(Apm[1], W[1])))
+            else:
+                Xpm = torch.bmm(Apm.float(), W).double()
+                LOWP_PRODUCTS["fp32"] += 1
+        out = torch.empty_like(l2c)
+        _lib.check(lib.ccsd_vvvv_pm_unpack(_lib.ptr(Xpm), o, v, Xpm.shape[-1], float(scale), _lib.ptr(out),
+                                           _lib.current_stream_ptr()), "ccsd_vvvv_pm_unpack")
+        return out
+    # host form of the same steps (torch ops)
+    A = l2[iu, ju].reshape(-1, v * v)
+    x, y = A[:, cd], A[:, dc]
+    Apm = torch.stack(((x + y) * wd, x - y))
     if precision == "fp64":
-        # B[(a, b), (c, d)] = (ac|bd) is symmetric: X = A B (DGEMM on the cached matrix)
-        B = _cached(vvvv, ("vvvv-abcd-f64",), lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).contiguous())
-        X = _fp64_matmul_ksplit(A[0], B)
-    elif precision == "bf16x3":
-        B = _cached(vvvv, ("vvvv-abcd",), lambda: mfma.split(vvvv.permute(0, 2, 1, 3).reshape(1, v * v, v * v)))
-        As = mfma.split(A)
-        X = mfma.gemm_nt(As, B, ksplit=mfma.tuned_ksplit(As, B))[0]
-        LOWP_PRODUCTS["bf16x3"] += 1
+        Xpm = torch.stack((Apm[0] @ W[0], Apm[1] @ W[1]))
+    elif precision == "fp32":
+        Xpm = torch.bmm(Apm.float(), W).double()
     else:
-        B = _cached(vvvv, ("vvvv-abcd-f32",),
-                    lambda: vvvv.permute(0, 2, 1, 3).reshape(v * v, v * v).float().contiguous())
-        X = (A[0].float() @ B).double()
-        LOWP_PRODUCTS["fp32"] += 1
-    X = X.reshape(-1, v, v)
-    if X.is_cuda:
-        from .pairops import unpack_pairs
-        return unpack_pairs(X.contiguous(), iu, ju, o, scale)
+        raise ValueError(f"precision {precision!r} needs the device")
+    X = ((Xpm[0][:, pidx] + Xpm[1][:, pidx] * sign) * 0.5).reshape(-1, v, v)
     out = torch.empty_like(l2)
     out[iu, ju] = X * scale
     out[ju, iu] = X.transpose(1, 2) * scale

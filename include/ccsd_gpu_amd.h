@@ -217,6 +217,27 @@ int ccsd_l1_terms(const double* x, int o, int v, const double* y, long long sy0,
 int ccsd_unpack_pairs(const double* x, const long long* iu, const long long* ju, int o, int v, long long npair,
                       double scale, double* out, void* stream);
 
+/* The l2 (ac|bd) term over index pairs, packing step: for the i <= j pair blocks
+ * (iu[r], ju[r]) (device int64, r < npair) of l2 (o, o, v, v) and the index pairs
+ * (tc[P], td[P]), tc >= td (device int64, P < np = v(v+1)/2), writes
+ * A+[r,P] = (l2[i,j,c,d] + l2[i,j,d,c]) (halved when c = d) and A-[r,P] = l2[i,j,c,d] - l2[i,j,d,c]
+ * as (2, npair, ld) rows, columns np..ld-1 zero: float64 into out (split 0) or
+ * split-BF16 hi / lo planes into out / lo (split 1). With W+-[Q,P] = (ac|bd) +- (ad|bc)
+ * the products A+- W+-^T hold the symmetric and antisymmetric halves of
+ * sum_cd l2[i,j,c,d] (ac|bd) at a quarter of the o^2 v^4 work of the reference's
+ * contraction (ccsd_gpu/gpu_ccsd_lambda.py:238-243). */
+int ccsd_vvvv_pm_pack(const double* l2, int o, int v, const long long* iu, const long long* ju, long long npair,
+                      const long long* tc, const long long* td, long long np, long long ld, int split, void* out,
+                      void* lo, void* stream);
+
+/* Unpacking step: from x = (X+, X-) (2, o(o+1)/2, ldx) float64, the products of the
+ * packing step for all i <= j pairs in triu order, writes out (o, o, v, v):
+ * out[i,j,a,b] = scale (X+[Q] + s X-[Q]) / 2 of the pair (i, j) for i < j and of
+ * (j, i) with (a, b) swapped for i > j, Q = (max(a,b), min(a,b)) packed as
+ * max(max+1)/2 + min, s = +1 when the first index is the larger; on i = j the
+ * symmetrised block scale X+[Q] / 2. */
+int ccsd_vvvv_pm_unpack(const double* x, int o, int v, long long ldx, double scale, double* out, void* stream);
+
 /* out[i,j,a,b] = beta * out + (c1 x[i,j,a,b] + c2 x[i,j,b,a] + c3 x[j,i,a,b] + c4 x[j,i,b,a]) * d[i,j,a,b]
  * for x and out any strided (o, o, v, v) views (host arrays of the four element
  * strides of i, j, a, b; reads and writes are coalesced when the b strides are

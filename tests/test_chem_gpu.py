@@ -142,8 +142,11 @@ def test_bf16x3_nt_scatter_destination_variants(gpu, gemm_variant, variant):
 
 
 @pytest.mark.gpu
-def test_vvvv_term_symmetric_half_matches_full(gpu):
-    from ccsd_gpu_amd.precision import static_operands, vvvv_term
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_vvvv_term_pair_halves_match_full(gpu, precision, tol):
+    """The l2.vvvv term through the symmetric / antisymmetric pair products against
+    the full einsum, twice (the second call reads the cached pair matrices)."""
+    from ccsd_gpu_amd.precision import vvvv_term
     gen = torch.Generator(device="cuda").manual_seed(3)
     o, v = 6, 11
     x = torch.randn((v, v, v, v), dtype=torch.float64, device="cuda", generator=gen)
@@ -153,10 +156,11 @@ def test_vvvv_term_symmetric_half_matches_full(gpu):
     l = torch.randn((o, o, v, v), dtype=torch.float64, device="cuda", generator=gen)
     l2 = l + l.permute(1, 0, 3, 2)
     ref = torch.einsum("ijcd,acbd->ijab", l2, vvvv)
-    with static_operands(vvvv):
-        for _ in range(2):
-            got = vvvv_term(l2, vvvv, "bf16x3")
-            assert float((got - ref).abs().max() / ref.abs().max()) < 1e-4
+    for _ in range(2):
+        got = vvvv_term(l2, vvvv, precision, scale=0.5)
+        err = float((got - 0.5 * ref).abs().max() / ref.abs().max())
+        print(precision, err, flush=True)
+        assert err < tol
 
 
 @pytest.mark.gpu

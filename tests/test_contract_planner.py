@@ -185,3 +185,20 @@ def test_pack_pairs_sym_symmetrises_and_packs():
     ref = sym[k, l].clone()
     ref[k == l] *= 0.5
     assert torch.allclose(_pack_pairs_sym(t), ref, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("o,v", [(1, 1), (3, 5), (4, 9)])
+def test_vvvv_pair_halves_match_einsum_cpu(o, v):
+    """The symmetric / antisymmetric pair form of the l2.vvvv term (host logic: pair
+    maps, pair matrices, recombination) against the full contraction, FP64 on the CPU."""
+    from ccsd_gpu_amd.precision import vvvv_term
+    g = torch.Generator().manual_seed(o * 10 + v)
+    x = torch.randn((v, v, v, v), dtype=torch.float64, generator=g)
+    vvvv = x + x.permute(1, 0, 2, 3)
+    vvvv = vvvv + vvvv.permute(0, 1, 3, 2)
+    vvvv = vvvv + vvvv.permute(2, 3, 0, 1)
+    t = torch.randn((o, o, v, v), dtype=torch.float64, generator=g)
+    l2 = t + t.permute(1, 0, 3, 2)
+    ref = torch.einsum("ijcd,acbd->ijab", l2, vvvv)
+    got = vvvv_term(l2, vvvv, "fp64", scale=0.5)
+    assert torch.allclose(got, 0.5 * ref, atol=1e-12 * float(ref.abs().max()), rtol=0)
