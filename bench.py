@@ -456,6 +456,8 @@ def bench_grad(args):
     torch.cuda.synchronize()
     log(f"setup (RHF {mf.e_tot:.10f}, CCSD corr {mycc.e_corr:.10f}) {time.perf_counter() - t0:.1f} s")
     res = []
+    first = None      # the first gradient on this plan and CCSD object (cold: kernel choices timed,
+    #                   lambda update graph captured, vvvv pair matrices built), when warm-up ran
     # two small fills (12345 int16 elements, a kernel nothing else launches) bracket each
     # timed step outside its timed region, so a kernel trace of this run can cut out one
     # mixed gradient
@@ -471,6 +473,8 @@ def bench_grad(args):
         log("lambda history: " + " ".join(f"{p[:2]}{'c' if c else ''}:{n:.1e}" for p, c, n in hist))
         if s >= args.warmup:
             res.append((de, info))
+        elif s == 0:
+            first = info
     wall = float(np.mean([i["total_s"] for _, i in res]))
     de_mixed = res[-1][0]
     # the plain path warmed once too (its first call builds the per-block work lists)
@@ -513,6 +517,9 @@ def bench_grad(args):
                    "speedup_mixed_vs_plain": info_plain["total_s"] / wall,
                    "max_abs_diff_mixed_vs_plain_Eh_per_Bohr": diff,
                    "net_force": net_force, "net_torque": torque, "accuracy_gate_1e-6_ok": gate_ok,
+                   "first_gradient_cold": None if first is None else {
+                       "total_s": first["total_s"], "lambda_phases_s": first["lambda_phases_s"],
+                       "phases_s": {k: round(v, 4) for k, v in first["phases_s"].items()}},
                    "outside_timed_region_s": {k: (round(v, 3) if isinstance(v, float) else v)
                                               for k, v in setup_s.items()},
                    "ao_eris_note": "the Cartesian AO ERIs are computed once (int2e, register store kernels) "

@@ -242,6 +242,7 @@ struct ccsd_plan {
     bool concurrent = true;
     bool ket_shell_order = false;                    // unscreened kets in shell order inside a primitive group
     int force_split = 0;                             // primitive split of the register kernels (0: by wave count)
+    bool tune_generic = false;                       // time the generic kernel for register-capable classes too
     std::vector<int> eff;                            // surviving primitive pairs of same-molecule shell pairs
     std::vector<long long> mol_eff_off;
     std::vector<C2sTermHost> c2s;                    // per spherical AO
@@ -685,9 +686,13 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
         }
     }
     // generic kernel: runs of 1, 4 or 16 quartets per wave; work lists kept below
-    // 4M entries (the bound counts every same-molecule quartet, screened or not)
+    // 4M entries (the bound counts every same-molecule quartet, screened or not).
+    // Where register candidates exist they are the only ones timed unless
+    // tune_generic is set: building the generic work lists dominated the first pass
+    // of a plan, and the register kernels win those classes.
     const long long upper = (long long)a.nbra * a.nket;
     for (int ch : {1, 4, 16}) {
+        if (reg && !cand.empty() && !p->tune_generic) break;
         if (upper / ch > 4000000 && ch != 16) continue;
         Choice c;
         c.kind = 0; c.wpb = 1; c.chunk = ch;
@@ -1725,6 +1730,13 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
     if (option == 5) {
         p->persistent = value != 0;
         clear_bra_cache(p);
+        return CCSD_OK;
+    }
+    if (option == 6) {
+        p->tune_generic = value != 0;
+        clear_bra_cache(p);
+        p->choice.clear();
+        p->choice_lds.clear();
         return CCSD_OK;
     }
     return fail(CCSD_ERR_SHAPE, "unknown option");

@@ -81,7 +81,10 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * persistent launch of the LDS kernel over every class (waves draw work items,
  * runs of one class's chunks ordered by decreasing cost, from a device counter
  * until the list is exhausted; chunk length per class from the autotuner), 0
- * (default) one launch per class as above. */
+ * (default) one launch per class as above. option 6: 0 (default) times only the
+ * register-kernel candidates of classes that have them (the generic kernel is
+ * timed for the other classes), 1 times the generic candidates everywhere (a
+ * slower first pass of a plan: their work lists are the largest). */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices

@@ -283,20 +283,43 @@ def grad_elec(mycc, t1, t2, l1, l2, release=None):
     return de
 
 
-def ccsd_gradient(mol, frozen=None, log=print):
-    """RHF -> CCSD -> lambda -> total dE/dR; returns (E_hf, E_corr, dE/dR)."""
+def ccsd_gradient(mol, frozen=None, log=print, checkpoint=None):
+    """RHF -> CCSD -> lambda -> total dE/dR; returns (E_hf, E_corr, dE/dR).
+
+    checkpoint: a file prefix; the converged amplitudes (t1, t2, E_corr) and lambdas
+    are saved to <prefix>_t.npz / <prefix>_l.npz as they are reached and read back
+    by a later call with the same prefix instead of being solved again (the RHF is
+    always rerun: it is deterministic and cheap next to the rest)."""
+    import os
     import time
     t0 = time.perf_counter()
     mf = RHF(mol)
     mf.kernel()
     log(f"RHF {mf.e_tot:.12f} ({time.perf_counter() - t0:.0f} s)")
     cc = CCSD(mf, frozen=frozen)
-    cc.kernel()
-    log(f"CCSD corr {cc.e_corr:.12f} ({time.perf_counter() - t0:.0f} s)")
+    tfile = lfile = None
+    if checkpoint:
+        tfile, lfile = checkpoint + "_t.npz", checkpoint + "_l.npz"
+    if tfile and os.path.exists(tfile):
+        with np.load(tfile) as z:
+            cc.t1, cc.t2, cc.e_corr = z["t1"], z["t2"], float(z["e_corr"])
+        log(f"CCSD corr {cc.e_corr:.12f} read from {tfile}")
+    else:
+        cc.kernel()
+        log(f"CCSD corr {cc.e_corr:.12f} ({time.perf_counter() - t0:.0f} s)")
+        if tfile:
+            np.savez(tfile, t1=cc.t1, t2=cc.t2, e_corr=cc.e_corr)
     eris = cc.ao2mo()
-    conv, l1, l2 = chem.solve_lambda(cc.t1, cc.t2, eris)
-    assert conv
-    log(f"lambda ({time.perf_counter() - t0:.0f} s)")
+    if lfile and os.path.exists(lfile):
+        with np.load(lfile) as z:
+            l1, l2 = z["l1"], z["l2"]
+        log(f"lambda read from {lfile}")
+    else:
+        conv, l1, l2 = chem.solve_lambda(cc.t1, cc.t2, eris)
+        assert conv
+        log(f"lambda ({time.perf_counter() - t0:.0f} s)")
+        if lfile:
+            np.savez(lfile, l1=l1, l2=l2)
     holder = [eris]
 
     def release():

@@ -1,6 +1,8 @@
 """Writes grad_golden.npz: FP64 oracle CCSD gradients of seeded water systems.
 
 Run in the build container (CPU only): python tests/golden/gen_grad_golden.py [names...]
+(GOLDEN_CHECKPOINT=<file prefix> keeps the large system's converged amplitudes and
+lambdas on disk, so an interrupted run resumes after its last finished stage).
 
 For each system the oracle (oracle/chem.py: RHF, CCSD, lambda, RDMs, Z-vector,
 gradient, restating ccsd_gpu/gpu_ccsd_lambda.py and gpu_ccsd_grad.py) runs from
@@ -43,7 +45,8 @@ def run(name):
     t0 = time.perf_counter()
     if name in LARGE:
         from oracle import chem_large
-        e_hf, e_corr, de = chem_large.ccsd_gradient(mol, log=lambda m: print(f"  {name}: {m}", flush=True))
+        e_hf, e_corr, de = chem_large.ccsd_gradient(mol, log=lambda m: print(f"  {name}: {m}", flush=True),
+                                                    checkpoint=os.environ.get("GOLDEN_CHECKPOINT"))
     else:
         e_tot, mf, cc = chem.ccsd_total_energy(mol)
         conv, l1, l2 = chem.solve_lambda(cc.t1, cc.t2, cc.ao2mo())
