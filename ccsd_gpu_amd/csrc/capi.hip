@@ -22,8 +22,9 @@ hipError_t ccsd_launch_eri_class(const EriClassArgs& a, hipStream_t stream);
 int ccsd_persistent_blocks_per_cu(size_t lds);
 hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
                                       unsigned int* counter, const double* gamma, const double* dm,
-                                      const double* pm, double* de, double scale, int gamma_mode, int nblocks,
-                                      size_t lds, hipStream_t stream);
+                                      const double* pm, unsigned long long* defx, double scale, int gamma_mode,
+                                      int nblocks, size_t lds, hipStream_t stream);
+hipError_t ccsd_launch_defx_to_de(const unsigned long long* defx, int natm, double* de, hipStream_t stream);
 hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const void* tall, int nfi_s, int nao_s,
                            long long npair_s, int nao_c, long long npair_c, int nfi_c, int ncomp, hipStream_t stream);
 hipError_t ccsd_launch_rys(int n, const double* T, int count, double* r, double* w, hipStream_t stream);
@@ -232,7 +233,8 @@ struct ccsd_plan {
     std::vector<double> q;                           // nbas x nbas, symmetric
     long long last_quartets = 0;
     std::vector<std::array<long long, 10>> last_launches;   // see ccsd_plan_last_launches
-    double* d_de_scratch = nullptr;                  // autotuner's throw-away gradient
+    unsigned long long* d_de_scratch = nullptr;      // autotuner's throw-away gradient (fixed point)
+    unsigned long long* d_defx = nullptr;            // fixed-point gradient of a fused pass (1 + 6 natm words)
     std::vector<double> coords_h;                    // natm x 3
     // side streams (default priority) that run independent class launches of a fused pass
     // concurrently; forked from and joined back to the caller's stream with events
@@ -700,7 +702,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
     }
     Choice best = cand[0];
     if (p->autotune && cand.size() > 1) {
-        if (!p->d_de_scratch) HIP_TRY(hipMalloc(&p->d_de_scratch, sizeof(double) * 3 * p->natm));
+        if (!p->d_de_scratch) HIP_TRY(hipMalloc(&p->d_de_scratch, sizeof(unsigned long long) * (1 + 6 * p->natm)));
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
@@ -715,7 +717,7 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             EriClassArgs t = a;
             t.chunks = cl->d;
             t.nchunks = nsample;
-            t.de = p->d_de_scratch;
+            t.defx = p->d_de_scratch;
             HIP_TRY(hipEventRecord(e0, stream));
             rc = launch_choice(t, c, stream);
             if (rc) return rc;
@@ -947,6 +949,15 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
     return CCSD_OK;
 }
 
+// Zeroed fixed-point gradient words of a fused pass (allocated on first use).
+int defx_begin(ccsd_plan* p, hipStream_t stream)
+{
+    const long long n = 1 + 6LL * p->natm;
+    if (!p->d_defx) HIP_TRY(hipMalloc(&p->d_defx, sizeof(unsigned long long) * n));
+    HIP_TRY(ccsd_launch_fill((double*)p->d_defx, n, 0.0, stream));   // all-zero bits
+    return CCSD_OK;
+}
+
 // The fused pass as one launch of the persistent kernel (plan option 5).
 int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
                          const double* gamma, const double* dm, const double* pm, hipStream_t stream)
@@ -975,9 +986,12 @@ int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* d
     p->last_quartets = pp.quartets;
     p->last_launches = pp.launches;
     if (pp.nitems == 0) return CCSD_OK;
+    int rc = defx_begin(p, stream);
+    if (rc) return rc;
     HIP_TRY(ccsd_launch_fill((double*)pp.d_counter, 1, 0.0, stream));
-    HIP_TRY(ccsd_launch_eri_persistent(pp.d_cls, pp.d_items, pp.nitems, pp.d_counter, gamma, dm, pm, de, scale,
-                                       gamma_mode, pp.nblocks, pp.lds, stream));
+    HIP_TRY(ccsd_launch_eri_persistent(pp.d_cls, pp.d_items, pp.nitems, pp.d_counter, gamma, dm, pm, p->d_defx,
+                                       scale, gamma_mode, pp.nblocks, pp.lds, stream));
+    HIP_TRY(ccsd_launch_defx_to_de(p->d_defx, p->natm, de, stream));
     return CCSD_OK;
 }
 
@@ -989,7 +1003,8 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
     p->last_quartets = 0;
     p->last_launches.clear();
     // kernel choices first (the autotuner synchronises), then the launches, spread over
-    // the side streams: they only add into de with atomics, so their order is free
+    // the side streams: they only add into the fixed-point gradient with integer atomics,
+    // so neither their order nor the result depends on the streams
     for (int bc = 0; bc < (int)bs.cls.size(); bc++)
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
@@ -1002,6 +1017,10 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c);
             if (rc) return rc;
         }
+    {
+        int rc = defx_begin(p, stream);
+        if (rc) return rc;
+    }
     const bool conc = p->concurrent;
     if (conc) {
         int rc = ensure_side_streams(p);
@@ -1026,7 +1045,7 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             if (rc) return rc;
             a.chunks = cl->d;
             a.nchunks = cl->n;
-            a.de = de;
+            a.defx = p->d_defx;
             if (cl->n == 0) continue;
             hipStream_t st = conc ? p->side[nl % ccsd_plan::kSideStreams] : stream;
             nl++;
@@ -1042,6 +1061,7 @@ int run_fused(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double s
             HIP_TRY(hipStreamWaitEvent(stream, p->join_ev[i], 0));
         }
     }
+    HIP_TRY(ccsd_launch_defx_to_de(p->d_defx, p->natm, de, stream));
     return CCSD_OK;
 }
 
@@ -1263,6 +1283,7 @@ int ccsd_plan_destroy_impl(ccsd_plan* p)
     if (p->d_mol_nao) (void)hipFree(p->d_mol_nao);
     if (p->d_mol_dm_off) (void)hipFree(p->d_mol_dm_off);
     if (p->d_de_scratch) (void)hipFree(p->d_de_scratch);
+    if (p->d_defx) (void)hipFree(p->d_defx);
     for (int i = 0; i < ccsd_plan::kSideStreams; i++) {
         if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
         if (p->join_ev[i]) (void)hipEventDestroy(p->join_ev[i]);

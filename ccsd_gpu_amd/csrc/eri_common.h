@@ -57,9 +57,11 @@ struct EriClassArgs {
     int i_base, nao;
     long long npair;
     long long comp_stride;
-    // fused gradient (deriv == 1, out == nullptr): de[atom_i][3] += scale * sum ip1 * gamma
+    // fused gradient (deriv == 1, out == nullptr): dE/dR[atom_i][3] += scale * sum ip1 * gamma,
+    // accumulated in fixed point (defx_add below) so that the sum is independent of the
+    // order in which waves finish
     const double* gamma;       // Cartesian (nao, nao, npair) packed like the output
-    double* de;
+    unsigned long long* defx;  // overflow count, then [natm][3][2] fixed-point words
     double gamma_scale;
     // work list
     const int4* chunks;        // (bra index, ket begin, ket end, unused)
@@ -96,6 +98,35 @@ struct GammaSrc {
     const double* D;
     const double* P;
 };
+
+// Fixed-point gradient accumulation. A contribution x is held as two 64-bit integers,
+// h = rint(x 2^20) and l = rint((x 2^20 - h) 2^40), i.e. x to 2^-60; both are added with
+// integer atomics, which are exact and associative, so the accumulated sums (and the
+// gradient formed from them once the pass ends) are the same bit for bit whatever the
+// order of the additions -- unlike float64 atomics. Contributions must stay below 2^40
+// in magnitude (gradient terms are many orders smaller); a larger or non-finite one is
+// counted in the overflow word, and the conversion then returns NaN.
+__device__ __forceinline__ void defx_add1(unsigned long long* w, unsigned long long* overflow, double x)
+{
+    const double s = x * 1048576.0;                 // x 2^20
+    if (!(fabs(s) < 1.152921504606847e18)) {        // |x| < 2^40, false for NaN
+        atomicAdd(overflow, 1ull);
+        return;
+    }
+    const double h = rint(s);
+    const double l = rint((s - h) * 1099511627776.0);   // the remainder (exact) times 2^40
+    atomicAdd(w, (unsigned long long)(long long)h);
+    atomicAdd(w + 1, (unsigned long long)(long long)l);
+}
+
+__device__ __forceinline__ void defx_add(unsigned long long* defx, int atom, double x, double y, double z)
+{
+    unsigned long long* w = defx + 1 + 6 * atom;
+    unsigned long long* ov = defx;
+    defx_add1(w, ov, x);
+    defx_add1(w + 2, ov, y);
+    defx_add1(w + 4, ov, z);
+}
 
 __device__ __forceinline__ GammaSrc gamma_src(const EriClassArgs& a, int mol)
 {

@@ -341,15 +341,13 @@ __global__ void __launch_bounds__(64) eri_quartet_kernel(const EriClassArgs a)
         }
         if (threadIdx.x == 0) {
             const int atom = a.shells[a.bra[ch.x].x].atom;
-            atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * gdx);
-            atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * gdy);
-            atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * gdz);
+            defx_add(a.defx, atom, a.gamma_scale * gdx, a.gamma_scale * gdy, a.gamma_scale * gdz);
         }
     }
 }
 
-// Wave reduction of a chunk's three derivative sums and one atomic add per component
-// into dE/dR of the bra pair's first atom (all lanes call it; lane 0 adds).
+// Wave reduction of a chunk's three derivative sums, added (fixed point) into dE/dR of
+// the bra pair's first atom (all lanes call it; lane 0 adds).
 __device__ __forceinline__ void add_chunk_gradient(const EriClassArgs& a, int bq, double gdx, double gdy, double gdz)
 {
     for (int off = 32; off > 0; off >>= 1) {
@@ -359,9 +357,7 @@ __device__ __forceinline__ void add_chunk_gradient(const EriClassArgs& a, int bq
     }
     if (threadIdx.x == 0) {
         const int atom = a.shells[a.bra[bq].x].atom;
-        atomicAdd(a.de + 3 * atom + 0, a.gamma_scale * gdx);
-        atomicAdd(a.de + 3 * atom + 1, a.gamma_scale * gdy);
-        atomicAdd(a.de + 3 * atom + 2, a.gamma_scale * gdz);
+        defx_add(a.defx, atom, a.gamma_scale * gdx, a.gamma_scale * gdy, a.gamma_scale * gdz);
     }
 }
 
@@ -370,7 +366,7 @@ struct PersistDyn {
     const double* gamma;
     const double* dm;
     const double* pm;
-    double* de;
+    unsigned long long* defx;
     double scale;
     int gamma_mode;
 };
@@ -398,7 +394,7 @@ __global__ void __launch_bounds__(64) eri_persistent_kernel(const EriClassArgs* 
         a.gamma = dyn.gamma;
         a.dm = dyn.dm;
         a.pm = dyn.pm;
-        a.de = dyn.de;
+        a.defx = dyn.defx;
         a.gamma_scale = dyn.scale;
         a.gamma_mode = dyn.gamma_mode;
         const LdsLayout L = lds_layout(a);
@@ -490,17 +486,38 @@ int ccsd_persistent_blocks_per_cu(size_t lds)
     return nb;
 }
 
+namespace ccsd {
+// de[3 natm] += the fixed-point sums of defx (h 2^-20 + l 2^-60 per component); NaN
+// everywhere when any contribution overflowed.
+__global__ void defx_to_de_kernel(const unsigned long long* __restrict__ defx, int n3, double* __restrict__ de)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n3) return;
+    const long long h = (long long)defx[1 + 2 * t], l = (long long)defx[2 + 2 * t];
+    const double v = (double)h * 9.5367431640625e-07 + (double)l * 8.673617379884035e-19;   // 2^-20, 2^-60
+    de[t] = defx[0] ? __builtin_nan("") : de[t] + v;
+}
+}  // namespace ccsd
+
+hipError_t ccsd_launch_defx_to_de(const unsigned long long* defx, int natm, double* de, hipStream_t stream)
+{
+    const int n3 = 3 * natm;
+    if (n3 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ccsd::defx_to_de_kernel, dim3((n3 + 255) / 256), dim3(256), 0, stream, defx, n3, de);
+    return hipGetLastError();
+}
+
 hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
                                       unsigned int* counter, const double* gamma, const double* dm,
-                                      const double* pm, double* de, double scale, int gamma_mode, int nblocks,
-                                      size_t lds, hipStream_t stream)
+                                      const double* pm, unsigned long long* defx, double scale, int gamma_mode,
+                                      int nblocks, size_t lds, hipStream_t stream)
 {
     if (nitems <= 0 || nblocks <= 0) return hipSuccess;
     ccsd::PersistDyn dyn;
     dyn.gamma = gamma;
     dyn.dm = dm;
     dyn.pm = pm;
-    dyn.de = de;
+    dyn.defx = defx;
     dyn.scale = scale;
     dyn.gamma_mode = gamma_mode;
     hipLaunchKernelGGL(ccsd::eri_persistent_kernel, dim3((unsigned)nblocks), dim3(64), lds, stream, classes, items,

@@ -424,11 +424,7 @@ __global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
             sy += __shfl_down(sy, off, 64);
             sz += __shfl_down(sz, off, 64);
         }
-        if (lane == 0) {
-            atomicAdd(a.de + 3 * si.atom + 0, a.gamma_scale * sx);
-            atomicAdd(a.de + 3 * si.atom + 1, a.gamma_scale * sy);
-            atomicAdd(a.de + 3 * si.atom + 2, a.gamma_scale * sz);
-        }
+        if (lane == 0) defx_add(a.defx, si.atom, a.gamma_scale * sx, a.gamma_scale * sy, a.gamma_scale * sz);
     }
 }
 

@@ -2,12 +2,13 @@
 //
 // The reference evaluates these on the CPU through PySCF (hcore_generator,
 // get_ovlp, int1e_ipovlp / ipkin / ipnuc / iprinv at ccsd_gpu/gpu_ccsd_grad.py:554-583).
-// One thread per (ordered shell pair, bra primitive) adds its part of the pair's
-// Cartesian block for overlap and kinetic (1-D Obara-Saika tables); the
-// nuclear-attraction kinds use one thread per (shell pair, bra primitive,
-// nucleus), Rys quadrature with point charges (the ERI tables, n <= 5 for l <= 4
-// plus one derivative). The parts of one element are added atomically (the
-// output must start at zero), so small bases still fill the device.
+// One thread per ordered shell pair sums the pair's Cartesian block over its
+// primitive pairs: 1-D Obara-Saika tables for overlap and kinetic, Rys quadrature
+// with point charges for the nuclear attraction (the ERI tables, n <= 5 for l <= 4
+// plus one derivative; kind 2 sums every nucleus in the same thread, kind 5 runs one
+// thread per (shell pair, nucleus), each nucleus having its own output). Every output
+// element is owned by one thread and summed in a fixed order (no atomics: the same
+// bits on every run); the output must start at zero.
 //
 // kinds: 0 <i|j>  1 <i|T|j>  2 <i|V|j>  3 <d/dr i|j>  4 <d/dr i|T|j>
 //        5 <d/dr i| -Z_C/|r-C| |j> for every nucleus C (natm x 3 matrices)
@@ -68,20 +69,20 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                              const double* __restrict__ env, const double* __restrict__ coords,
                              const double* __restrict__ charges, int natm, int nbas, int nao, int maxprim)
 {
-    // one thread per (shell pair, bra primitive[, nucleus for the nuclear-attraction
-    // kinds]); the threads of one element add atomically
-    const bool per_nucleus = kind == 2 || kind == 5;
+    // one thread per shell pair (and nucleus, for the nuclear-attraction derivative,
+    // whose nuclei have outputs of their own): each output element has one owner
+    // thread, which sums its primitive pairs (and, for kind 2, the nuclei) in a fixed
+    // order, so the result is the same bit for bit on every run
+    const bool per_nucleus = kind == 5;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long npair = (long long)nbas * nbas;
-    const long long nunit = npair * maxprim;
-    if (t >= (per_nucleus ? nunit * natm : nunit)) return;
+    if (t >= (per_nucleus ? npair * natm : npair)) return;
     const long long u = per_nucleus ? t / natm : t;
-    const int pair = (int)(u / maxprim), ip_only = (int)(u - (u / maxprim) * maxprim);
+    const int pair = (int)u;
     const int c_first = per_nucleus ? (int)(t - u * natm) : 0;
     const int c_last = per_nucleus ? c_first + 1 : natm;
     const int ish = pair / nbas, jsh = pair - ish * nbas;
     const ShellInfo si = shells[ish], sj = shells[jsh];
-    if (ip_only >= si.nprim) return;
     const int la = si.l, lb = sj.l;
     const int deriv = kind >= 3;
     const int lae = la + deriv;
@@ -92,7 +93,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
     const size_t nn = (size_t)nao * nao;
     double AB2 = 0.0;
     for (int x = 0; x < 3; x++) AB2 += (A[x] - B[x]) * (A[x] - B[x]);
-    for (int ip = ip_only; ip <= ip_only; ip++)
+    for (int ip = 0; ip < si.nprim; ip++)
         for (int jp = 0; jp < sj.nprim; jp++) {
             const double a = env[si.pexp + ip], b = env[sj.pexp + jp], p = a + b;
             const double K = exp(-a * b / p * AB2) * nrm;
@@ -131,7 +132,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                     const double v = (kind == 0 || kind == 3)
                                                          ? s[0] * s[1] * s[2]
                                                          : tk[0] * s[1] * s[2] + s[0] * tk[1] * s[2] + s[0] * s[1] * tk[2];
-                                    atomicAdd(out + c * nn + (size_t)(i0 + fa) * nao + j0 + fb, cc * v);
+                                    out[c * nn + (size_t)(i0 + fa) * nao + j0 + fb] += cc * v;
                                 }
                             }
                     } else {
@@ -155,7 +156,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                         comp1(lb, fb, cb);
                                         const size_t e = (size_t)(i0 + fa) * nao + j0 + fb;
                                         if (!deriv) {
-                                            atomicAdd(out + e, I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]]);
+                                            out[e] += I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]];
                                         } else {
                                             for (int c = 0; c < 3; c++) {
                                                 double f[3];
@@ -164,7 +165,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                                     f[x] = x == c ? -2 * a * I[x][i + 1][j] + (i > 0 ? i * I[x][i - 1][j] : 0.0)
                                                                   : I[x][i][j];
                                                 }
-                                                atomicAdd(out + (size_t)(C * 3 + c) * nn + e, f[0] * f[1] * f[2]);
+                                                out[(size_t)(C * 3 + c) * nn + e] += f[0] * f[1] * f[2];
                                             }
                                         }
                                     }
@@ -180,7 +181,8 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
 hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
                              const double* charges, int natm, int nbas, int nao, int maxprim, hipStream_t stream)
 {
-    const long long n = (long long)nbas * nbas * maxprim * ((kind == 2 || kind == 5) ? natm : 1);
+    (void)maxprim;
+    const long long n = (long long)nbas * nbas * (kind == 5 ? natm : 1);
     if (n <= 0) return hipSuccess;
     if ((n + 63) / 64 > 2147483647LL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(int1e_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, kind, out, shells, env,

@@ -423,3 +423,32 @@ def test_zvector_nonconvergence_is_flagged(gpu):
     assert mycc._zvector_converged is False and mycc._zvector_iterations == 1
     _response_dm1_gpu(mycc, x, mf._ints())
     assert mycc._zvector_converged is True
+
+
+# ----------------------------------------------------------------------------------
+# run twice, bitwise
+# ----------------------------------------------------------------------------------
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["plain", "mixed"])
+def test_gradient_run_twice_is_bitwise_identical(gpu, mode):
+    """The whole pipeline twice from scratch (integrals, RHF, CCSD, lambda, dE/dR) and
+    the gradient once more on the second setup give the same bits under
+    deterministic(): no step sums in an order that depends on scheduling (the fused
+    pass and the one-electron kernels accumulate without float atomics; torch's
+    library calls take their deterministic forms)."""
+    import warnings
+
+    from ccsd_gpu_amd import pipeline
+    from ccsd_gpu_amd.determinism import deterministic
+    mol = systems.water_cluster(2, seed=3, basis="cc-pvdz")
+    with deterministic(), warnings.catch_warnings(record=True) as caught:
+        warnings.simplefilter("always")
+        mf1, cc1, de1, _ = _pipeline_gradient(mol, mode)
+        mf2, cc2, de2, _ = _pipeline_gradient(mol, mode)
+        de3, _ = pipeline.ccsd_gradient(mf2, cc2, mode, "bf16x3")
+    flagged = sorted({str(w.message)[:120] for w in caught if "deterministic" in str(w.message)})
+    print("torch operations without a deterministic form:", flagged, flush=True)
+    assert not flagged
+    print(mode, "E", mf1.e_tot, mf2.e_tot, "max |d1 - d2|", np.abs(de1 - de2).max(), flush=True)
+    assert mf1.e_tot == mf2.e_tot and cc1.e_corr == cc2.e_corr
+    assert np.array_equal(de1, de2) and np.array_equal(de2, de3)

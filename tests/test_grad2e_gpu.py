@@ -205,3 +205,31 @@ def test_add_separable_then_explicit_pass_equals_in_kernel_separable(gpu, make):
     scale = np.abs(ref).max()
     print(make.__name__, "add_separable + explicit vs in-kernel separable", np.abs(got - ref).max() / scale, flush=True)
     assert np.abs(got - ref).max() <= 1e-11 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("make", [molecules.h2o_ccpvdz, molecules.ch4_ccpvdz])
+def test_fused_gradient_is_bitwise_reproducible(gpu, make):
+    """The fused pass adds into dE/dR through exact fixed-point integer atomics: on one
+    plan (kernel choices fixed) repeated passes, on four side streams or on one
+    stream, give the same bits; the persistent single launch is reproducible too."""
+    from ccsd_gpu_amd import _lib
+    mol = make()
+    g_t = torch.as_tensor(_random_gamma(mol, 31), device="cuda")
+    plan = _lib.get_plan(mol)
+    lib = _lib.load()
+    runs = [contract_int2e_ip1(mol, g_t, scale=-2.0) for _ in range(3)]
+    try:
+        _lib.check(lib.ccsd_plan_set_option(plan, 2, 0), "set_option")     # one stream
+        runs.append(contract_int2e_ip1(mol, g_t, scale=-2.0))
+        _lib.check(lib.ccsd_plan_set_option(plan, 5, 1), "set_option")     # persistent launch
+        pers = [contract_int2e_ip1(mol, g_t, scale=-2.0) for _ in range(2)]
+    finally:
+        lib.ccsd_plan_set_option(plan, 2, 1)
+        lib.ccsd_plan_set_option(plan, 5, 0)
+    for r in runs[1:]:
+        assert torch.equal(r, runs[0])
+    assert torch.equal(pers[0], pers[1])
+    scale = float(runs[0].abs().max())
+    print("persistent vs per-class", float((pers[0] - runs[0]).abs().max()) / scale, flush=True)
+    assert float((pers[0] - runs[0]).abs().max()) <= 1e-11 * scale
