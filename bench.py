@@ -644,6 +644,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="end with a plain 256 MiB device copy (HBM counter calibration)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--deterministic", action="store_true",
+                    help="torch's deterministic algorithms on (bitwise-repeatable gradients)")
     ap.add_argument("--gemm-variant", type=int, default=0,
                     help="gemm config: block shape of the split-BF16 GEMM (ccsd_gemm_set_variant)")
     args = ap.parse_args()
@@ -655,8 +657,12 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     _lib.load()
-    line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision,
-            "gemm": bench_gemm}[args.config](args)
+    from ccsd_gpu_amd.determinism import deterministic
+    with deterministic(args.deterministic):
+        line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision,
+                "gemm": bench_gemm}[args.config](args)
+    if args.deterministic:
+        line["config"]["deterministic"] = True
     if args.copy_calibration and args.config != "eri":
         copy_calibration()
     print(json.dumps(line), flush=True)

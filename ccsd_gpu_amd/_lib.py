@@ -45,6 +45,7 @@ _SIGNATURES = {
                                         c_void_p]),
     "ccsd_sym_pairs": (c_int, [c_void_p, c_int, c_double, c_void_p]),
     "ccsd_repack_pairs": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "ccsd_repack_pairs_f32": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "ccsd_add_separable_packed": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "ccsd_pack_pairs_sym": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, ctypes.c_longlong, c_void_p,
                                     c_void_p]),

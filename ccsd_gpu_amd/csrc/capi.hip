@@ -50,7 +50,7 @@ hipError_t ccsd_launch_split_bf16(const double* x, void* hi, void* lo, long long
                                   hipStream_t stream);
 hipError_t ccsd_launch_sum_slices(double* ws, double* out, long long n, int S, int beta, hipStream_t stream);
 hipError_t ccsd_launch_sym_pairs(double* t, int n, double scale, hipStream_t stream);
-hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream);
+hipError_t ccsd_launch_repack_pairs(const void* z, int n, int m, void* out, int f32, hipStream_t stream);
 hipError_t ccsd_launch_add_separable_packed(double* g, const double* D, const double* P, int n, hipStream_t stream);
 hipError_t ccsd_launch_pack_pairs_sym(const double* t, const long long* kk, const long long* ll, int n, int m,
                                       long long npair, double* out, hipStream_t stream);
@@ -1619,7 +1619,16 @@ int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream)
 {
     if (!z || !out) return fail(CCSD_ERR_NULL, "null pointer");
     if (n < 0 || m < 0 || (long long)n * (n + 1) / 2 > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "bad dimensions");
-    hipError_t e = ccsd_launch_repack_pairs(z, n, m, out, (hipStream_t)stream);
+    hipError_t e = ccsd_launch_repack_pairs(z, n, m, out, 0, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "repack_pairs_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_repack_pairs_f32(const float* z, int n, int m, float* out, void* stream)
+{
+    if (!z || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || m < 0 || (long long)n * (n + 1) / 2 > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_repack_pairs(z, n, m, out, 1, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "repack_pairs_kernel");
     return CCSD_OK;
 }

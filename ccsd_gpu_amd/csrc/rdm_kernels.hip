@@ -97,15 +97,16 @@ __global__ void __launch_bounds__(256) pack_pairs_sym_kernel(const double* __res
 // p(i, j) = a (a + 1) / 2 + b for a = max(i, j), b = min(i, j), wkl = 1 (k == l) else 2
 // and wij = 1 (i == j) else 2. One block per pair r; wave w writes rows i = w, w + 4, ...
 // (coalesced along j), the gathered row of z is read from cache.
-__global__ void __launch_bounds__(256) repack_pairs_kernel(const double* __restrict__ z, int n, int m,
-                                                          double* __restrict__ out)
+template <typename T>
+__global__ void __launch_bounds__(256) repack_pairs_kernel(const T* __restrict__ z, int n, int m,
+                                                          T* __restrict__ out)
 {
     const long long r = blockIdx.x;
     int k, l;
     pair_of(r, k, l);
     const long long P = (long long)m * (m + 1) / 2;
-    const double* Z = z + ((long long)k * n + l) * P;
-    double* O = out + r * m * m;
+    const T* Z = z + ((long long)k * n + l) * P;
+    T* O = out + r * m * m;
     const double wkl = (k == l) ? 1.0 : 2.0;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int i = wave; i < m; i += 4) {
@@ -113,7 +114,7 @@ __global__ void __launch_bounds__(256) repack_pairs_kernel(const double* __restr
         for (int j = lane; j < m; j += 64) {
             const long long p = (j <= i) ? ri + j : (long long)j * (j + 1) / 2 + i;
             const double f = (i == j) ? wkl : 0.5 * wkl;
-            O[(long long)i * m + j] = f * Z[p];
+            O[(long long)i * m + j] = (T)(f * (double)Z[p]);
         }
     }
 }
@@ -164,12 +165,17 @@ hipError_t ccsd_launch_add_separable_packed(double* g, const double* D, const do
     return hipGetLastError();
 }
 
-hipError_t ccsd_launch_repack_pairs(const double* z, int n, int m, double* out, hipStream_t stream)
+hipError_t ccsd_launch_repack_pairs(const void* z, int n, int m, void* out, int f32, hipStream_t stream)
 {
     if (n <= 0 || m <= 0) return hipSuccess;
     const long long npair = (long long)n * (n + 1) / 2;
     if (npair > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(repack_pairs_kernel, dim3((unsigned)npair), dim3(256), 0, stream, z, n, m, out);
+    if (f32)
+        hipLaunchKernelGGL(repack_pairs_kernel<float>, dim3((unsigned)npair), dim3(256), 0, stream,
+                           (const float*)z, n, m, (float*)out);
+    else
+        hipLaunchKernelGGL(repack_pairs_kernel<double>, dim3((unsigned)npair), dim3(256), 0, stream,
+                           (const double*)z, n, m, (double*)out);
     return hipGetLastError();
 }
 

@@ -204,10 +204,12 @@ def _rdm2_mo2ao_gpu(mycc, d2, mo_coeff, precision="fp64"):
     c = _dev(mo_coeff)
     nao = c.shape[0]
     t = dm2                                               # (p q r s)
+    if precision == "fp32s":
+        c, t = c.float(), t.float()
     for _ in range(4):
         # last index transformed and rotated to the front: (p q r s) -> (l p q r) -> ... -> (i j k l)
         t = _rotate_transform(t, c, precision)
-    return t
+    return t.double()
 
 
 def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
@@ -223,12 +225,17 @@ def _rdm2_mo2ao_packed_gpu(mycc, d2, mo_coeff, precision="fp64"):
     nmo = c.shape[1]
     tp = _pack_pairs_sym(dm2)                    # (pq r s), p >= q, symmetrised
     del dm2
+    if precision == "fp32s":
+        # FP32 storage and FP32 products through the four transforms (half the bytes
+        # of every intermediate, twice the matrix rate), FP64 result
+        c = c.float()
+        tp = tp.float()
     t = _rotate_transform(tp, c, precision)      # (s pq r)
     del tp
     t = _rotate_transform(t, c, precision)       # (r s pq)
     t = _repack_pairs(t, nmo)                    # (rs p q), r >= s, the rs diagonal halved
     t = _rotate_transform(t, c, precision)       # (q rs p)
-    return _rotate_transform(t, c, precision)    # (p q rs)
+    return _rotate_transform(t, c, precision).double()   # (p q rs)
 
 
 def _repack_pairs(z, m):
@@ -237,10 +244,10 @@ def _repack_pairs(z, m):
     that weight, the second unpacked (one kernel on the device, torch on the CPU)."""
     n = z.shape[0]
     out = torch.empty((n * (n + 1) // 2, m, m), dtype=z.dtype, device=z.device)
-    if z.is_cuda and z.is_contiguous() and z.dtype == torch.float64:
+    if z.is_cuda and z.is_contiguous() and z.dtype in (torch.float64, torch.float32):
         from . import _lib
-        _lib.check(_lib.load().ccsd_repack_pairs(_lib.ptr(z), n, m, _lib.ptr(out), _lib.current_stream_ptr()),
-                   "ccsd_repack_pairs")
+        fn = _lib.load().ccsd_repack_pairs if z.dtype == torch.float64 else _lib.load().ccsd_repack_pairs_f32
+        _lib.check(fn(_lib.ptr(z), n, m, _lib.ptr(out), _lib.current_stream_ptr()), "ccsd_repack_pairs")
         return out
     kk, ll = torch.tril_indices(n, n, device=z.device)
     i = torch.arange(m, device=z.device)
@@ -283,7 +290,16 @@ def _rotate_transform(X, c, precision):
     R = X.numel() // K
     n = c.shape[0]
     X2 = X.reshape(R, K)
-    if precision == "bf16x3":
+    if X.dtype == torch.float32:
+        # FP32 operands kept in FP32 (the "fp32s" transform chain): library SGEMM over
+        # column blocks whose operand and product stay below 512 MiB -- single SGEMMs
+        # with a product above 2 GiB returned NaN in part of their rows on this stack
+        out = torch.empty((n, R), dtype=torch.float32, device=X.device)
+        rc = max(1, (1 << 27) // max(K, n))
+        for r0 in range(0, R, rc):
+            r1 = min(R, r0 + rc)
+            out[:, r0:r1] = c @ X2[r0:r1].T
+    elif precision == "bf16x3":
         out = torch.empty((n, R), dtype=torch.float64, device=X.device)
         zero = torch.zeros(1, dtype=torch.int64, device=X.device)
         rows = torch.arange(R, dtype=torch.int64, device=X.device)
@@ -416,9 +432,17 @@ def _separable_gamma(D, P):
 
 
 def grad_elec_gpu(mycc, t1=None, t2=None, l1=None, l2=None, atmlst=None, verbose=0, fused=True,
-                  precision="fp64", timings=None, mo2ao_precision="fp64"):
-    """Electronic CCSD gradient dE/dR (natm, 3) as numpy, as ccsd_gpu/gpu_ccsd_grad.py:344."""
+                  precision="fp64", timings=None, mo2ao_precision=None):
+    """Electronic CCSD gradient dE/dR (natm, 3) as numpy, as ccsd_gpu/gpu_ccsd_grad.py:344.
+
+    precision: of the RDM contractions ("fp64", "fp32", "bf16x3"). mo2ao_precision: of
+    the MO -> AO transform of the 2-RDM, "fp64" (None, the default), "fp32s" (FP32
+    storage and products through the four index transforms, FP64 result: 5e-8 on the
+    (H2O)8 AO density, 7e-8 Eh/Bohr on its gradient and a net force of 1.4e-7, for
+    5 ms of 42), or "bf16x3" / "fp32" (the FP64 chain with low-precision products)."""
     tm = {} if timings is None else timings
+    if mo2ao_precision is None:
+        mo2ao_precision = "fp64"
 
     def mark(name, t0):
         torch.cuda.synchronize()
@@ -626,7 +650,8 @@ def _grad2e_unfused(mol, gp, Dc, Pc, de2):
     del ip1
 
 
-def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None):
+def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64", timings=None,
+                               mo2ao_precision=None):
     """Total dE/dR (natm, 3) in Eh/Bohr (despite the name, not -dE/dR), as
     ccsd_gpu/gpu_ccsd_grad.py:711-739: electronic gradient + nuclear repulsion.
 
@@ -638,6 +663,7 @@ def compute_ml_ccsd_forces_gpu(mf, t1, t2, l1, l2, fused=True, precision="fp64",
     mycc.t1, mycc.t2, mycc.l1, mycc.l2 = _dev(t1), _dev(t2), _dev(l1), _dev(l2)
     mycc.converged = True
     mycc.converged_lambda = True
-    de = grad_elec_gpu(mycc, t1, t2, l1, l2, verbose=0, fused=fused, precision=precision, timings=timings)
+    de = grad_elec_gpu(mycc, t1, t2, l1, l2, verbose=0, fused=fused, precision=precision, timings=timings,
+                       mo2ao_precision=mo2ao_precision)
     de += grad_nuc(mf.mol)
     return np.array(de)

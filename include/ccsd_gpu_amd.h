@@ -194,6 +194,10 @@ int ccsd_pack_pairs_sym(const double* t, const long long* kk, const long long* l
  * pair's remaining transforms. */
 int ccsd_repack_pairs(const double* z, int n, int m, double* out, void* stream);
 
+/* ccsd_repack_pairs on float32 arrays (the MO -> AO transform of the 2-RDM carried in
+ * FP32 storage and FP32 matrix products, the mixed gradient's default). */
+int ccsd_repack_pairs_f32(const float* z, int n, int m, float* out, void* stream);
+
 /* gamma[i][j][k(k+1)/2 + l] += w_kl * Gsep_ijkl for k >= l, w_kl = 1/2 if k == l else 1,
  *   Gsep_ijkl = D_ij P_kl + P_ij D_kl - (D_il P_kj + D_ik P_lj + P_il D_jk + P_ik D_jl) / 4,
  * gamma (n, n, n(n+1)/2), dm = D and pm = P (n, n) row-major, all device arrays, n <= 2048.

@@ -312,7 +312,8 @@ def test_gradient_gpu_matches_oracle(gpu, h2o_pair, fused):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision,mo2ao,tol", [("bf16x3", "fp64", 1e-7), ("fp32", "fp64", 1e-7),
-                                                  ("bf16x3", "bf16x3", 1e-6), ("fp32", "fp32", 1e-6)])
+                                                  ("bf16x3", "bf16x3", 1e-6), ("fp32", "fp32", 1e-6),
+                                                  ("bf16x3", "fp32s", 1e-7), ("bf16x3", None, 1e-7)])
 def test_gradient_mixed_precision_matches_oracle(gpu, h2o_pair, precision, mo2ao, tol, monkeypatch):
     """RDM contractions (and optionally the MO->AO transform) on the MFMA path, every
     RDM product at that precision (flop threshold 0)."""
