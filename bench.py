@@ -28,15 +28,17 @@ quartets/s. config.persistent_single_launch compares, on the first
 class with the per-class launches.
 
 --config gemm: the split-BF16 NT GEMM of the lambda / RDM contractions on the
-(H2O)8 sweep shapes (random operands); value = algorithmic TFLOP/s (2 M N K per
-launch) of the largest product; the three bf16 products the kernel issues per
-split pair are reported apart (hardware_bf16_tflops).
+(H2O)8 sweep shapes (random operands), each beside the same plain bf16 products
+through the vendor BLAS; value = algorithmic TFLOP/s (2 batch M N K per launch)
+of the l2 (ac|bd) pair product; the three bf16 products the kernel issues per
+split pair are reported apart (hardware_bf16_tflops). --gemm-variant picks the
+kernel's block shape, --deterministic turns torch's deterministic algorithms on.
 
 Every line carries `roofline` (the dominant kernel, time from HIP events on the
 launch stream) and `cpu_baseline`. In the grad line the dominant kernel is the
 split-BF16 GEMM (most of the timed step's kernel time), on the lambda sweep's
-largest product (bf16 MFMA roofline, achieved = 2 M N K / time); the fused ERI'
-pass's FP64 roofline is in config.eri_prime_roofline, with its rate on the
+l2 (ac|bd) pair product (bf16 MFMA roofline, achieved = 2 batch M N K / time); the
+fused ERI' pass's FP64 roofline is in config.eri_prime_roofline, with its rate on the
 quartets it evaluates (and the reference-equivalent count beside it). The eri and
 batch lines use the ERI kernels' FP64 roofline. The grad line's cpu_baseline is
 the FP64 oracle timed on this host's cores on samples with exact multiplicities
@@ -494,8 +496,8 @@ def bench_grad(args):
     del gp
     roof, flops, quartets, refq = roofline(mol, kern_ms)
     # the lambda sweeps' largest product (l2 (ac|bd), i <= j) on the MFMA roofline
-    _, gm, gn, gk = LAMBDA_GEMMS[0]
-    gemm_ms = timed_gemm(gm, gn, gk, reps=3)
+    _, gm, gn, gk, gb = LAMBDA_GEMMS[0]
+    gemm_ms = timed_gemm(gm, gn, gk, reps=3, nb=gb)
     setup_s = getattr(mf, "_setup_times", {})
 
     cpu = None
@@ -532,27 +534,29 @@ def bench_grad(args):
                    "quartets_reference_equivalent": refq,
                    "eri_prime_pass_ms": kern_ms,
                    "eri_prime_roofline": roof,
-                   "lambda_gemm": {"shape_mnk": [gm, gn, gk], "ms": gemm_ms,
-                                   "hardware_bf16_tflops": 3 * 2.0 * gm * gn * gk / (gemm_ms * 1e-3) / 1e12}},
-        "roofline": gemm_roofline(gm, gn, gk, gemm_ms), "cpu_baseline": cpu}
+                   "lambda_gemm": {"shape_mnk": [gm, gn, gk], "batch": gb, "ms": gemm_ms,
+                                   "hardware_bf16_tflops": 3 * 2.0 * gb * gm * gn * gk / (gemm_ms * 1e-3) / 1e12}},
+        "roofline": gemm_roofline(gm, gn, gk, gemm_ms, gb), "cpu_baseline": cpu}
     return line
 
 
 BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA peak
 
-# the split-BF16 products of one (H2O)8 lambda sweep (nocc 40, nvir 152): (name, M, N, K)
-LAMBDA_GEMMS = [("l2 (ac|bd), i<=j", 820, 23104, 23104), ("ring terms (ov x ov)", 6080, 6080, 6080),
-                ("l2t1 ovvv", 1600, 23104, 6080), ("l2tau voov / woooo l2", 1600, 23104, 1600)]
+# the split-BF16 products of one (H2O)8 lambda sweep (nocc 40, nvir 152): (name, M, N, K, batch).
+# The l2 (ac|bd) term runs as its symmetric and antisymmetric pair halves, one batch of two
+# products over the v(v+1)/2 = 11628 index pairs (K padded to 11632 for 16-byte rows).
+LAMBDA_GEMMS = [("l2 (ac|bd) pair halves, i<=j", 820, 11628, 11632, 2), ("ring terms (ov x ov)", 6080, 6080, 6080, 1),
+                ("l2t1 ovvv", 1600, 23104, 6080, 1), ("l2tau voov / woooo l2", 1600, 23104, 1600, 1)]
 
 
-def timed_gemm(m, n, k, reps=5, seed=0):
-    """HIP-event time (ms) of one split-BF16 NT GEMM (no K split) on random planes."""
+def timed_gemm(m, n, k, reps=5, seed=0, nb=1):
+    """HIP-event time (ms) of one split-BF16 NT GEMM launch (batch nb, no K split) on random planes."""
     import torch
     from ccsd_gpu_amd import mfma
     gen = torch.Generator(device="cuda").manual_seed(seed)
-    A = mfma.split(torch.randn((1, m, k), dtype=torch.float64, device="cuda", generator=gen))
-    B = mfma.split(torch.randn((1, n, k), dtype=torch.float64, device="cuda", generator=gen))
-    out = torch.empty((1, m, n), dtype=torch.float64, device="cuda")
+    A = mfma.split(torch.randn((nb, m, k), dtype=torch.float64, device="cuda", generator=gen))
+    B = mfma.split(torch.randn((nb, n, k), dtype=torch.float64, device="cuda", generator=gen))
+    out = torch.empty((nb, m, n), dtype=torch.float64, device="cuda")
     stream = torch.cuda.current_stream()
     mfma.gemm_nt(A, B, out=out, ksplit=1)
     a = torch.cuda.Event(enable_timing=True)
@@ -587,17 +591,17 @@ def timed_vendor_bf16(m, n, k, reps=5, seed=0):
 GEMM_TRAFFIC = os.path.join(ROOT, "profiles", "gemm_bf16x3_traffic.json")
 
 
-def gemm_roofline(m, n, k, ms):
-    """MFMA roofline of a split-BF16 GEMM on its algorithmic work, 2 M N K per launch
+def gemm_roofline(m, n, k, ms, nb=1):
+    """MFMA roofline of a split-BF16 GEMM launch on its algorithmic work, 2 nb M N K
     (the three bf16 products the kernel issues per element pair are reported apart,
     as hardware_bf16_tflops). `traffic` is the HBM bytes per launch of this shape
     measured by rocprofv3 FETCH_SIZE + WRITE_SIZE (copy-calibrated) in the committed
     profile, when it has a row for the shape."""
-    achieved = 2.0 * m * n * k / (ms * 1e-3) / 1e12
+    achieved = 2.0 * nb * m * n * k / (ms * 1e-3) / 1e12
     traffic = None
     if os.path.exists(GEMM_TRAFFIC):
         for row in json.load(open(GEMM_TRAFFIC)).get("shapes", []):
-            if [row.get("m"), row.get("n"), row.get("k")] == [m, n, k]:
+            if [row.get("m"), row.get("n"), row.get("k"), row.get("batch", 1)] == [m, n, k, nb]:
                 traffic = row.get("hbm_bytes")
     return {"bound": "mfma", "achieved": achieved, "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / BF16_PEAK_TFLOPS, "traffic": traffic,
@@ -609,24 +613,25 @@ def bench_gemm(args):
     from ccsd_gpu_amd import _lib
     _lib.check(_lib.load().ccsd_gemm_set_variant(args.gemm_variant), "ccsd_gemm_set_variant")
     rows = []
-    for name, m, n, k in LAMBDA_GEMMS:
-        ms = timed_gemm(m, n, k, reps=max(1, args.steps))
-        roof = gemm_roofline(m, n, k, ms)
-        vms = timed_vendor_bf16(m, n, k, reps=max(1, args.steps))
-        rows.append({"product": name, "m": m, "n": n, "k": k, "ms": ms, "algorithmic_tflops": roof["achieved"],
-                     "frac": roof["frac"], "hardware_bf16_tflops": roof["hardware_bf16_tflops"],
-                     "vendor_bf16_mm_ms": vms, "vendor_bf16_tflops": 2.0 * m * n * k / vms / 1e9})
-        log(f"{name}: {m}x{n}x{k} {ms:.3f} ms, {roof['achieved']:.0f} TF/s algorithmic ({roof['frac']:.1%} of the "
-            f"bf16 peak), {roof['hardware_bf16_tflops']:.0f} TF/s of bf16 MFMA work; one plain bf16 product "
-            f"through the vendor library {vms:.3f} ms ({2.0 * m * n * k / vms / 1e9:.0f} TF/s)")
+    for name, m, n, k, nb in LAMBDA_GEMMS:
+        ms = timed_gemm(m, n, k, reps=max(1, args.steps), nb=nb)
+        roof = gemm_roofline(m, n, k, ms, nb)
+        vms = nb * timed_vendor_bf16(m, n, k, reps=max(1, args.steps))
+        rows.append({"product": name, "m": m, "n": n, "k": k, "batch": nb, "ms": ms,
+                     "algorithmic_tflops": roof["achieved"], "frac": roof["frac"],
+                     "hardware_bf16_tflops": roof["hardware_bf16_tflops"],
+                     "vendor_bf16_mm_ms": vms, "vendor_bf16_tflops": 2.0 * nb * m * n * k / vms / 1e9})
+        log(f"{name}: {nb} x {m}x{n}x{k} {ms:.3f} ms, {roof['achieved']:.0f} TF/s algorithmic ({roof['frac']:.1%} "
+            f"of the bf16 peak), {roof['hardware_bf16_tflops']:.0f} TF/s of bf16 MFMA work; the same plain bf16 "
+            f"products through the vendor library {vms:.3f} ms ({2.0 * nb * m * n * k / vms / 1e9:.0f} TF/s)")
     head = rows[0]
     return {
-        "metric": METRIC, "value": head["algorithmic_tflops"], "unit": "TFLOP/s (2 M N K per launch)",
+        "metric": METRIC, "value": head["algorithmic_tflops"], "unit": "TFLOP/s (2 batch M N K per launch)",
         "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": head["ms"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3 (fp32 accumulate, fp64 out)", "data": "synthetic",
         "config": {"workload": "split-BF16 NT GEMM on the (H2O)8/cc-pVDZ lambda-sweep shapes, random operands",
                    "gemm_variant": args.gemm_variant, "products": rows},
-        "roofline": gemm_roofline(head["m"], head["n"], head["k"], head["ms"]), "cpu_baseline": None}
+        "roofline": gemm_roofline(head["m"], head["n"], head["k"], head["ms"], head["batch"]), "cpu_baseline": None}
 
 
 def main():

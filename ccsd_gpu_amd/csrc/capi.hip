@@ -78,8 +78,10 @@ hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o,
 hipError_t ccsd_launch_gemm_mixed(int mode, const double* A, const double* B, double* C, int M, int N, int K,
                                   long long lda, long long ldb, long long ldc, long long sA, long long sB,
                                   long long sC, int batch, hipStream_t stream);
-hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
-                             const double* charges, int natm, int nbas, int nao, int maxprim, hipStream_t stream);
+hipError_t ccsd_launch_int1e(int kind, unsigned long long* fx, const ShellInfo* shells, const double* env,
+                             const double* coords, const double* charges, int natm, int nbas, int nao, int maxprim,
+                             hipStream_t stream);
+hipError_t ccsd_launch_int1e_convert(const unsigned long long* fx, long long n, double* out, hipStream_t stream);
 
 namespace {
 
@@ -235,6 +237,8 @@ struct ccsd_plan {
     std::vector<std::array<long long, 10>> last_launches;   // see ccsd_plan_last_launches
     unsigned long long* d_de_scratch = nullptr;      // autotuner's throw-away gradient (fixed point)
     unsigned long long* d_defx = nullptr;            // fixed-point gradient of a fused pass (1 + 6 natm words)
+    unsigned long long* d_int1e_fx = nullptr;        // fixed-point one-electron integrals of the last ccsd_int1e
+    long long int1e_fx_words = 0;
     std::vector<double> coords_h;                    // natm x 3
     // side streams (default priority) that run independent class launches of a fused pass
     // concurrently; forked from and joined back to the caller's stream with events
@@ -1284,6 +1288,7 @@ int ccsd_plan_destroy_impl(ccsd_plan* p)
     if (p->d_mol_dm_off) (void)hipFree(p->d_mol_dm_off);
     if (p->d_de_scratch) (void)hipFree(p->d_de_scratch);
     if (p->d_defx) (void)hipFree(p->d_defx);
+    if (p->d_int1e_fx) (void)hipFree(p->d_int1e_fx);
     for (int i = 0; i < ccsd_plan::kSideStreams; i++) {
         if (p->side[i]) (void)hipStreamDestroy(p->side[i]);
         if (p->join_ev[i]) (void)hipEventDestroy(p->join_ev[i]);
@@ -1437,13 +1442,22 @@ int ccsd_int1e(ccsd_plan* p, int kind, double* out, void* stream)
     if (kind < 0 || kind > 5) return fail(CCSD_ERR_SHAPE, "int1e kind must be 0..5");
     const long long nn = (long long)p->nao_cart * p->nao_cart;
     const long long nmat = kind <= 2 ? 1 : (kind <= 4 ? 3 : 3LL * p->natm);
-    hipError_t e = ccsd_launch_fill(out, nmat * nn, 0.0, (hipStream_t)stream);
+    const long long nw = 1 + 2 * nmat * nn;     // fixed-point words: overflow count, then 2 per element
+    if (nw > p->int1e_fx_words) {
+        if (p->d_int1e_fx) (void)hipFree(p->d_int1e_fx);
+        p->d_int1e_fx = nullptr;
+        p->int1e_fx_words = 0;
+        HIP_TRY(hipMalloc(&p->d_int1e_fx, sizeof(unsigned long long) * nw));
+        p->int1e_fx_words = nw;
+    }
+    hipError_t e = ccsd_launch_fill((double*)p->d_int1e_fx, nw, 0.0, (hipStream_t)stream);
     if (e == hipSuccess) {
         int maxprim = 1;
         for (const ShellInfo& sh : p->shells) maxprim = std::max(maxprim, sh.nprim);
-        e = ccsd_launch_int1e(kind, out, p->d_shells, p->d_env, p->d_coords, p->d_charges, p->natm, p->nbas,
-                              p->nao_cart, maxprim, (hipStream_t)stream);
+        e = ccsd_launch_int1e(kind, p->d_int1e_fx, p->d_shells, p->d_env, p->d_coords, p->d_charges, p->natm,
+                              p->nbas, p->nao_cart, maxprim, (hipStream_t)stream);
     }
+    if (e == hipSuccess) e = ccsd_launch_int1e_convert(p->d_int1e_fx, nmat * nn, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "int1e_kernel");
     return CCSD_OK;
 }

@@ -2,13 +2,14 @@
 //
 // The reference evaluates these on the CPU through PySCF (hcore_generator,
 // get_ovlp, int1e_ipovlp / ipkin / ipnuc / iprinv at ccsd_gpu/gpu_ccsd_grad.py:554-583).
-// One thread per ordered shell pair sums the pair's Cartesian block over its
-// primitive pairs: 1-D Obara-Saika tables for overlap and kinetic, Rys quadrature
-// with point charges for the nuclear attraction (the ERI tables, n <= 5 for l <= 4
-// plus one derivative; kind 2 sums every nucleus in the same thread, kind 5 runs one
-// thread per (shell pair, nucleus), each nucleus having its own output). Every output
-// element is owned by one thread and summed in a fixed order (no atomics: the same
-// bits on every run); the output must start at zero.
+// One thread per (ordered shell pair, bra primitive) adds its part of the pair's
+// Cartesian block for overlap and kinetic (1-D Obara-Saika tables); the
+// nuclear-attraction kinds use one thread per (shell pair, bra primitive,
+// nucleus), Rys quadrature with point charges (the ERI tables, n <= 5 for l <= 4
+// plus one derivative). The parts of one element are added with exact fixed-point
+// integer atomics (defx_add1: two 64-bit words per element, x to 2^-60, must start
+// at zero) and converted to float64 once at the end, so small bases still fill the
+// device and the result is the same bit for bit on every run.
 //
 // kinds: 0 <i|j>  1 <i|T|j>  2 <i|V|j>  3 <d/dr i|j>  4 <d/dr i|T|j>
 //        5 <d/dr i| -Z_C/|r-C| |j> for every nucleus C (natm x 3 matrices)
@@ -65,24 +66,29 @@ __device__ void nuc1d(double I[L1 + 1][L1 + 1], int la, int lb, double PA, doubl
         for (int j = 0; j <= lb; j++) I[i][j] = g[i][j];
 }
 
-__global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo* __restrict__ shells,
+__device__ __forceinline__ void fx_add(unsigned long long* fx, size_t e, double v)
+{
+    defx_add1(fx + 1 + 2 * e, fx, v);
+}
+
+__global__ void int1e_kernel(int kind, unsigned long long* __restrict__ fx, const ShellInfo* __restrict__ shells,
                              const double* __restrict__ env, const double* __restrict__ coords,
                              const double* __restrict__ charges, int natm, int nbas, int nao, int maxprim)
 {
-    // one thread per shell pair (and nucleus, for the nuclear-attraction derivative,
-    // whose nuclei have outputs of their own): each output element has one owner
-    // thread, which sums its primitive pairs (and, for kind 2, the nuclei) in a fixed
-    // order, so the result is the same bit for bit on every run
-    const bool per_nucleus = kind == 5;
+    // one thread per (shell pair, bra primitive[, nucleus for the nuclear-attraction
+    // kinds]); the threads of one element add atomically (fixed point)
+    const bool per_nucleus = kind == 2 || kind == 5;
     const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     const long long npair = (long long)nbas * nbas;
-    if (t >= (per_nucleus ? npair * natm : npair)) return;
+    const long long nunit = npair * maxprim;
+    if (t >= (per_nucleus ? nunit * natm : nunit)) return;
     const long long u = per_nucleus ? t / natm : t;
-    const int pair = (int)u;
+    const int pair = (int)(u / maxprim), ip_only = (int)(u - (u / maxprim) * maxprim);
     const int c_first = per_nucleus ? (int)(t - u * natm) : 0;
     const int c_last = per_nucleus ? c_first + 1 : natm;
     const int ish = pair / nbas, jsh = pair - ish * nbas;
     const ShellInfo si = shells[ish], sj = shells[jsh];
+    if (ip_only >= si.nprim) return;
     const int la = si.l, lb = sj.l;
     const int deriv = kind >= 3;
     const int lae = la + deriv;
@@ -93,7 +99,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
     const size_t nn = (size_t)nao * nao;
     double AB2 = 0.0;
     for (int x = 0; x < 3; x++) AB2 += (A[x] - B[x]) * (A[x] - B[x]);
-    for (int ip = 0; ip < si.nprim; ip++)
+    for (int ip = ip_only; ip <= ip_only; ip++)
         for (int jp = 0; jp < sj.nprim; jp++) {
             const double a = env[si.pexp + ip], b = env[sj.pexp + jp], p = a + b;
             const double K = exp(-a * b / p * AB2) * nrm;
@@ -132,7 +138,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                     const double v = (kind == 0 || kind == 3)
                                                          ? s[0] * s[1] * s[2]
                                                          : tk[0] * s[1] * s[2] + s[0] * tk[1] * s[2] + s[0] * s[1] * tk[2];
-                                    out[c * nn + (size_t)(i0 + fa) * nao + j0 + fb] += cc * v;
+                                    fx_add(fx, c * nn + (size_t)(i0 + fa) * nao + j0 + fb, cc * v);
                                 }
                             }
                     } else {
@@ -156,7 +162,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                         comp1(lb, fb, cb);
                                         const size_t e = (size_t)(i0 + fa) * nao + j0 + fb;
                                         if (!deriv) {
-                                            out[e] += I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]];
+                                            fx_add(fx, e, I[0][ca[0]][cb[0]] * I[1][ca[1]][cb[1]] * I[2][ca[2]][cb[2]]);
                                         } else {
                                             for (int c = 0; c < 3; c++) {
                                                 double f[3];
@@ -165,7 +171,7 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
                                                     f[x] = x == c ? -2 * a * I[x][i + 1][j] + (i > 0 ? i * I[x][i - 1][j] : 0.0)
                                                                   : I[x][i][j];
                                                 }
-                                                out[(size_t)(C * 3 + c) * nn + e] += f[0] * f[1] * f[2];
+                                                fx_add(fx, (size_t)(C * 3 + c) * nn + e, f[0] * f[1] * f[2]);
                                             }
                                         }
                                     }
@@ -176,16 +182,35 @@ __global__ void int1e_kernel(int kind, double* __restrict__ out, const ShellInfo
         }
 }
 
+// out[e] = h 2^-20 + l 2^-60 of the fixed-point words (NaN everywhere after an overflow)
+__global__ void int1e_convert_kernel(const unsigned long long* __restrict__ fx, long long n, double* __restrict__ out)
+{
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const long long h = (long long)fx[1 + 2 * e], l = (long long)fx[2 + 2 * e];
+        out[e] = fx[0] ? __builtin_nan("") : (double)h * 9.5367431640625e-07 + (double)l * 8.673617379884035e-19;
+    }
+}
+
 }  // namespace
 
-hipError_t ccsd_launch_int1e(int kind, double* out, const ShellInfo* shells, const double* env, const double* coords,
-                             const double* charges, int natm, int nbas, int nao, int maxprim, hipStream_t stream)
+// out (n elements) from the fixed-point accumulator fx (1 + 2 n words)
+hipError_t ccsd_launch_int1e_convert(const unsigned long long* fx, long long n, double* out, hipStream_t stream)
 {
-    (void)maxprim;
-    const long long n = (long long)nbas * nbas * (kind == 5 ? natm : 1);
+    if (n <= 0) return hipSuccess;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(int1e_convert_kernel, dim3((unsigned)nb), dim3(256), 0, stream, fx, n, out);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_int1e(int kind, unsigned long long* fx, const ShellInfo* shells, const double* env,
+                             const double* coords, const double* charges, int natm, int nbas, int nao, int maxprim,
+                             hipStream_t stream)
+{
+    const long long n = (long long)nbas * nbas * maxprim * ((kind == 2 || kind == 5) ? natm : 1);
     if (n <= 0) return hipSuccess;
     if ((n + 63) / 64 > 2147483647LL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(int1e_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, kind, out, shells, env,
+    hipLaunchKernelGGL(int1e_kernel, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, stream, kind, fx, shells, env,
                        coords, charges, natm, nbas, nao, maxprim);
     return hipGetLastError();
 }

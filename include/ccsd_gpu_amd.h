@@ -141,7 +141,9 @@ int ccsd_grad_ip1_gamma_sep(ccsd_plan* plan, double* de, const double* gamma, co
  * (all nuclei), 3 <d/dr i|j> (3 matrices), 4 <d/dr i|T|j> (3), 5 <d/dr i|-Z_C/|r-C||j>
  * for every nucleus C (natm x 3). These replace the PySCF one-electron pieces of
  * the gradient -- mf_grad.get_ovlp, hcore_generator (int1e_ipkin, int1e_ipnuc,
- * int1e_iprinv) -- called at ccsd_gpu/gpu_ccsd_grad.py:554-583. */
+ * int1e_iprinv) -- called at ccsd_gpu/gpu_ccsd_grad.py:554-583. Parts of an element
+ * are summed exactly in fixed point (the same bits on every run) in an accumulator
+ * the plan owns: calls on one plan that overlap (other streams or threads) race. */
 int ccsd_int1e(ccsd_plan* plan, int kind, double* out, void* stream);
 
 /* Batched C = A B on the matrix cores, float64 operands in device memory:
