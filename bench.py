@@ -1,6 +1,6 @@
 """Benchmark of the CCSD-gradient hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad|eri|batch|precision|gemm] [--waters N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config grad|eri|batch|precision|gemm|int2e] [--waters N]
 
 --config grad (default): the full CCSD gradient of a seeded water octamer
 (H2O)8 / cc-pVDZ (192 AOs, nocc 40, nvir 152; rigid monomers, random
@@ -26,6 +26,10 @@ the whole batch with a separable density Gamma(D) per molecule; value = survivin
 quartets/s. config.persistent_single_launch compares, on the first
 --persistent-molecules molecules, the pass as one persistent launch over every
 class with the per-class launches.
+
+--config int2e: the store-mode drop-ins, int2e and int2e_ip1 over all shells of
+(H2O)8 (Cartesian s2kl, --waters N for others); value = int2e quartets/s, with the
+bytes written per second and the FP64 flop-model roofline of each.
 
 --config gemm: the split-BF16 NT GEMM of the lambda / RDM contractions on the
 (H2O)8 sweep shapes (random operands), each beside the same plain bf16 products
@@ -88,6 +92,56 @@ def roofline(mol, kern_ms):
     achieved = flops / (kern_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}, flops, quartets, ref_quartets
+
+
+def bench_int2e(args):
+    """The store-mode drop-ins: int2e and int2e_ip1 over all shells of (H2O)N (Cartesian
+    s2kl, the layout the gradient's AO integrals use), each timed with HIP events on the
+    launch stream into a preallocated output."""
+    import torch
+    from ccsd_gpu_amd import _lib, flopcount, systems
+    mol = systems.water_cluster(args.waters or 8, seed=1, cart=True)
+    nao = mol.nao_cart()
+    npair = nao * (nao + 1) // 2
+    lib = _lib.load()
+    plan = _lib.get_plan(mol)
+    stream = torch.cuda.current_stream()
+    res = {}
+    for name, fn, ncomp, deriv in (("int2e", lib.ccsd_int2e_s2kl, 1, 0), ("int2e_ip1", lib.ccsd_int2e_ip1_s2kl, 3, 1)):
+        out = torch.empty((ncomp, nao, nao, npair), dtype=torch.float64, device="cuda")
+
+        def call():
+            _lib.check(fn(plan, _lib.ptr(out), 0, mol.nbas, 1, _lib.current_stream_ptr()), name)
+        for _ in range(max(1, args.warmup)):
+            call()                          # first call: kernel choices and work lists
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(args.steps):
+            call()
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b) / args.steps
+        flops, quartets, refq = flopcount.eri_work(mol, deriv)
+        gb = out.numel() * 8 / 1e9
+        res[name] = {"ms": ms, "quartets": quartets, "quartets_per_s": quartets / (ms * 1e-3),
+                     "output_GB": gb, "write_GBps": gb / (ms * 1e-3),
+                     "roofline": {"bound": "valu", "achieved": flops / (ms * 1e-3) / 1e12, "peak": FP64_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                                  "traffic": None}}
+        log(f"{name}: {ms:.2f} ms, {quartets / (ms * 1e-3):.3e} quartets/s, {gb:.2f} GB written "
+            f"({gb / (ms * 1e-3):.0f} GB/s), {res[name]['roofline']['achieved']:.2f} TF/s FP64 model")
+        del out
+    if args.copy_calibration:
+        copy_calibration()
+    head = res["int2e"]
+    return {
+        "metric": METRIC, "value": head["quartets_per_s"], "unit": "ERI quartets/s (int2e store, s2kl)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["ms"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"int2e / int2e_ip1 s2kl store over all shells, (H2O){args.waters or 8}/cc-pVDZ "
+                               "Cartesian", "results": res},
+        "roofline": head["roofline"], "cpu_baseline": None}
 
 
 def cpu_eri_rate(mol, nshell=3):
@@ -639,7 +693,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision", "gemm"])
+    ap.add_argument("--config", default="grad", choices=["grad", "eri", "batch", "precision", "gemm", "int2e"])
     ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--persistent-molecules", type=int, default=4,
@@ -665,7 +719,7 @@ def main():
     from ccsd_gpu_amd.determinism import deterministic
     with deterministic(args.deterministic):
         line = {"grad": bench_grad, "eri": bench_eri, "batch": bench_batch, "precision": bench_precision,
-                "gemm": bench_gemm}[args.config](args)
+                "gemm": bench_gemm, "int2e": bench_int2e}[args.config](args)
     if args.deterministic:
         line["config"]["deterministic"] = True
     if args.copy_calibration and args.config != "eri":
