@@ -8,7 +8,8 @@ The public names of the reference package (ccsd_gpu/__init__.py:1-13):
     mp               -- None: the MP2 module of the reference's T-amplitude driver is
                         outside this build
 plus the integral drop-ins compute_int2e_gpu and compute_int2e_ip1_gpu
-(ccsd_gpu/cuda/int2e_gpu.py:186, int2e_ip1_gpu.py:296). Names are resolved on
+(ccsd_gpu/cuda/int2e_gpu.py:186, int2e_ip1_gpu.py:296) and deterministic() (a context
+manager for bitwise-repeatable runs, no reference counterpart). Names are resolved on
 first use, so importing the package does not load the HIP library.
 """
 import importlib
@@ -22,12 +23,13 @@ _LAZY = {
     "compute_ml_ccsd_forces_gpu": ("gpu_ccsd_grad", "compute_ml_ccsd_forces_gpu"),
     "compute_int2e_gpu": ("hip.int2e_gpu", "compute_int2e_gpu"),
     "compute_int2e_ip1_gpu": ("hip.int2e_ip1_gpu", "compute_int2e_ip1_gpu"),
+    "deterministic": ("determinism", "deterministic"),
     "cc": ("cc", None),
     "pyscf_cc": ("cc", None),
 }
 
 __all__ = ["cc", "has_gpu4pyscf", "mp", "pyscf_cc", "compute_ml_ccsd_forces_gpu", "grad_elec_gpu",
-           "solve_lambda_gpu", "compute_int2e_gpu", "compute_int2e_ip1_gpu"]
+           "solve_lambda_gpu", "compute_int2e_gpu", "compute_int2e_ip1_gpu", "deterministic"]
 
 
 def __getattr__(name):
