@@ -9,13 +9,14 @@
 // accumulation). C is float64 [batch][M][N] (row stride ldc), overwritten
 // (beta = 0) or accumulated into (beta = 1).
 //
-// Block: 256 threads = 4 waves, 128x128 C tile, each wave 64x64 = 2x2 MFMA
-// 32x32 tiles; K staged by 64 (32 when K <= 256) through LDS, next tile
-// prefetched into registers while the current one is multiplied. Tiles are handed out XCD-aware: the
-// hardware deals consecutive workgroups round-robin to the 8 XCDs, so the linear
-// block index is remapped to give each XCD a contiguous range of tiles, ordered
-// in groups of 8 M-tiles (N-tile by N-tile inside a group) so that the blocks
-// running together on one XCD share A and B tiles in that XCD's L2.
+// Default block: 256 threads = 4 waves, 128x128 C tile, each wave 64x64 = 2x2 MFMA
+// 32x32 tiles; K staged by 64 (32 when K <= 256) through LDS, next tile prefetched
+// into registers while the current one is multiplied (other block shapes and a
+// two-stage LDS pipeline are selectable, see gemm_bf16x3_nt_body). For long K, tiles
+// are handed out XCD-aware: the hardware deals consecutive workgroups round-robin to
+// the 8 XCDs, so the linear block index is remapped to give each XCD a contiguous
+// range of tiles, ordered in groups of 8 M-tiles (N-tile by N-tile inside a group) so
+// that the blocks running together on one XCD share A and B tiles in that XCD's L2.
 #include <hip/hip_runtime.h>
 
 #include "bf16_split.h"
