@@ -88,3 +88,21 @@ def test_frozen_mask_convention():
         frozen_mask(5, [7])
     with pytest.raises(ValueError):
         frozen_mask(5, 9)
+
+
+def test_pair_form_and_tuning_entries_reject_bad_arguments():
+    """Argument checks of the vvvv pair-form kernels, the FP32 repack and the GEMM
+    block-shape switch (all before any device work)."""
+    lib = _lib.load()
+    buf = (ctypes.c_double * 16)()
+    p = ctypes.cast(buf, ctypes.c_void_p)
+    # np must be v (v + 1) / 2, ld >= np, npair <= o (o + 1) / 2; split needs the lo plane
+    assert lib.ccsd_vvvv_pm_pack(p, 2, 3, p, p, 3, p, p, 5, 6, 0, p, None, None) == ERR_SHAPE
+    assert lib.ccsd_vvvv_pm_pack(p, 2, 3, p, p, 3, p, p, 6, 5, 0, p, None, None) == ERR_SHAPE
+    assert lib.ccsd_vvvv_pm_pack(p, 2, 3, p, p, 4, p, p, 6, 6, 0, p, None, None) == ERR_SHAPE
+    assert lib.ccsd_vvvv_pm_pack(p, 2, 3, p, p, 3, p, p, 6, 8, 1, p, None, None) == ERR_NULL
+    assert lib.ccsd_vvvv_pm_unpack(p, 2, 3, 5, 1.0, p, None) == ERR_SHAPE
+    assert lib.ccsd_vvvv_pm_unpack(None, 2, 3, 6, 1.0, p, None) == ERR_NULL
+    assert lib.ccsd_repack_pairs_f32(None, 2, 3, p, None) == ERR_NULL
+    assert lib.ccsd_gemm_set_variant(3) == ERR_SHAPE and lib.ccsd_gemm_set_variant(-1) == ERR_SHAPE
+    assert lib.ccsd_gemm_set_variant(0) == 0
