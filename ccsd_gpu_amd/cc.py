@@ -250,6 +250,8 @@ class DIIS:
         self.B = np.zeros((space, space))
         self.count = 0
         self.prev = None
+        self.res_norm = None        # |sum_s c_s E[s]| of the last extrapolation
+        self.c_abs = None           # sum_s |c_s| of the last extrapolation
 
     def _alloc(self, n, like):
         if self.X is None:
@@ -300,7 +302,12 @@ class DIIS:
         A[n, n] = 0.0
         rhs = np.zeros(n + 1)
         rhs[n] = -1.0
-        c = torch.as_tensor(np.linalg.lstsq(A, rhs, rcond=None)[0][:n], dtype=self.X.dtype, device=self.X.device)
+        cn = np.linalg.lstsq(A, rhs, rcond=None)[0][:n]
+        # |sum_s c_s E[s]|, from the overlap matrix: see input_combination
+        self.res_norm = float(np.sqrt(max(float(cn @ self.B[:n, :n] @ cn), 0.0)))
+        self.c_abs = float(np.abs(cn).sum())
+        c = torch.as_tensor(cn, dtype=self.X.dtype, device=self.X.device)
+        self._c = c
         if self.X.is_cuda:
             # sum_s c_s X[s]: the column GEMV (one coalesced pass over the n stored vectors)
             from . import mfma
@@ -309,6 +316,23 @@ class DIIS:
             out = c @ self.X[:n]
         self.prev = out
         return out, float(np.sqrt(max(row[slot], 0.0)))
+
+    def input_combination(self):
+        """sum_s c_s x_s for the coefficients of the last extrapolation, where x_s is the
+        vector that stored vector X[s] was computed from (x_s = X[s] - E[s]).
+
+        The extrapolation returns sum_s c_s X[s]; when X[s] = F(x_s) for an affine map F
+        and sum_s c_s = 1, the vector x = sum_s c_s x_s has the residual
+        F(x) - x = sum_s c_s E[s] exactly, whose norm is res_norm: a converged iterate
+        known without another application of F. Call it before the next push."""
+        n = min(self.count, self.space)
+        if self.X.is_cuda:
+            from . import mfma
+            ebar = mfma.gemv(self.E[:n].T, self._c, torch.empty(self.E.shape[1], dtype=self.E.dtype,
+                                                                   device=self.E.device))
+        else:
+            ebar = self._c @ self.E[:n]
+        return self.prev - ebar
 
 
 def amplitudes_to_vector(t1, t2):

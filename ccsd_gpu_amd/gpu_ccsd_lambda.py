@@ -464,8 +464,12 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
             tick("graph capture (incl. first update)")
         return entry(l1, l2, homogeneous)
 
-    def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None):
-        """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters)."""
+    def fixed_point(l1, l2, prec, tol_, ncyc, rhs=None, accept=None):
+        """Iterate l <- F(l) (rhs None) or d <- A d + rhs; returns (conv, l1, l2, norm, iters).
+
+        accept = (tol, margin, state): after each DIIS step, stop with DIIS's input
+        combination (DIIS.input_combination) when its residual norm plus `margin` times
+        sum |c| |d| is below tol; state["residual"] is then that norm."""
         adiis = DIIS(diis_space) if use_diis else None
         rhs_vec = amplitudes_to_vector(*rhs) if rhs is not None else None
         normt = None
@@ -486,6 +490,13 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
                 normt = float(torch.linalg.norm(vec - prev))
             history.append((prec, rhs is not None, normt))
             prev = vec
+            if accept is not None and adiis is not None and adiis.res_norm is not None:
+                atol, margin, state = accept
+                res = adiis.res_norm
+                if res + margin * adiis.c_abs * float(torch.linalg.norm(vec)) < atol:
+                    state["residual"] = res
+                    x1, x2 = vector_to_amplitudes(adiis.input_combination(), nocc, nvir)
+                    return True, x1, x2, res, it + 1
             l1, l2 = vector_to_amplitudes(vec, nocc, nvir)
             if normt < tol_:
                 return True, l1, l2, normt, it + 1
@@ -553,6 +564,12 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
     # otherwise the FP64 residual is recomputed.
     res_update = getattr(mycc, "lambda_residual_update", True)
     est_margin = getattr(mycc, "lambda_est_margin", 1e-3)
+    # DIIS's own residual: with A d in low precision the correction iteration is affine,
+    # so the combination of the correction sweeps' inputs with the DIIS coefficients has
+    # the residual sum_s c_s E[s] exactly up to the low precision's rounding of each A d_s
+    # (bounded by est_margin sum_s |c_s| |d|); accepting on it saves the sweep of the
+    # residual update
+    diis_res = getattr(mycc, "lambda_diis_residual", True)
     r1 = r2 = None
     for _round in range(max(8, max_cycle // 4)):
         if r1 is None:
@@ -572,11 +589,17 @@ def _mixed_solve(mycc, fixed_point, update, t1, t2, l1, l2, eris, imds, use_cpu_
         # round to 1e-3)
         rtol = getattr(mycc, "lambda_corr_rtol", 1e-2)
         ncyc = getattr(mycc, "lambda_corr_max_cycle", 5)
+        state = {}
         _, d1, d2, _, it = fixed_point(r1.clone(), r2.clone(), precision, max(normt * rtol, tol * 1e-2), ncyc,
-                                       rhs=(r1, r2))
+                                       rhs=(r1, r2), accept=(tol, est_margin, state) if diis_res else None)
         total += it
         l1, l2 = l1 + d1, l2 + d2
         tick("correction sweeps")
+        if "residual" in state:
+            normt = state["residual"]
+            mycc._lambda_history.append(("diis-residual", True, normt))
+            conv = True
+            break
         if res_update:
             a1, a2 = update(mycc, t1, t2, d1, d2, eris, imds, use_cpu_vvvv, precision, homogeneous=True)
             e1, e2 = r1 + a1 - d1, r2 + a2 - d2

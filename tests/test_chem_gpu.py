@@ -266,6 +266,7 @@ def test_lambda_residual_update_accepts_only_true_convergence(gpu, h2o_pair, pre
     mycc.conv_tol_normt = tol
     mycc.lambda_corr_rtol = 1e-4      # a tight correction, so that the estimate is what accepts
     mycc.lambda_corr_max_cycle = 30
+    mycc.lambda_diis_residual = False
     g1, g2 = solve_lambda_gpu(mycc, precision=precision, return_tensors=True)
     hist = list(mycc._lambda_history)
     assert mycc.converged_lambda
@@ -275,6 +276,37 @@ def test_lambda_residual_update_accepts_only_true_convergence(gpu, h2o_pair, pre
     solve_lambda_gpu(mycc, l1=g1.clone(), l2=g2.clone(), precision="fp64")
     true_res = mycc._lambda_history[0][2]
     print(precision, "estimated residual", est, "FP64 residual", true_res, flush=True)
+    assert true_res < tol
+    assert abs(true_res - est) < 1e-9
+    assert float(np.abs(g1.cpu().numpy() - l1).max()) < 1e-7
+    assert float(np.abs(g2.cpu().numpy() - l2).max()) < 1e-7
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["bf16x3", "fp32"])
+def test_lambda_diis_residual_accepts_only_true_convergence(gpu, h2o_pair, precision, monkeypatch):
+    """Convergence accepted on the correction DIIS's own residual (the input combination
+    of its sweeps, whose residual is sum_s c_s E[s]): the FP64 residual of the answer,
+    measured as the first change norm of an FP64 solve started from it, is below the
+    tolerance and agrees with that norm, and the answer is the oracle's lambda."""
+    from ccsd_gpu_amd import precision as P
+    from ccsd_gpu_amd.gpu_ccsd_lambda import solve_lambda_gpu
+    monkeypatch.setattr(P, "MIXED_MIN_FLOPS", 0.0)     # real low-precision noise in A d
+    mol, omf, occ, (l1, l2), gmf = h2o_pair
+    mycc = _gpu_cc_with_oracle_orbitals(gmf, omf, occ)
+    tol = 1e-8
+    mycc.conv_tol_normt = tol
+    mycc.lambda_corr_rtol = 1e-6      # the inner target out of reach: DIIS's residual accepts
+    mycc.lambda_corr_max_cycle = 30
+    g1, g2 = solve_lambda_gpu(mycc, precision=precision, return_tensors=True)
+    hist = list(mycc._lambda_history)
+    assert mycc.converged_lambda
+    assert hist[-1][0] == "diis-residual"
+    assert mycc._lambda_fp64_updates == 1
+    est = hist[-1][2]
+    solve_lambda_gpu(mycc, l1=g1.clone(), l2=g2.clone(), precision="fp64")
+    true_res = mycc._lambda_history[0][2]
+    print(precision, "DIIS residual", est, "FP64 residual", true_res, flush=True)
     assert true_res < tol
     assert abs(true_res - est) < 1e-9
     assert float(np.abs(g1.cpu().numpy() - l1).max()) < 1e-7

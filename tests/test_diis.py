@@ -35,3 +35,28 @@ def test_push_first_call_without_start_has_no_norm():
     p = (torch.ones(2, dtype=torch.float64), torch.zeros(3, dtype=torch.float64))
     v, nrm = d.push(p)
     assert nrm is None and np.allclose(v.numpy(), [1, 1, 0, 0, 0])
+
+
+def test_input_combination_residual_is_res_norm_for_an_affine_map():
+    """For x -> F(x) = A x + b, DIIS's input combination has the residual F(x) - x of
+    norm res_norm (computed from the overlap matrix alone), here checked by applying F."""
+    g = torch.Generator().manual_seed(5)
+    n = 40
+    q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64))
+    A = q @ torch.diag(torch.linspace(-0.8, 0.9, n, dtype=torch.float64)) @ q.T
+    b = torch.randn(n, generator=g, dtype=torch.float64)
+    F = lambda x: A @ x + b  # noqa: E731
+    for space in (3, 6):
+        d = DIIS(space)
+        x = torch.zeros(n, dtype=torch.float64)
+        checked = 0
+        for _ in range(12):
+            y, _ = d.push((F(x)[:5], F(x)[5:]))
+            if d.res_norm is not None:
+                xb = d.input_combination()
+                r = float(torch.linalg.norm(F(xb) - xb))
+                assert abs(r - d.res_norm) <= 1e-9 * max(r, 1e-12) + 1e-14, (space, r, d.res_norm)
+                assert d.c_abs >= 1.0 - 1e-12
+                checked += 1
+            x = y
+        assert checked >= 10
