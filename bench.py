@@ -22,10 +22,13 @@ gradient against the FP64 lambda; value = split-BF16 lambda wall-s.
 
 --config batch: 64 seeded C/N/O/H molecules (3-12 heavy atoms, cc-pVDZ) in one
 plan; one step = one Schwarz-screened (1e-12) fused int2e_ip1 x Gamma pass over
-the whole batch with a separable density Gamma(D) per molecule; value = surviving
-quartets/s. config.persistent_single_launch compares, on the first
---persistent-molecules molecules, the pass as one persistent launch over every
-class with the per-class launches.
+the whole batch with a separable density Gamma(D) per molecule (one launch per
+angular-momentum class for the whole batch, four streams); value = surviving
+quartets/s. config.persistent_register_launch times the same pass as ONE
+persistent launch of the register-resident per-class bodies over every class of
+every molecule; config.persistent_generic_launch_first_molecules the persistent
+launch of the LDS kernel's generic body on the first --persistent-molecules
+molecules.
 
 --config int2e: the store-mode drop-ins, int2e and int2e_ip1 over all shells of
 (H2O)8 (Cartesian s2kl, --waters N for others); value = int2e quartets/s, with the
@@ -248,7 +251,8 @@ def timed_batch_pass(b, dms, reps):
 
 def persistent_vs_per_class(mols, dms, thr):
     """Screened separable fused pass over `mols` as per-class launches (register kernels,
-    four streams) and as one persistent launch; times (HIP events) and max |difference|."""
+    four streams) and as one persistent launch of the LDS kernel's generic body; times
+    (HIP events) and max |difference|."""
     import torch
     from ccsd_gpu_amd.batch import MoleculeBatch
     b = MoleculeBatch(mols)
@@ -256,16 +260,16 @@ def persistent_vs_per_class(mols, dms, thr):
     b.grad2e_separable(dms)
     cls_ms, de_c = timed_batch_pass(b, dms, 2)
     de_c = torch.cat(de_c).clone()
-    b.set_persistent(True)
+    b.set_persistent(True, body="generic")
     b.grad2e_separable(dms)       # first call: generic-kernel choices, work items
     pers_ms, de_p = timed_batch_pass(b, dms, 2)
     q = b.last_quartets()
     diff = float((torch.cat(de_p) - de_c).abs().max())
     b.close()
-    log(f"{len(mols)} molecules: one persistent launch {pers_ms:.1f} ms vs per-class launches {cls_ms:.1f} ms "
-        f"({q} quartets, max |diff| {diff:.1e})")
-    return {"molecules": len(mols), "quartets": q, "persistent_ms": pers_ms, "per_class_ms": cls_ms,
-            "persistent_quartets_per_s": q / (pers_ms * 1e-3), "max_abs_diff": diff}
+    log(f"{len(mols)} molecules: one persistent launch (generic body) {pers_ms:.1f} ms vs per-class launches "
+        f"{cls_ms:.1f} ms ({q} quartets, max |diff| {diff:.1e})")
+    return {"molecules": len(mols), "quartets": q, "persistent_generic_ms": pers_ms, "per_class_ms": cls_ms,
+            "persistent_generic_quartets_per_s": q / (pers_ms * 1e-3), "max_abs_diff": diff}
 
 
 def bench_batch(args):
@@ -293,6 +297,19 @@ def bench_batch(args):
     de_s = torch.cat(de).clone()
     surv = b.last_quartets()
     kern_ms, _ = timed_batch_pass(b, dms, 2)
+    # the same pass as ONE persistent launch of the register-resident bodies over every
+    # class of every molecule (plus one generic-body launch for the classes without a
+    # register body), against the per-class launch set above
+    b.set_persistent(True, body="register", order=args.persistent_order)
+    b.grad2e_separable(dms)           # first call: the persistent work items
+    reg_ms, de_r = timed_batch_pass(b, dms, 1)
+    launches = b.last_launches()
+    reg_diff = float((torch.cat(de_r) - de_s).abs().max())
+    b.set_persistent(False)
+    n_reg = sum(1 for r in launches if r["kernel"] == 3)
+    log(f"one persistent launch (register bodies, {args.persistent_order} order, {n_reg} of {len(launches)} "
+        f"classes; the rest in one generic-body launch) {reg_ms:.1f} ms vs per-class launches {kern_ms:.1f} ms "
+        f"(ratio {reg_ms / kern_ms:.2f}, max |diff| {reg_diff:.1e})")
     q = b.schwarz().cpu().numpy()
     flops = 0.0
     s0 = 0
@@ -303,9 +320,8 @@ def bench_batch(args):
     achieved = flops / (kern_ms * 1e-3) / 1e12
     roof = {"bound": "valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None}
-    # the screened pass as ONE persistent launch over every class of every molecule (generic
-    # wave-per-quartet kernel, chunk length per class autotuned) against the per-class
-    # launches, on the first few molecules (the generic kernel would take minutes on all)
+    # the persistent launch with the LDS kernel's generic wave-per-quartet body against the
+    # per-class launches, on the first few molecules (it would take minutes on all)
     pers = persistent_vs_per_class(mols[:args.persistent_molecules], dms[:args.persistent_molecules], args.thr)
     # the same pass unscreened, for the error and gain of screening
     b.screen(0.0)
@@ -334,7 +350,12 @@ def bench_batch(args):
                    "schwarz_threshold": args.thr, "quartets_evaluated": surv,
                    "quartets_unscreened": b.quartets_unscreened(), "quartets_reference_equivalent": refq,
                    "unscreened_pass_ms": full_ms, "screened_pass_ms": kern_ms,
-                   "persistent_single_launch": pers,
+                   "persistent_register_launch": {"ms": reg_ms, "per_class_launches_ms": kern_ms,
+                                                  "ratio_to_per_class": reg_ms / kern_ms,
+                                                  "register_body_classes": n_reg, "classes": len(launches),
+                                                  "item_order": args.persistent_order,
+                                                  "max_abs_diff_vs_per_class": reg_diff},
+                   "persistent_generic_launch_first_molecules": pers,
                    "max_abs_diff_screened_vs_unscreened": err, "max_net_force_unscreened": net},
         "roofline": roof, "cpu_baseline": cpu}
 
@@ -697,7 +718,9 @@ def main():
     ap.add_argument("--molecules", type=int, default=64, help="batch config: number of molecules")
     ap.add_argument("--thr", type=float, default=1e-12, help="batch config: Schwarz threshold")
     ap.add_argument("--persistent-molecules", type=int, default=4,
-                    help="batch config: molecules of the persistent-launch comparison")
+                    help="batch config: molecules of the generic-body persistent-launch comparison")
+    ap.add_argument("--persistent-order", default="class", choices=["class", "cost"],
+                    help="batch config: work-item order of the register-body persistent launch")
     ap.add_argument("--waters", type=int, default=0)
     ap.add_argument("--precision", default="bf16x3", choices=["bf16x3", "fp32"])
     ap.add_argument("--copy-calibration", action="store_true",

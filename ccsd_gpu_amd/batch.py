@@ -86,11 +86,16 @@ class MoleculeBatch:
     def set_option(self, option, value):
         _lib.check(self._lib.ccsd_plan_set_option(self._plan, int(option), int(value)), "ccsd_plan_set_option")
 
-    def set_persistent(self, on=True):
+    def set_persistent(self, on=True, body="register", order="class"):
         """Run each fused pass as one persistent launch over every class of every
-        molecule (waves draw cost-ordered work items from a device counter), instead
-        of one launch per class on four streams (the default)."""
-        self.set_option(5, 1 if on else 0)
+        molecule (one-wave blocks draw work items from a device counter), instead of
+        one launch per class on four streams (the default). body="register": the
+        register-resident per-class bodies in that one kernel (the LDS kernel's body if
+        some class has none); "generic": the LDS kernel's body. order="class": work
+        items class by class, costliest class first; "cost": by decreasing cost only."""
+        from .hip.grad2e import PERSISTENT_BODIES
+        self.set_option(7, {"class": 0, "cost": 1}[order])
+        self.set_option(5, PERSISTENT_BODIES[body] if on else 0)
 
     def screen(self, thr):
         _lib.check(self._lib.ccsd_plan_screen(self._plan, float(thr), _lib.current_stream_ptr()), "ccsd_plan_screen")
@@ -100,6 +105,11 @@ class MoleculeBatch:
         q = torch.empty((nbas, nbas), dtype=torch.float64, device="cuda")
         _lib.check(self._lib.ccsd_schwarz(self._plan, _lib.ptr(q), _lib.current_stream_ptr()), "ccsd_schwarz")
         return q
+
+    def last_launches(self):
+        """Launch records of the last fused pass (hip.grad2e.last_launches)."""
+        from .hip.grad2e import last_launches
+        return last_launches(plan=self._plan)
 
     def last_quartets(self):
         n = ctypes.c_longlong()

@@ -16,9 +16,11 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "libccsd_gpu_amd.so")
 # (source, object name, extra defines)
+# the persistent-launch unit first: it holds every register body of the fused pass and
+# compiles longest, so it runs beside all the others
 UNITS = [(s, s.replace(".hip", ""), []) for s in
-         ["eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip", "fused_ip1_reg.hip",
-          "rdm_kernels.hip", "lambda_kernels.hip", "tensor_kernels.hip", "capi.hip"]]
+         ["fused_persist.hip", "eri_kernels.hip", "int1e_kernels.hip", "gemm_kernels.hip", "gemm_bf16x3.hip",
+          "fused_ip1_reg.hip", "rdm_kernels.hip", "lambda_kernels.hip", "tensor_kernels.hip", "capi.hip"]]
 UNITS += [("fused_ip1_reg_tu.hip", f"fused_ip1_reg_l{i}{j}{k}",
            [f"-DCCSD_REG_LI={i}", f"-DCCSD_REG_LJ={j}", f"-DCCSD_REG_LK={k}"])
           for i in range(3) for j in range(3) for k in range(3)]

@@ -20,6 +20,13 @@
 size_t ccsd_eri_lds_bytes(const EriClassArgs& a);
 hipError_t ccsd_launch_eri_class(const EriClassArgs& a, hipStream_t stream);
 int ccsd_persistent_blocks_per_cu(size_t lds);
+int ccsd_reg_persist_parts(int li, int lj, int lk, int ll, int max_ncij, int max_nckl);
+size_t ccsd_reg_persist_lds(int max_nij);
+int ccsd_reg_persist_blocks_per_cu(size_t lds);
+hipError_t ccsd_launch_reg_persistent(EriClassArgs* classes, int nclasses, const int4* items, int nitems,
+                                      unsigned int* counter, const double* gamma, const double* dm, const double* pm,
+                                      unsigned long long* defx, double scale, int gamma_mode, int nblocks, size_t lds,
+                                      hipStream_t stream);
 hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
                                       unsigned int* counter, const double* gamma, const double* dm,
                                       const double* pm, unsigned long long* defx, double scale, int gamma_mode,
@@ -175,18 +182,32 @@ struct Choice {
     int chunk = 64;  // ket pairs per chunk; bits 16+: primitive split of the register kernel (see get_chunks)
 };
 
-// The whole fused pass of a bra set as one persistent launch of the generic kernel:
-// the launch description of every class (work lists included) and the work items
-// (class, chunk range) in decreasing order of estimated cost.
-struct PersistPlan {
+// One persistent launch: the launch description of every class it runs (work lists
+// included) and its work items (runs of one class's units), drawn through a counter.
+struct PersistPart {
     EriClassArgs* d_cls = nullptr;
     int4* d_items = nullptr;
     unsigned int* d_counter = nullptr;   // 8 bytes, cleared by a fill kernel before each launch
-    int nitems = 0, nblocks = 0;
+    int ncls = 0, nitems = 0, nblocks = 0;
     size_t lds = 0;
+};
+
+// The whole fused pass of a bra set as persistent launches: `reg` runs the
+// register-resident bodies (plan option 5 = 1) of every class that has one, `gen` the
+// LDS kernel's generic body for the others (every class when option 5 = 2).
+struct PersistPlan {
+    PersistPart reg, gen;
     long long quartets = 0;
     std::vector<std::array<long long, 10>> launches;
 };
+
+void free_persist_part(PersistPart& q)
+{
+    if (q.d_cls) (void)hipFree(q.d_cls);
+    if (q.d_items) (void)hipFree(q.d_items);
+    if (q.d_counter) (void)hipFree(q.d_counter);
+    q = PersistPart();
+}
 
 struct BraSet {
     std::vector<PairClass> cls;
@@ -224,7 +245,10 @@ struct ccsd_plan {
     // numbering does not depend on the geometry or the screening threshold
     std::map<std::tuple<int, int, int, int>, Choice> choice;
     std::map<std::tuple<int, int, int, int>, Choice> choice_lds;   // generic kernel only (persistent pass)
-    bool persistent = false;                         // fused pass as one persistent launch
+    // fused pass as one persistent launch: 0 off, 1 register bodies (when every class of
+    // the pass has one; the generic body otherwise), 2 the generic body
+    int persistent = 0;
+    int persist_order = 0;                           // register items: 0 class by class, 1 by cost only
     int n_cu = 0;                                    // compute units of the plan's device
     bool has_c2s = false;
     bool fused_reg = true;                           // register kernels (fused gradient and s2kl store)
@@ -426,9 +450,8 @@ void free_braset(BraSet& b)
         if (kv.second.d) (void)hipFree(kv.second.d);
     b.chunks.clear();
     for (auto& kv : b.persist) {
-        if (kv.second.d_cls) (void)hipFree(kv.second.d_cls);
-        if (kv.second.d_items) (void)hipFree(kv.second.d_items);
-        if (kv.second.d_counter) (void)hipFree(kv.second.d_counter);
+        free_persist_part(kv.second.reg);
+        free_persist_part(kv.second.gen);
     }
     b.persist.clear();
 }
@@ -872,85 +895,138 @@ int ensure_side_streams(ccsd_plan* p)
     return CCSD_OK;
 }
 
-// Work items of the persistent pass: consecutive chunks of one class are grouped until
-// their estimated cost reaches 1/8 of a resident wave's share of the whole pass (a
-// chunk costlier than that is an item of its own); items sorted by decreasing cost.
-int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassArgs& proto, bool screened,
-                  hipStream_t stream, PersistPlan* out)
+// Work items of one persistent launch over `cls` (work lists `lists`, units per chunk
+// `parts`): consecutive units of one class are grouped until their estimated cost
+// reaches 1/8 of a resident wave's share of the launch (a unit costlier than that is
+// an item of its own). Order: class by class (`grouped`: the costliest class first,
+// decreasing cost inside a class), so that the waves resident on a CU mostly run one
+// body and share its instructions in cache, or by decreasing cost only.
+int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const std::vector<const ChunkList*>& lists,
+                       const std::vector<int>& parts, bool reg, bool grouped, PersistPart* out)
 {
-    // kernel choices (chunk length per class) first: the autotuner drops the work lists
-    // of the candidates it does not take, so none is referenced before all are chosen
-    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
-        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
-            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
-            a.gamma_mode = proto.gamma_mode;
-            a.gamma = proto.gamma;
-            a.dm = proto.dm;
-            a.pm = proto.pm;
-            a.gamma_scale = proto.gamma_scale;
-            Choice c;
-            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c, true);
-            if (rc) return rc;
-        }
-    // built in place in *out, so that the caller frees whatever was allocated if a step fails
-    PersistPlan& pp = *out;
-    std::vector<EriClassArgs> cls;
-    std::vector<const ChunkList*> lists;
-    double total = 0.0;
-    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
-        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
-            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
-            Choice c;
-            int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, &c, true);
-            if (rc) return rc;
-            ChunkList* cl = nullptr;
-            rc = get_chunks(p, bs, bc, kc, c.chunk, screened, true, &cl);
-            if (rc) return rc;
-            if (cl->n == 0) continue;
-            a.chunks = cl->d;
-            a.nchunks = cl->n;
-            const size_t lds = ccsd_eri_lds_bytes(a);
-            if (lds > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
-            pp.lds = std::max(pp.lds, lds);
-            pp.quartets += cl->quartets;
-            pp.launches.push_back({a.li, a.lj, a.lk, a.ll, (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0, 2, 1,
-                                   c.chunk, 0, cl->quartets});
-            for (float w : cl->cost) total += w;
-            cls.push_back(a);
-            lists.push_back(cl);
-        }
+    PersistPart& q = *out;
     if (cls.empty()) return CCSD_OK;
+    for (const EriClassArgs& a : cls) {
+        const size_t lds = reg ? ccsd_reg_persist_lds(a.max_nij) : ccsd_eri_lds_bytes(a);
+        if (lds > 65536) return fail(CCSD_ERR_UNSUPPORTED, "class needs more than 64 KiB LDS");
+        q.lds = std::max(q.lds, lds);
+    }
     if (p->n_cu == 0) HIP_TRY(hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, p->device));
-    const int bpc = ccsd_persistent_blocks_per_cu(pp.lds);
+    const int bpc = reg ? ccsd_reg_persist_blocks_per_cu(q.lds) : ccsd_persistent_blocks_per_cu(q.lds);
     if (bpc <= 0) return fail(CCSD_ERR_UNSUPPORTED, "persistent kernel cannot be resident at this LDS size");
-    pp.nblocks = bpc * p->n_cu;
-    const double target = total / ((double)pp.nblocks * 8.0);
-    std::vector<std::pair<double, int4>> items;
+    q.nblocks = bpc * p->n_cu;
+    double total = 0.0;
+    std::vector<double> cls_cost(cls.size(), 0.0);
+    for (size_t ci = 0; ci < cls.size(); ci++)
+        for (float w : lists[ci]->cost) cls_cost[ci] += w;
+    for (double w : cls_cost) total += w;
+    const double target = total / ((double)q.nblocks * 8.0);
+    // (sort key, item); the key orders classes first when items stay class by class
+    std::vector<std::pair<std::pair<double, double>, int4>> items;
     for (size_t ci = 0; ci < cls.size(); ci++) {
         const ChunkList* cl = lists[ci];
-        int c0 = 0;
+        const int np = parts[ci];
+        if ((long long)cl->n * np > 2147483647LL) return fail(CCSD_ERR_UNSUPPORTED, "work list too long");
+        const int nu = cl->n * np;
+        int u0 = 0;
         double acc = 0.0;
-        for (int c = 0; c < cl->n; c++) {
-            acc += cl->cost[c];
-            if (acc >= target || c + 1 == cl->n) {
-                items.push_back(std::make_pair(acc, make_int4((int)ci, c0, c + 1, 0)));
-                c0 = c + 1;
+        for (int u = 0; u < nu; u++) {
+            acc += cl->cost[u / np] / np;
+            if (acc >= target || u + 1 == nu) {
+                items.push_back(std::make_pair(std::make_pair(grouped ? cls_cost[ci] : 0.0, acc),
+                                               make_int4((int)ci, u0, u + 1, 0)));
+                u0 = u + 1;
                 acc = 0.0;
             }
         }
     }
     std::stable_sort(items.begin(), items.end(),
-                     [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) { return x.first > y.first; });
+                     [](const std::pair<std::pair<double, double>, int4>& x,
+                        const std::pair<std::pair<double, double>, int4>& y) {
+                         if (x.first.first != y.first.first) return x.first.first > y.first.first;
+                         if (x.first.first != 0.0 && x.second.x != y.second.x)
+                             return x.second.x < y.second.x;   // equal-cost classes stay apart
+                         return x.first.second > y.first.second;
+                     });
     std::vector<int4> h(items.size());
     for (size_t t = 0; t < items.size(); t++) h[t] = items[t].second;
-    pp.nitems = (int)h.size();
-    pp.nblocks = std::min(pp.nblocks, pp.nitems);
-    HIP_TRY(hipMalloc(&pp.d_cls, sizeof(EriClassArgs) * cls.size()));
-    HIP_TRY(hipMemcpy(pp.d_cls, cls.data(), sizeof(EriClassArgs) * cls.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&pp.d_items, sizeof(int4) * h.size()));
-    HIP_TRY(hipMemcpy(pp.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&pp.d_counter, sizeof(double)));
+    q.ncls = (int)cls.size();
+    q.nitems = (int)h.size();
+    q.nblocks = std::min(q.nblocks, q.nitems);
+    HIP_TRY(hipMalloc(&q.d_cls, sizeof(EriClassArgs) * cls.size()));
+    HIP_TRY(hipMemcpy(q.d_cls, cls.data(), sizeof(EriClassArgs) * cls.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&q.d_items, sizeof(int4) * h.size()));
+    HIP_TRY(hipMemcpy(q.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&q.d_counter, sizeof(double)));
     return CCSD_OK;
+}
+
+// The persistent pass of a bra set. Option 5 = 1: every class with a register body runs
+// in one launch of those bodies (its work list as the autotuner chose it, split along
+// the ket primitive pairs or not; P units per chunk), the classes without one in one
+// launch of the generic body; option 5 = 2: every class in one launch of the generic
+// body (one unit per chunk, chunk length from the autotuner).
+int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassArgs& proto, bool screened,
+                  hipStream_t stream, PersistPlan* out)
+{
+    const bool use_reg = p->persistent == 1 && p->fused_reg;
+    // A class runs in the register-body launch when it has a body there and the
+    // per-class choice is a register kernel (the same work list, hence the same
+    // per-wave sums); otherwise in the generic-body launch, with the generic kernel's
+    // own choice of chunk length (a register work list is not a generic one: its
+    // chunks may be pieces of a primitive split).
+    auto pick = [&](int bc, int kc, EriClassArgs a, Choice* c, bool* reg) {
+        a.gamma_mode = proto.gamma_mode;
+        a.gamma = proto.gamma;
+        a.dm = proto.dm;
+        a.pm = proto.pm;
+        a.gamma_scale = proto.gamma_scale;
+        *reg = use_reg && ccsd_reg_persist_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) > 0;
+        int rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, c, !use_reg);
+        if (rc) return rc;
+        *reg = *reg && c->kind == 1;
+        if (!*reg && c->kind == 1) rc = choose(p, bs, ish0, ish1, bc, kc, a, screened, stream, c, true);
+        return rc;
+    };
+    // kernel choices (chunk length per class) first: the autotuner drops the work lists
+    // of the candidates it does not take, so none is referenced before all are chosen
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            Choice c;
+            bool reg;
+            int rc = pick(bc, kc, class_args(p, bs.cls[bc], p->ket[kc], 1), &c, &reg);
+            if (rc) return rc;
+        }
+    // built in place in *out, so that the caller frees whatever was allocated if a step fails
+    PersistPlan& pp = *out;
+    std::vector<EriClassArgs> cls[2];
+    std::vector<const ChunkList*> lists[2];
+    std::vector<int> parts[2];
+    for (int bc = 0; bc < (int)bs.cls.size(); bc++)
+        for (int kc = 0; kc < (int)p->ket.size(); kc++) {
+            EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
+            Choice c;
+            bool reg;
+            int rc = pick(bc, kc, a, &c, &reg);
+            if (rc) return rc;
+            const int chunk = c.chunk;
+            ChunkList* cl = nullptr;
+            rc = get_chunks(p, bs, bc, kc, chunk, screened, true, &cl);
+            if (rc) return rc;
+            if (cl->n == 0) continue;
+            a.chunks = cl->d;
+            a.nchunks = cl->n;
+            const int np = reg ? ccsd_reg_persist_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) : 1;
+            pp.quartets += cl->quartets;
+            pp.launches.push_back({a.li, a.lj, a.lk, a.ll, (a.max_ncij > 1 || a.max_nckl > 1) ? 1 : 0, reg ? 3 : 2, 1,
+                                   chunk, reg ? np : 0, cl->quartets});
+            cls[reg].push_back(a);
+            lists[reg].push_back(cl);
+            parts[reg].push_back(np);
+        }
+    int rc = build_persist_part(p, cls[1], lists[1], parts[1], true, p->persist_order == 0, &pp.reg);
+    if (rc) return rc;
+    return build_persist_part(p, cls[0], lists[0], parts[0], false, false, &pp.gen);
 }
 
 // Zeroed fixed-point gradient words of a fused pass (allocated on first use).
@@ -962,7 +1038,9 @@ int defx_begin(ccsd_plan* p, hipStream_t stream)
     return CCSD_OK;
 }
 
-// The fused pass as one launch of the persistent kernel (plan option 5).
+// The fused pass as persistent launches (plan option 5): the register-body launch on
+// `stream`, the generic-body launch (classes without a register body) on a side stream
+// beside it; both add into the fixed-point gradient with integer atomics.
 int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* de, double scale, int gamma_mode,
                          const double* gamma, const double* dm, const double* pm, hipStream_t stream)
 {
@@ -979,9 +1057,8 @@ int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* d
         PersistPlan pp;
         int rc = build_persist(p, bs, ish0, ish1, proto, screened, stream, &pp);
         if (rc) {
-            if (pp.d_cls) (void)hipFree(pp.d_cls);
-            if (pp.d_items) (void)hipFree(pp.d_items);
-            if (pp.d_counter) (void)hipFree(pp.d_counter);
+            free_persist_part(pp.reg);
+            free_persist_part(pp.gen);
             return rc;
         }
         it = bs.persist.emplace(screened ? 1 : 0, pp).first;
@@ -989,12 +1066,31 @@ int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* d
     const PersistPlan& pp = it->second;
     p->last_quartets = pp.quartets;
     p->last_launches = pp.launches;
-    if (pp.nitems == 0) return CCSD_OK;
     int rc = defx_begin(p, stream);
     if (rc) return rc;
-    HIP_TRY(ccsd_launch_fill((double*)pp.d_counter, 1, 0.0, stream));
-    HIP_TRY(ccsd_launch_eri_persistent(pp.d_cls, pp.d_items, pp.nitems, pp.d_counter, gamma, dm, pm, p->d_defx,
-                                       scale, gamma_mode, pp.nblocks, pp.lds, stream));
+    const bool both = pp.reg.nitems > 0 && pp.gen.nitems > 0;
+    if (both) {
+        rc = ensure_side_streams(p);
+        if (rc) return rc;
+    }
+    hipStream_t gst = both ? p->side[0] : stream;
+    if (pp.reg.nitems > 0) HIP_TRY(ccsd_launch_fill((double*)pp.reg.d_counter, 1, 0.0, stream));
+    if (pp.gen.nitems > 0) HIP_TRY(ccsd_launch_fill((double*)pp.gen.d_counter, 1, 0.0, stream));
+    if (both) {
+        HIP_TRY(hipEventRecord(p->fork_ev, stream));
+        HIP_TRY(hipStreamWaitEvent(gst, p->fork_ev, 0));
+    }
+    if (pp.gen.nitems > 0)
+        HIP_TRY(ccsd_launch_eri_persistent(pp.gen.d_cls, pp.gen.d_items, pp.gen.nitems, pp.gen.d_counter, gamma, dm,
+                                           pm, p->d_defx, scale, gamma_mode, pp.gen.nblocks, pp.gen.lds, gst));
+    if (pp.reg.nitems > 0)
+        HIP_TRY(ccsd_launch_reg_persistent(pp.reg.d_cls, pp.reg.ncls, pp.reg.d_items, pp.reg.nitems,
+                                           pp.reg.d_counter, gamma, dm, pm, p->d_defx, scale, gamma_mode,
+                                           pp.reg.nblocks, pp.reg.lds, stream));
+    if (both) {
+        HIP_TRY(hipEventRecord(p->join_ev[0], gst));
+        HIP_TRY(hipStreamWaitEvent(stream, p->join_ev[0], 0));
+    }
     HIP_TRY(ccsd_launch_defx_to_de(p->d_defx, p->natm, de, stream));
     return CCSD_OK;
 }
@@ -1772,7 +1868,13 @@ int ccsd_plan_set_option(ccsd_plan* p, int option, int value)
         return CCSD_OK;
     }
     if (option == 5) {
-        p->persistent = value != 0;
+        if (value < 0 || value > 2) return fail(CCSD_ERR_SHAPE, "persistent mode must be 0, 1 or 2");
+        p->persistent = value;
+        clear_bra_cache(p);
+        return CCSD_OK;
+    }
+    if (option == 7) {
+        p->persist_order = value != 0;
         clear_bra_cache(p);
         return CCSD_OK;
     }

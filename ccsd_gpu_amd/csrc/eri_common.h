@@ -87,16 +87,46 @@ struct EriClassArgs {
 };
 
 #ifdef __HIPCC__
+// Global-memory pointer types for device code: a load through a pointer of this type
+// is a global load also where the compiler cannot trace the pointer to a kernel
+// argument (inside non-inlined functions, where a plain pointer would be read with
+// flat instructions, which also wait for LDS). Plain pointers in the host pass, which
+// type-checks the same templates.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CCSD_AS1 __attribute__((address_space(1)))
+#else
+#define CCSD_AS1
+#endif
+template <typename T>
+__device__ __forceinline__ const CCSD_AS1 T* gld(const T* p)
+{
+    return (const CCSD_AS1 T*)p;
+}
+template <typename T>
+__device__ __forceinline__ CCSD_AS1 T* gst(T* p)
+{
+    return (CCSD_AS1 T*)p;
+}
+
+// Three doubles (a centre's coordinates) read from global memory.
+__device__ __forceinline__ void load3(const double* p, double (&v)[3])
+{
+    const CCSD_AS1 double* g = gld(p);
+    v[0] = g[0];
+    v[1] = g[1];
+    v[2] = g[2];
+}
+
 // Where the two-particle density of one molecule lives: resolved once per quartet, so
 // that the element reads below carry no dependent metadata loads of their own and can
 // all be in flight together.
 struct GammaSrc {
-    const double* gamma;
+    const CCSD_AS1 double* gamma;
     long long nao, npair;
     int mode;
     int o, n;            // molecule's first Cartesian AO and AO count (separable part)
-    const double* D;
-    const double* P;
+    const CCSD_AS1 double* D;
+    const CCSD_AS1 double* P;
 };
 
 // Fixed-point gradient accumulation. A contribution x is held as two 64-bit integers,
@@ -131,7 +161,7 @@ __device__ __forceinline__ void defx_add(unsigned long long* defx, int atom, dou
 __device__ __forceinline__ GammaSrc gamma_src(const EriClassArgs& a, int mol)
 {
     GammaSrc s;
-    s.gamma = a.gamma;
+    s.gamma = gld(a.gamma);
     s.nao = a.nao;
     s.npair = a.npair;
     s.mode = a.gamma_mode;
@@ -140,10 +170,10 @@ __device__ __forceinline__ GammaSrc gamma_src(const EriClassArgs& a, int mol)
     s.D = nullptr;
     s.P = nullptr;
     if (s.mode != 0) {
-        s.o = a.mol_ao0[mol];
-        s.n = a.mol_nao[mol];
-        s.D = a.dm + a.mol_dm_off[mol];
-        s.P = a.pm + a.mol_dm_off[mol];
+        s.o = gld(a.mol_ao0)[mol];
+        s.n = gld(a.mol_nao)[mol];
+        s.D = gld(a.dm) + gld(a.mol_dm_off)[mol];
+        s.P = gld(a.pm) + gld(a.mol_dm_off)[mol];
     }
     return s;
 }
@@ -156,8 +186,8 @@ __device__ __forceinline__ double gamma_at(const GammaSrc& s, int i, int j, int 
                                   : 0.0;
     if (s.mode == 0) return ge;
     const int n = s.n;
-    const double* D = s.D;
-    const double* P = s.P;
+    const CCSD_AS1 double* D = s.D;
+    const CCSD_AS1 double* P = s.P;
     i -= s.o; j -= s.o; k -= s.o; l -= s.o;
     const double g = D[i * n + j] * P[k * n + l] + P[i * n + j] * D[k * n + l] -
                      0.25 * (D[i * n + l] * P[k * n + j] + D[i * n + k] * P[l * n + j] + P[i * n + l] * D[j * n + k] +

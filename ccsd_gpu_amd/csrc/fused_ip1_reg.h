@@ -56,10 +56,7 @@ __device__ __forceinline__ double shell_norm_dev(int l)
     return l == 0 ? 0.28209479177387814347 : (l == 1 ? 0.48860251190291992159 : 1.0);
 }
 
-#ifndef CCSD_REG_BUDGET
-#define CCSD_REG_BUDGET 110
-#endif
-constexpr int kBudget = CCSD_REG_BUDGET;   // doubles a lane may hold for Gamma and the 1-D factors
+constexpr int kBudget = 110;   // doubles a lane may hold for Gamma and the 1-D factors
 constexpr int kMaxParts = 8;   // beyond this the class stays on the LDS kernel
 
 constexpr int ncart_c(int l) { return (l + 1) * (l + 2) / 2; }
@@ -115,13 +112,15 @@ constexpr int need(int li, int lj, int lk, int ll, int gc, int p)
     return worst;
 }
 
-// Smallest part count that fits the budget (0: class not supported).
-constexpr int min_parts(int li, int lj, int lk, int ll, int gc)
+// Smallest part count that fits `budget` doubles (0: class not supported). The budget
+// is an argument, not a per-file macro: this inline function must mean the same in
+// every object it is compiled into.
+constexpr int min_parts(int li, int lj, int lk, int ll, int gc, int budget = kBudget)
 {
     if (li > 2 || lj > 2 || lk > 2 || ll > 2) return 0;
     const int nf = ncart_c(li) * ncart_c(lj) * ncart_c(lk) * ncart_c(ll);
     for (int p = 1; p <= kMaxParts && p <= nf; p *= 2)
-        if (need(li, lj, lk, ll, gc, p) <= kBudget) return p;
+        if (need(li, lj, lk, ll, gc, p) <= budget) return p;
     return 0;
 }
 
@@ -191,10 +190,11 @@ __device__ __forceinline__ int fill_bra(const EriClassArgs& a, const ShellInfo& 
 {
     const int nij = si.nprim * sj.nprim;
     const double rab2 = AB[0] * AB[0] + AB[1] * AB[1] + AB[2] * AB[2];
-    const double* ei = a.env + si.pexp;
-    const double* ej = a.env + sj.pexp;
-    const double* cfi = a.env + si.pcoef;
-    const double* cfj = a.env + sj.pcoef;
+    const CCSD_AS1 double* env = gld(a.env);
+    const CCSD_AS1 double* ei = env + si.pexp;
+    const CCSD_AS1 double* ej = env + sj.pexp;
+    const CCSD_AS1 double* cfi = env + si.pcoef;
+    const CCSD_AS1 double* cfj = env + sj.pcoef;
     const int ncij = si.nctr * sj.nctr;
     int count = 0;
     for (int p0 = 0; p0 < nij; p0 += 64) {
@@ -250,14 +250,16 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
 
     const int kq = ch.y + lane;
     if (kq >= ch.z) return;
-    const int2 bp = a.bra[ch.x];
-    const int2 kp = a.ket[kq];
-    const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
-    const ShellInfo sk = a.shells[kp.x], sl = a.shells[kp.y];
-    const double* RA = a.coords + 3 * si.atom;
-    const double* RB = a.coords + 3 * sj.atom;
-    const double* RC = a.coords + 3 * sk.atom;
-    const double* RD = a.coords + 3 * sl.atom;
+    const int2 bp = gld(a.bra)[ch.x];
+    const int2 kp = gld(a.ket)[kq];
+    const CCSD_AS1 ShellInfo* shells = gld(a.shells);
+    const ShellInfo si = shells[bp.x], sj = shells[bp.y];
+    const ShellInfo sk = shells[kp.x], sl = shells[kp.y];
+    double RA[3], RB[3], RC[3], RD[3];
+    load3(a.coords + 3 * si.atom, RA);
+    load3(a.coords + 3 * sj.atom, RB);
+    load3(a.coords + 3 * sk.atom, RC);
+    load3(a.coords + 3 * sl.atom, RD);
     const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
     const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
     const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
@@ -291,10 +293,11 @@ __device__ __forceinline__ void quartet_part(const EriClassArgs& a, const int4 c
 
     // ket primitive pairs past the cut, precomputed per ket pair (no divergent skips);
     // a chunk with a primitive split (w = s << 8 | S) takes pairs s, s + S, ...
-    const int t1 = a.ket_pp_off[kq + 1];
+    const CCSD_AS1 int* pp_off = gld(a.ket_pp_off);
+    const int t1 = pp_off[kq + 1];
     const int split = ch.w & 255, sstep = split > 1 ? split : 1;
-    for (int t = a.ket_pp_off[kq] + (split > 1 ? (ch.w >> 8) : 0); t < t1; t += sstep) {
-            const double* kpd = a.ket_pp + (size_t)t * 10;
+    for (int t = pp_off[kq] + (split > 1 ? (ch.w >> 8) : 0); t < t1; t += sstep) {
+            const CCSD_AS1 double* kpd = gld(a.ket_pp) + (size_t)t * 10;
             const double akl = kpd[0];
             const double Qc[3] = {kpd[1], kpd[2], kpd[3]};
             const double KcdN = kpd[4];
@@ -400,14 +403,16 @@ __global__ void __launch_bounds__(256) fused_reg_kernel(const EriClassArgs a)
         const int c = gw / P;
         const bool active = c < a.nchunks;   // every wave reaches the barrier below
         const int part = gw - c * P;
-        const int4 ch = active ? a.chunks[c] : make_int4(0, 0, 0, 0);
+        int4 ch = make_int4(0, 0, 0, 0);
+        if (active) ch = gld(a.chunks)[c];
         double* tab = lds_tab + (size_t)wave * a.max_nij * kBraFields;
-        const int2 bp = a.bra[ch.x];
-        const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+        const int2 bp = gld(a.bra)[ch.x];
+        const ShellInfo si = gld(a.shells)[bp.x], sj = gld(a.shells)[bp.y];
         int nij = 0;
         if (active) {
-            const double* RA = a.coords + 3 * si.atom;
-            const double* RB = a.coords + 3 * sj.atom;
+            double RA[3], RB[3];
+            load3(a.coords + 3 * si.atom, RA);
+            load3(a.coords + 3 * sj.atom, RB);
             const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
             const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
                                 shell_norm_dev(LL);
@@ -497,14 +502,16 @@ __device__ __forceinline__ void quartet_store(const EriClassArgs& a, const int4 
 
     const int kq = ch.y + lane;
     if (kq >= ch.z) return;
-    const int2 bp = a.bra[ch.x];
-    const int2 kp = a.ket[kq];
-    const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
-    const ShellInfo sk = a.shells[kp.x], sl = a.shells[kp.y];
-    const double* RA = a.coords + 3 * si.atom;
-    const double* RB = a.coords + 3 * sj.atom;
-    const double* RC = a.coords + 3 * sk.atom;
-    const double* RD = a.coords + 3 * sl.atom;
+    const int2 bp = gld(a.bra)[ch.x];
+    const int2 kp = gld(a.ket)[kq];
+    const CCSD_AS1 ShellInfo* shells = gld(a.shells);
+    const ShellInfo si = shells[bp.x], sj = shells[bp.y];
+    const ShellInfo sk = shells[kp.x], sl = shells[kp.y];
+    double RA[3], RB[3], RC[3], RD[3];
+    load3(a.coords + 3 * si.atom, RA);
+    load3(a.coords + 3 * sj.atom, RB);
+    load3(a.coords + 3 * sk.atom, RC);
+    load3(a.coords + 3 * sl.atom, RD);
     const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
     const double CD[3] = {RD[0] - RC[0], RD[1] - RC[1], RD[2] - RC[2]};
     const int ncij = si.nctr * sj.nctr, nckl = sk.nctr * sl.nctr;
@@ -519,9 +526,10 @@ __device__ __forceinline__ void quartet_store(const EriClassArgs& a, const int4 
 #pragma unroll
                 for (int c = 0; c < NC; c++) acc[cij][ckl][t][c] = 0.0;
 
-    const int t1 = a.ket_pp_off[kq + 1];
-    for (int t = a.ket_pp_off[kq]; t < t1; t++) {
-        const double* kpd = a.ket_pp + (size_t)t * 10;
+    const CCSD_AS1 int* pp_off = gld(a.ket_pp_off);
+    const int t1 = pp_off[kq + 1];
+    for (int t = pp_off[kq]; t < t1; t++) {
+        const CCSD_AS1 double* kpd = gld(a.ket_pp) + (size_t)t * 10;
         const double akl = kpd[0];
         const double Qc[3] = {kpd[1], kpd[2], kpd[3]};
         const double KcdN = kpd[4];
@@ -626,7 +634,7 @@ __device__ __forceinline__ void quartet_store(const EriClassArgs& a, const int4 
                 const long long idx =
                     ((long long)(i - a.i_base) * a.nao + j) * a.npair + (long long)k * (k + 1) / 2 + l;
 #pragma unroll
-                for (int c = 0; c < NC; c++) a.out[c * a.comp_stride + idx] = acc[cij][ckl][t][c];
+                for (int c = 0; c < NC; c++) gst(a.out)[c * a.comp_stride + idx] = acc[cij][ckl][t][c];
             }
         }
 }
@@ -653,14 +661,16 @@ __global__ void __launch_bounds__(256) store_reg_kernel(const EriClassArgs a)
         const int c = gw / P;
         const bool active = c < a.nchunks;   // every wave reaches the barrier below
         const int part = gw - c * P;
-        const int4 ch = active ? a.chunks[c] : make_int4(0, 0, 0, 0);
+        int4 ch = make_int4(0, 0, 0, 0);
+        if (active) ch = gld(a.chunks)[c];
         double* tab = lds_tab + (size_t)wave * a.max_nij * kBraFields;
-        const int2 bp = a.bra[ch.x];
-        const ShellInfo si = a.shells[bp.x], sj = a.shells[bp.y];
+        const int2 bp = gld(a.bra)[ch.x];
+        const ShellInfo si = gld(a.shells)[bp.x], sj = gld(a.shells)[bp.y];
         int nij = 0;
         if (active) {
-            const double* RA = a.coords + 3 * si.atom;
-            const double* RB = a.coords + 3 * sj.atom;
+            double RA[3], RB[3];
+            load3(a.coords + 3 * si.atom, RA);
+            load3(a.coords + 3 * sj.atom, RB);
             const double AB[3] = {RB[0] - RA[0], RB[1] - RA[1], RB[2] - RA[2]};
             const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
                                 shell_norm_dev(LL);

@@ -95,10 +95,16 @@ def add_separable(gamma_cart, dm, pm):
     return gamma_cart
 
 
-def set_persistent(mol, on=True):
-    """Fused passes of `mol`'s plan as one persistent launch of the generic kernel over
-    every class (True) or one launch per class on four streams (False, the default)."""
-    _lib.check(_lib.load().ccsd_plan_set_option(_lib.get_plan(mol), 5, 1 if on else 0), "ccsd_plan_set_option")
+PERSISTENT_BODIES = {"register": 1, "generic": 2}
+
+
+def set_persistent(mol, on=True, body="register"):
+    """Fused passes of `mol`'s plan as one persistent launch over every class (True) or
+    one launch per class on four streams (False, the default). body="register" runs the
+    register-resident per-class bodies in that one kernel (when every class of the pass
+    has one; the LDS kernel's body otherwise), "generic" the LDS kernel's body."""
+    mode = PERSISTENT_BODIES[body] if on else 0
+    _lib.check(_lib.load().ccsd_plan_set_option(_lib.get_plan(mol), 5, mode), "ccsd_plan_set_option")
 
 
 def schwarz_factors(mol):

@@ -78,13 +78,18 @@ int ccsd_grad_ip1_gamma(ccsd_plan* plan, double* de, const double* gamma, double
  * most ~1/2048 of the class's work; the autotuner decides per class), 1 turns
  * the split off, S = 2..254 splits every chunk S ways (with option 1 = 0 that
  * split register kernel is the one taken). option 5: 1 runs a fused pass as ONE
- * persistent launch of the LDS kernel over every class (waves draw work items,
- * runs of one class's chunks ordered by decreasing cost, from a device counter
- * until the list is exhausted; chunk length per class from the autotuner), 0
- * (default) one launch per class as above. option 6: 0 (default) times only the
- * register-kernel candidates of classes that have them (the generic kernel is
- * timed for the other classes), 1 times the generic candidates everywhere (a
- * slower first pass of a plan: their work lists are the largest). */
+ * persistent launch over every class of every molecule (one-wave blocks draw
+ * work items, runs of one class's units, from a device counter until the list
+ * is exhausted): with the register-resident bodies of every class in one
+ * kernel when each class of the pass has one (work lists as the autotuner
+ * chose them), with the LDS kernel's body otherwise; 2 the same with the LDS
+ * kernel's body always; 0 (default) one launch per class as above. option 6:
+ * 0 (default) times only the register-kernel candidates of classes that have
+ * them (the generic kernel is timed for the other classes), 1 times the
+ * generic candidates everywhere (a slower first pass of a plan: their work
+ * lists are the largest). option 7: order of the register-body persistent
+ * launch's work items: 0 (default) class by class (costliest class first),
+ * 1 by decreasing cost only. */
 int ccsd_plan_set_option(ccsd_plan* plan, int option, int value);
 
 /* Plan over a batch of molecules: atm/bas/env concatenated (shell atom indices
