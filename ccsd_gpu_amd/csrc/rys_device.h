@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#define RYS_STORAGE static __device__
+// 16-byte aligned: a series' coefficients are read as pairs (every series starts at a
+// multiple of RYS_DEGREE + 1 = 14 doubles)
+#define RYS_STORAGE static __device__ __attribute__((aligned(16)))
 #include "rys_table.inc"
 
 namespace ccsd {
@@ -20,12 +22,23 @@ __device__ inline void rys_roots_dev(int n, double T, double* __restrict__ r, do
         }
         return;
     }
+    static_assert((RYS_DEGREE + 1) % 2 == 0, "coefficients are read in pairs");
+    constexpr int NP = (RYS_DEGREE + 1) / 2;
     const int it = (int)(T * RYS_INV_WIDTH);
     const double u = (T - it * RYS_WIDTH) * (2.0 / RYS_WIDTH) - 1.0;
     const double u2 = 2.0 * u;
-    const double* c = RYS_TABLE + RYS_OFFSET[n - 1] + (long)it * 2 * n * (RYS_DEGREE + 1);
+    // the lanes of a wave read different intervals: one 16-byte gather per coefficient
+    // pair halves the vector memory instructions of the 8-byte form
+    const double2* c = reinterpret_cast<const double2*>(RYS_TABLE + RYS_OFFSET[n - 1] +
+                                                        (long)it * 2 * n * (RYS_DEGREE + 1));
     for (int s = 0; s < 2 * n; s++) {
-        const double* cs = c + s * (RYS_DEGREE + 1);
+        double cs[RYS_DEGREE + 1];
+#pragma unroll
+        for (int h = 0; h < NP; h++) {
+            const double2 v = c[s * NP + h];
+            cs[2 * h] = v.x;
+            cs[2 * h + 1] = v.y;
+        }
         double b1 = 0.0, b2 = 0.0;
 #pragma unroll
         for (int k = RYS_DEGREE; k >= 1; k--) {
