@@ -79,6 +79,8 @@ hipError_t ccsd_launch_combine(int ndim, const long long* sizes, double* dst, co
 hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, long long sy0, long long sy1,
                                 double c1, double c2, double c3, double c4, double* ws, double* out,
                                 hipStream_t stream);
+hipError_t ccsd_launch_l2_finish(const double* X, int o, int v, const double* u, const double* w, const double* S,
+                                 const double* sp, double sh, const double* D, double* out, hipStream_t stream);
 hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o, int v, double c1, double c2,
                                     double c3, double c4, const double* D, double* out, const long long* os, int beta,
                                     hipStream_t stream);
@@ -180,6 +182,11 @@ struct Choice {
     int kind = -1;   // 0 generic LDS kernel, 1 register kernel
     int wpb = 4;     // waves per block (register kernel)
     int chunk = 64;  // ket pairs per chunk; bits 16+: primitive split of the register kernel (see get_chunks)
+    // measured time per unit of the work list's cost (primitive products x quartets) of
+    // the chosen kernel, 0 when it was not timed: converts the within-class cost model
+    // into time across classes (a d-shell quartet costs far more per primitive product
+    // than an s-shell one)
+    double ms_per_cost = 0.0;
 };
 
 // One persistent launch: the launch description of every class it runs (work lists
@@ -756,6 +763,9 @@ int choose(ccsd_plan* p, BraSet& bs, int ish0, int ish1, int bc, int kc, EriClas
             if (per_q < best_t) {
                 best_t = per_q;
                 best = c;
+                double csum = 0.0;
+                for (int t2 = 0; t2 < nsample; t2++) csum += cl->cost[t2];
+                best.ms_per_cost = csum > 0.0 ? ms / csum : 0.0;
             }
         }
         (void)hipEventDestroy(e0);
@@ -902,7 +912,8 @@ int ensure_side_streams(ccsd_plan* p)
 // decreasing cost inside a class), so that the waves resident on a CU mostly run one
 // body and share its instructions in cache, or by decreasing cost only.
 int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const std::vector<const ChunkList*>& lists,
-                       const std::vector<int>& parts, bool reg, bool grouped, PersistPart* out)
+                       const std::vector<int>& parts, const std::vector<double>& wt, bool reg, bool grouped,
+                       PersistPart* out)
 {
     PersistPart& q = *out;
     if (cls.empty()) return CCSD_OK;
@@ -915,10 +926,16 @@ int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const
     const int bpc = reg ? ccsd_reg_persist_blocks_per_cu(q.lds) : ccsd_persistent_blocks_per_cu(q.lds);
     if (bpc <= 0) return fail(CCSD_ERR_UNSUPPORTED, "persistent kernel cannot be resident at this LDS size");
     q.nblocks = bpc * p->n_cu;
+    // the work lists' costs in time units: each class's cost model (primitive products x
+    // quartets) times its weight, so that items of different classes take about equally
+    // long (unweighted, a run of d-shell units cut to the same cost as an s-shell run took
+    // tens of times longer, and a few such items held the launch for most of its time)
     double total = 0.0;
     std::vector<double> cls_cost(cls.size(), 0.0);
-    for (size_t ci = 0; ci < cls.size(); ci++)
+    for (size_t ci = 0; ci < cls.size(); ci++) {
         for (float w : lists[ci]->cost) cls_cost[ci] += w;
+        cls_cost[ci] *= wt[ci];
+    }
     for (double w : cls_cost) total += w;
     const double target = total / ((double)q.nblocks * 8.0);
     // (sort key, item); the key orders classes first when items stay class by class
@@ -931,7 +948,7 @@ int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const
         int u0 = 0;
         double acc = 0.0;
         for (int u = 0; u < nu; u++) {
-            acc += cl->cost[u / np] / np;
+            acc += cl->cost[u / np] / np * wt[ci];
             if (acc >= target || u + 1 == nu) {
                 items.push_back(std::make_pair(std::make_pair(grouped ? cls_cost[ci] : 0.0, acc),
                                                make_int4((int)ci, u0, u + 1, 0)));
@@ -1002,6 +1019,8 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
     std::vector<EriClassArgs> cls[2];
     std::vector<const ChunkList*> lists[2];
     std::vector<int> parts[2];
+    std::vector<double> wt[2];   // measured time per cost unit of each class (0: not timed)
+    std::vector<double> model[2];   // the fallback: Rys roots x Cartesian functions
     for (int bc = 0; bc < (int)bs.cls.size(); bc++)
         for (int kc = 0; kc < (int)p->ket.size(); kc++) {
             EriClassArgs a = class_args(p, bs.cls[bc], p->ket[kc], 1);
@@ -1023,10 +1042,24 @@ int build_persist(ccsd_plan* p, BraSet& bs, int ish0, int ish1, const EriClassAr
             cls[reg].push_back(a);
             lists[reg].push_back(cl);
             parts[reg].push_back(np);
+            // a register body with more parts here than in its per-class kernel repeats
+            // the roots and 1-D factors per part: count that as half a chunk per extra part
+            const int np0 = reg ? ccsd_fused_reg_parts(a.li, a.lj, a.lk, a.ll, a.max_ncij, a.max_nckl) : 1;
+            const double extra = (reg && np0 > 0 && np > np0) ? 1.0 + 0.5 * ((double)np / np0 - 1.0) : 1.0;
+            wt[reg].push_back(c.ms_per_cost * extra);
+            const int L = a.li + a.lj + a.lk + a.ll + 1;
+            model[reg].push_back((double)(L / 2 + 1) * ncart(a.li) * ncart(a.lj) * ncart(a.lk) * ncart(a.ll) * extra);
         }
-    int rc = build_persist_part(p, cls[1], lists[1], parts[1], true, p->persist_order == 0, &pp.reg);
+    // one scale per launch: the measured weights where every class of it was timed
+    for (int r = 0; r < 2; r++)
+        for (double w : wt[r])
+            if (!(w > 0.0)) {
+                wt[r] = model[r];
+                break;
+            }
+    int rc = build_persist_part(p, cls[1], lists[1], parts[1], wt[1], true, p->persist_order == 0, &pp.reg);
     if (rc) return rc;
-    return build_persist_part(p, cls[0], lists[0], parts[0], false, false, &pp.gen);
+    return build_persist_part(p, cls[0], lists[0], parts[0], wt[0], false, false, &pp.gen);
 }
 
 // Zeroed fixed-point gradient words of a fused pass (allocated on first use).
@@ -1623,6 +1656,19 @@ int ccsd_vvvv_pm_unpack(const double* x, int o, int v, long long ldx, double sca
     if (o < 0 || v < 0 || ldx < (long long)v * (v + 1) / 2) return fail(CCSD_ERR_SHAPE, "bad dimensions");
     hipError_t e = ccsd_launch_vvvv_pm_unpack(x, o, v, ldx, scale, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "vvvv_pm_unpack_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_l2_finish(const double* x, int o, int v, const double* u, const double* w, const double* s,
+                   const double* sdev, double shost, const double* d, double* out, void* stream)
+{
+    if (!x || !u || !w || !d || !out) return fail(CCSD_ERR_NULL, "null pointer");
+    if (o < 0 || v < 0) return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    if (out == x) return fail(CCSD_ERR_SHAPE, "out must not overlap x");
+    const long long nt = (v + 31) / 32;
+    if ((long long)o * o * nt * nt > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "too many tiles for one launch");
+    hipError_t e = ccsd_launch_l2_finish(x, o, v, u, w, s, sdev, shost, d, out, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "l2_finish_kernel");
     return CCSD_OK;
 }
 

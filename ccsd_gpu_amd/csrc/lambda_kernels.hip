@@ -306,32 +306,119 @@ __global__ void __launch_bounds__(256) vvvv_pm_pack_kernel(const double* __restr
     }
 }
 
-// out[i,j,a,b] = scale X[a,b] of the pair (min(i,j), max(i,j)) (transposed when i > j)
-// from X+- (2, npair, ldx); on i = j the symmetrised block, scale X+[Q] / 2.
+// out (o, o, v, v) from X+- (2, npair, ldx), npair = o(o+1)/2 pairs i <= j in triu order.
+// With P = (X+ + X-)[Q] / 2 and M = (X+ - X-)[Q] / 2 at Q = (a, b), a >= b, the pair
+// (i, j), i < j, gives out[i,j,a,b] = P, out[i,j,b,a] = M, out[j,i,b,a] = P and
+// out[j,i,a,b] = M (M only for a > b; X-[Q] is zero on a = b); the pair (i, i) gives
+// out[i,i,a,b] = out[i,i,b,a] = X+[Q] / 2; everything times `scale`. One block per (pair,
+// 32 x 32 tile (A, B) with A >= B of the v x v block): the packed rows of a tile are
+// read along b, the entries a >= b written straight, the mirrored ones (b, a) from an
+// LDS copy, so that every global read and write runs along the last index.
 __global__ void __launch_bounds__(256) vvvv_pm_unpack_kernel(const double* __restrict__ X, int o, int v,
                                                               long long ldx, double scale, double* __restrict__ out)
 {
+    __shared__ double tp[PT][PT + 1];   // P (or X+ / 2 on i = j) at [a - a0][b - b0]
+    __shared__ double tm[PT][PT + 1];   // M
+    const int nt = (v + PT - 1) / PT;
+    const int ntri = nt * (nt + 1) / 2;
+    const long long r = blockIdx.x / ntri;
+    const int t = (int)(blockIdx.x - r * ntri);
+    int A = 0;
+    while ((A + 1) * (A + 2) / 2 <= t) A++;
+    const int B = t - A * (A + 1) / 2;
+    // pair (i, j) of triu row r: row i starts at i o - i (i - 1) / 2
+    int i = 0;
+    while (i + 1 < o && (long long)(i + 1) * o - (long long)(i + 1) * i / 2 <= r) i++;
+    const int j = i + (int)(r - ((long long)i * o - (long long)i * (i - 1) / 2));
+    const int a0 = A * PT, b0 = B * PT;
+    const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;   // 32 x 8
     const long long vv = (long long)v * v;
-    const long long n = (long long)o * o * vv;
     const long long plane = (long long)o * (o + 1) / 2 * ldx;
-    for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
-        const long long ij = t / vv, ab = t - ij * vv;
-        const int i = (int)(ij / o), j = (int)(ij - (long long)i * o);
-        const int a = (int)(ab / v), b = (int)(ab - (long long)a * v);
-        const int pi = min(i, j), pj = max(i, j);
-        const long long r = (long long)pi * o - (long long)pi * (pi - 1) / 2 + (pj - pi);   // triu order
-        const int p = i <= j ? a : b, q = i <= j ? b : a;                                   // X[p, q]
-        const int hi = max(p, q), lw = min(p, q);
-        const long long Q = (long long)hi * (hi + 1) / 2 + lw;
-        const double xp = X[r * ldx + Q];
-        double val;
-        if (i == j) {
-            val = 0.5 * xp;
-        } else {
-            const double xm = X[plane + r * ldx + Q];
-            val = 0.5 * (p >= q ? xp + xm : xp - xm);
+    const double* Xp = X + r * ldx;
+    const double* Xm = Xp + plane;
+    double* Oij = out + ((long long)i * o + j) * vv;
+    double* Oji = out + ((long long)j * o + i) * vv;
+    const bool diag = i == j;
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = a0 + ty + k, b = b0 + tx;
+        double P = 0.0, M = 0.0;
+        if (a < v && b <= a) {
+            const long long Q = (long long)a * (a + 1) / 2 + b;
+            const double xp = Xp[Q];
+            const double xm = diag ? 0.0 : Xm[Q];
+            P = 0.5 * (xp + xm);
+            M = 0.5 * (xp - xm);
+            Oij[(long long)a * v + b] = scale * P;
+            if (!diag) Oji[(long long)a * v + b] = scale * (a > b ? M : P);
         }
-        out[t] = scale * val;
+        tp[ty + k][tx] = P;
+        tm[ty + k][tx] = M;
+    }
+    __syncthreads();
+    // the mirrored entries: (row b, column a) for a > b, a = a0 + tx, b = b0 + ty + k
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = a0 + tx, b = b0 + ty + k;
+        if (a < v && b < a) {
+            const double P = tp[tx][ty + k], M = tm[tx][ty + k];
+            Oij[(long long)b * v + a] = scale * (diag ? P : M);
+            if (!diag) Oji[(long long)b * v + a] = scale * P;
+        }
+    }
+}
+
+// out[i,j,a,b] = (X[i,j,a,b] + X[j,i,b,a] + u[i,a] w[j,b] + u[j,b] w[i,a] + s S[i,j,a,b]) D[i,j,a,b]:
+// the closing symmetrisation of the lambda update (divide by D_ijab) with the two terms
+// that enter l2 just before it folded in (a rank-one l1 x w4 term and a fixed integral
+// term weighted by s; S already symmetric under (i,a) <-> (j,b)). X, S, D and out
+// contiguous (o, o, v, v), u and w contiguous (o, v); S may be NULL; s = *sp when sp is
+// given (a device scalar read when the kernel runs, so one captured graph serves any
+// weight), else sh. One block per (i, j, 32 x 32 tile); X[j,i] is transposed through LDS.
+__global__ void __launch_bounds__(256) l2_finish_kernel(const double* __restrict__ X, int o, int v,
+                                                         const double* __restrict__ u, const double* __restrict__ w,
+                                                         const double* __restrict__ S, const double* __restrict__ sp,
+                                                         double sh, const double* __restrict__ D,
+                                                         double* __restrict__ out)
+{
+    __shared__ double tji[PT][PT + 1];   // X[j][i][b][a] for the tile, indexed [b - b0][a - a0]
+    const int nt = (v + PT - 1) / PT;
+    const long long blk = blockIdx.x;
+    const int tile = (int)(blk % ((long long)nt * nt));
+    const long long ij = blk / ((long long)nt * nt);
+    const int i = (int)(ij / o), j = (int)(ij - (ij / o) * o);
+    const int a0 = (tile / nt) * PT, b0 = (tile - (tile / nt) * nt) * PT;
+    const int tx = threadIdx.x & (PT - 1), ty = threadIdx.x >> 5;   // 32 x 8
+    const long long vv = (long long)v * v;
+    const double* Xij = X + ((long long)i * o + j) * vv;
+    const double* Xji = X + ((long long)j * o + i) * vv;
+    const int vm = v - 1;
+    // clamped reads (results of clamped lanes are never stored)
+    double t1v[PT / 8];
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) t1v[k / 8] = Xji[(long long)min(b0 + ty + k, vm) * v + min(a0 + tx, vm)];
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) tji[ty + k][tx] = t1v[k / 8];
+    __syncthreads();
+    const double s = sp ? *sp : sh;
+    const int bc = min(b0 + tx, vm);
+    const double ujb = u[(long long)j * v + bc], wjb = w[(long long)j * v + bc];
+    const long long base = ((long long)i * o + j) * vv;
+    double xv[PT / 8], dv[PT / 8], sv[PT / 8], uw[PT / 8];
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = min(a0 + ty + k, vm);
+        const long long e = (long long)a * v + bc;
+        xv[k / 8] = Xij[e];
+        dv[k / 8] = D[base + e];
+        sv[k / 8] = S ? S[base + e] : 0.0;
+        uw[k / 8] = u[(long long)i * v + a] * wjb + ujb * w[(long long)i * v + a];
+    }
+#pragma unroll
+    for (int k = 0; k < PT; k += 8) {
+        const int a = a0 + ty + k, b = b0 + tx;
+        if (a < v && b < v)
+            out[base + (long long)a * v + b] = (xv[k / 8] + tji[tx][ty + k] + uw[k / 8] + s * sv[k / 8]) * dv[k / 8];
     }
 }
 
@@ -357,11 +444,22 @@ hipError_t ccsd_launch_vvvv_pm_pack(const double* l2, int o, int v, const long l
 hipError_t ccsd_launch_vvvv_pm_unpack(const double* X, int o, int v, long long ldx, double scale, double* out,
                                       hipStream_t stream)
 {
-    const long long n = (long long)o * o * v * v;
-    if (n <= 0) return hipSuccess;
-    long long nb = (n + 255) / 256;
-    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    if (o <= 0 || v <= 0) return hipSuccess;
+    const long long nt = (v + PT - 1) / PT;
+    const long long nb = (long long)o * (o + 1) / 2 * (nt * (nt + 1) / 2);
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(vvvv_pm_unpack_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, o, v, ldx, scale, out);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_l2_finish(const double* X, int o, int v, const double* u, const double* w, const double* S,
+                                 const double* sp, double sh, const double* D, double* out, hipStream_t stream)
+{
+    if (o <= 0 || v <= 0) return hipSuccess;
+    const long long nt = (v + PT - 1) / PT;
+    const long long nb = (long long)o * o * nt * nt;
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(l2_finish_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, o, v, u, w, S, sp, sh, D, out);
     return hipGetLastError();
 }
 

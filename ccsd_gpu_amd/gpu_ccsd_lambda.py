@@ -30,7 +30,7 @@ import torch
 
 from . import mfma
 from .cc import DIIS, amplitudes_to_vector, vector_to_amplitudes
-from .pairops import l1_terms, pair_combine
+from .pairops import l1_terms, l2_finish, pair_combine
 from .precision import contract, contract_acc, refresh_static, split_memo, static_operands, vvvv_term
 from .scf import DEV
 
@@ -158,7 +158,11 @@ def _static_pieces(imds, t1, t2, eris, level_shift):
     imds.theta = (t2 * 2 - t2.permute(0, 1, 3, 2)).contiguous()
     imds.tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
     imds.tmp_voov = (imds.wVOov * 2 + imds.wvOOv).contiguous()
-    imds.l2_const = (eris.ovvo.permute(0, 3, 1, 2) * 0.5).contiguous()     # voov.permute(1, 2, 0, 3) / 2
+    # the bare integral term of l2new, voov.permute(1, 2, 0, 3) / 2, as the closing
+    # symmetrisation sees it: c[i,j,a,b] + c[j,i,b,a]
+    l2c = eris.ovvo.permute(0, 3, 1, 2) * 0.5
+    imds.l2_sym = (l2c + l2c.permute(1, 0, 3, 2)).contiguous()
+    del l2c
     # integral combinations that two matrix-vector terms of every update read in two index
     # orders: one combined pass instead of two
     ovvv = eris.ovvv
@@ -223,7 +227,8 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     l1new = l1new + fov * inh + C("ib,ba->ia", l1, w1) - C("ja,ij->ia", l1, w2)
     l1new = l1new - C("ik,ka->ia", mij, imds.w4) - C("ca,ic->ia", mba, imds.w4)
     l1new = l1_terms(l1new.contiguous(), l2, imds.w3.T, c1=2.0, c2=-1.0)
-    l2new.addcmul_(l1[:, None, :, None], imds.w4[None, :, None, :])       # + l1[i,a] w4[j,b]
+    # l2new's l1[i,a] w4[j,b] term and its bare integral term join it in the closing
+    # symmetrisation (l2_finish): nothing reads l2new in between
     acc(l2new, "jibc,ca->jiba", l2, w1)
     acc(l2new, "jk,kiba->jiba", w2, l2, -1.0)
 
@@ -245,10 +250,6 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
 
     voov = eris.ovvo.permute(1, 0, 3, 2)
     l1new = l1new + C("jb,aijb->ia", l1, voov) * 2
-    if torch.is_tensor(inh):
-        l2new.addcmul_(imds.l2_const, inh)
-    elif inh:
-        l2new.add_(imds.l2_const)
     acc(l2new, "bjic,ca->jiba", voov, mba1, -1.0)
     acc(l2new, "bjka,ik->jiba", voov, mij1, -1.0)
     l1new = l1new + C("aijb,jb->ia", imds.g2voov_b, mia1)
@@ -272,8 +273,9 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
 
 
     l1new = l1new / imds.eia
-    # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op)
-    l2new = pair_combine(l2new, c1=1.0, c4=1.0, d=imds.inv_d2)
+    # symmetrize l2 and divide by D_ijab (the reference's per-i loop, :325-334, as one op),
+    # with the l1 w4 and bare integral terms (weight inh) added on the way
+    l2new = l2_finish(l2new, l1, imds.w4, imds.l2_sym, inh, imds.inv_d2)
     return l1new, l2new
 
 
@@ -392,7 +394,7 @@ class _GraphEntry:
         self.keys = {k for k in traced if k[0] not in fixed}
 
     # the intermediates a low-precision update reads (update_lambda_gpu)
-    IMDS_READ = ("w1", "w2", "w3", "w4", "woooo", "wvooo", "wvvov", "wvOOv", "theta", "tau", "tmp_voov", "l2_const",
+    IMDS_READ = ("w1", "w2", "w3", "w4", "woooo", "wvooo", "wvvov", "wvOOv", "theta", "tau", "tmp_voov", "l2_sym",
                  "g2ovvv", "g2voov_b", "eia", "inv_d2")
 
     def refresh(self, t1, t2, imds):

@@ -4,6 +4,8 @@
         out[i,a] += sum_{j,b} (c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a]) y[j,b]
     pair_combine(X, c1, c2, c3, c4, d=None, out=None, beta=0)
         out[i,j,a,b] (+)= (c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a]) * d[i,j,a,b]
+    l2_finish(X, u, w, S, s, d)
+        out[i,j,a,b] = (X[i,j,a,b] + X[j,i,b,a] + u[i,a] w[j,b] + u[j,b] w[i,a] + s S[i,j,a,b]) * d[i,j,a,b]
 
 Both read X once (l1_terms forms the four matrix-vector products of every v x v
 block in one pass; pair_combine stages the transposed blocks through LDS), in
@@ -82,4 +84,31 @@ def unpack_pairs(X, iu, ju, o, scale=1.0):
     out = torch.empty((o, o, v, v), dtype=torch.float64, device=X.device)
     _lib.check(_lib.load().ccsd_unpack_pairs(_lib.ptr(X), _lib.ptr(iu), _lib.ptr(ju), o, v, npair, float(scale),
                                              _lib.ptr(out), _lib.current_stream_ptr()), "ccsd_unpack_pairs")
+    return out
+
+
+def l2_finish(X, u, w, S, s, d):
+    """The closing step of the lambda update (ccsd_gpu/gpu_ccsd_lambda.py:327-334 with
+    :259 and :295 folded in), as a new contiguous tensor:
+        out[i,j,a,b] = (X[i,j,a,b] + X[j,i,b,a] + u[i,a] w[j,b] + u[j,b] w[i,a] + s S[i,j,a,b]) d[i,j,a,b]
+    X, S (or None), d contiguous float64 (o, o, v, v); u, w (o, v); s a float or a 0-d
+    float64 device tensor (read when the kernel runs)."""
+    o, v = _check_x(X)
+    for name, t in (("d", d), ("S", S)):
+        if t is not None and (tuple(t.shape) != tuple(X.shape) or not t.is_contiguous() or t.dtype != torch.float64):
+            raise ValueError(f"{name} must be contiguous and shaped like X")
+    u = u.contiguous()
+    w = w.contiguous()
+    if tuple(u.shape) != (o, v) or tuple(w.shape) != (o, v) or u.dtype != torch.float64 or w.dtype != torch.float64:
+        raise ValueError("u and w must be float64 (o, v) tensors")
+    if torch.is_tensor(s):
+        if s.numel() != 1 or s.dtype != torch.float64 or not s.is_cuda:
+            raise ValueError("a tensor weight must be a 0-d float64 device tensor")
+        sdev, shost = _lib.ptr(s), 0.0
+    else:
+        sdev, shost = None, float(s)
+    out = torch.empty_like(X)
+    _lib.check(_lib.load().ccsd_l2_finish(_lib.ptr(X), o, v, _lib.ptr(u), _lib.ptr(w),
+                                          _lib.ptr(S) if S is not None else None, sdev, shost, _lib.ptr(d),
+                                          _lib.ptr(out), _lib.current_stream_ptr()), "ccsd_l2_finish")
     return out

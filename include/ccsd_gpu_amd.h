@@ -262,6 +262,17 @@ int ccsd_vvvv_pm_unpack(const double* x, int o, int v, long long ldx, double sca
 int ccsd_pair_combine(const double* x, const long long* xstrides, int o, int v, double c1, double c2, double c3,
                       double c4, const double* d, double* out, const long long* ostrides, int beta, void* stream);
 
+/* Closing step of the lambda update:
+ *   out[i,j,a,b] = (x[i,j,a,b] + x[j,i,b,a] + u[i,a] w[j,b] + u[j,b] w[i,a] + sc s[i,j,a,b]) d[i,j,a,b]
+ * with x, s, d, out contiguous (o, o, v, v) (out not overlapping x) and u, w contiguous
+ * (o, v); s may be NULL; sc = *sdev when sdev is given (a device scalar read when the
+ * kernel runs), else shost. The symmetrisation and division by D_ijab of
+ * ccsd_gpu/gpu_ccsd_lambda.py:327-334 with the l1 x w4 term (:259) and the bare
+ * integral term (:295, s = its (i,a) <-> (j,b) symmetrised form) folded in, so that
+ * neither needs a pass of its own over l2new. */
+int ccsd_l2_finish(const double* x, int o, int v, const double* u, const double* w, const double* s,
+                   const double* sdev, double shost, const double* d, double* out, void* stream);
+
 /* Copy a strided view into a contiguous destination: ndim <= 6 dimensions in the
  * destination's order (sizes), with the source's element stride of each
  * (strides; any permutation). Writes float64 to dst, or, when dst is NULL,

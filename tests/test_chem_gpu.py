@@ -142,13 +142,15 @@ def test_bf16x3_nt_scatter_destination_variants(gpu, gemm_variant, variant):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision,tol", [("fp64", 1e-13), ("fp32", 1e-5), ("bf16x3", 1e-4)])
-def test_vvvv_term_pair_halves_match_full(gpu, precision, tol):
+@pytest.mark.parametrize("precision,tol,o,v", [("fp64", 1e-13, 6, 11), ("fp32", 1e-5, 6, 11),
+                                               ("bf16x3", 1e-4, 6, 11), ("fp64", 1e-13, 5, 45),
+                                               ("fp64", 1e-13, 3, 70), ("bf16x3", 1e-4, 4, 64)])
+def test_vvvv_term_pair_halves_match_full(gpu, precision, tol, o, v):
     """The l2.vvvv term through the symmetric / antisymmetric pair products against
-    the full einsum, twice (the second call reads the cached pair matrices)."""
+    the full einsum, twice (the second call reads the cached pair matrices); v past
+    one and two 32-wide unpack tiles, ragged and exact."""
     from ccsd_gpu_amd.precision import vvvv_term
-    gen = torch.Generator(device="cuda").manual_seed(3)
-    o, v = 6, 11
+    gen = torch.Generator(device="cuda").manual_seed(3 + o * v)
     x = torch.randn((v, v, v, v), dtype=torch.float64, device="cuda", generator=gen)
     vvvv = x + x.permute(1, 0, 2, 3)
     vvvv = vvvv + vvvv.permute(0, 1, 3, 2)

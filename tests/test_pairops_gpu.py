@@ -1,4 +1,4 @@
-"""Doubles-shaped lambda helpers (l1_terms, pair_combine) against torch FP64 einsums."""
+"""Doubles-shaped lambda helpers (l1_terms, pair_combine, l2_finish) against torch FP64 einsums."""
 import pytest
 import torch
 
@@ -82,3 +82,34 @@ def test_unpack_pairs_matches_scatter(dev, o, v):
     ref[iu[diag], iu[diag]] = 0.25 * (X[diag] + X[diag].transpose(1, 2))
     got = unpack_pairs(X, iu, ju, o, scale=0.5)
     assert torch.allclose(got, ref, rtol=1e-15, atol=1e-15)
+
+
+@pytest.mark.parametrize("o,v", [(4, 45), (3, 64), (2, 7)])
+@pytest.mark.parametrize("weight", ["host", "device", "none"])
+def test_l2_finish_matches_sum(dev, o, v, weight):
+    """(X + X[j,i,b,a] + u[i,a] w[j,b] + u[j,b] w[i,a] + s S) d, with the weight s given on
+    the host, as a device scalar (read when the kernel runs: a later value reaches the
+    next call), or with no S at all."""
+    from ccsd_gpu_amd.pairops import l2_finish
+    g = torch.Generator(device=dev).manual_seed(o * 100 + v)
+    R = lambda *s: torch.randn(*s, dtype=torch.float64, device=dev, generator=g)  # noqa: E731
+    X, S, d = R(o, o, v, v), R(o, o, v, v), R(o, o, v, v)
+    u, w = R(o, v), R(v, o).T          # w a strided view
+    s = 0.75
+    uw = u[:, None, :, None] * w[None, :, None, :]
+    ref = X + X.permute(1, 0, 3, 2) + uw + uw.permute(1, 0, 3, 2)
+    if weight != "none":
+        ref = ref + s * S
+    ref = ref * d
+    if weight == "host":
+        got = l2_finish(X, u, w, S, s, d)
+    elif weight == "device":
+        sd = torch.full((), -3.0, dtype=torch.float64, device=dev)
+        got = l2_finish(X, u, w, S, sd, d)
+        sd.fill_(s)                      # queued after the kernel: it read -3 ...
+        ref_old = (X + X.permute(1, 0, 3, 2) + uw + uw.permute(1, 0, 3, 2) - 3.0 * S) * d
+        assert torch.allclose(got, ref_old, rtol=1e-14, atol=1e-13)
+        got = l2_finish(X, u, w, S, sd, d)   # ... and the next call reads 0.75
+    else:
+        got = l2_finish(X, u, w, None, 0.0, d)
+    assert torch.allclose(got, ref, rtol=1e-14, atol=1e-13)
