@@ -24,13 +24,13 @@ int ccsd_reg_persist_parts(int li, int lj, int lk, int ll, int max_ncij, int max
 size_t ccsd_reg_persist_lds(int max_nij);
 int ccsd_reg_persist_blocks_per_cu(size_t lds);
 hipError_t ccsd_launch_reg_persistent(EriClassArgs* classes, int nclasses, const int4* items, int nitems,
-                                      unsigned int* counter, const double* gamma, const double* dm, const double* pm,
-                                      unsigned long long* defx, double scale, int gamma_mode, int nblocks, size_t lds,
-                                      hipStream_t stream);
-hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
-                                      unsigned int* counter, const double* gamma, const double* dm,
+                                      unsigned int* counter, unsigned int* ticks, const double* gamma, const double* dm,
                                       const double* pm, unsigned long long* defx, double scale, int gamma_mode,
                                       int nblocks, size_t lds, hipStream_t stream);
+hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
+                                      unsigned int* counter, unsigned int* ticks, const double* gamma,
+                                      const double* dm, const double* pm, unsigned long long* defx, double scale,
+                                      int gamma_mode, int nblocks, size_t lds, hipStream_t stream);
 hipError_t ccsd_launch_defx_to_de(const unsigned long long* defx, int natm, double* de, hipStream_t stream);
 hipError_t ccsd_launch_c2s(double* out, const double* in, const void* ti, const void* tall, int nfi_s, int nao_s,
                            long long npair_s, int nao_c, long long npair_c, int nfi_c, int ncomp, hipStream_t stream);
@@ -195,9 +195,27 @@ struct PersistPart {
     EriClassArgs* d_cls = nullptr;
     int4* d_items = nullptr;
     unsigned int* d_counter = nullptr;   // 8 bytes, cleared by a fill kernel before each launch
+    unsigned int* d_ticks = nullptr;     // per-item durations of a calibration launch (register part)
     int ncls = 0, nitems = 0, nblocks = 0;
     size_t lds = 0;
+    // what the item cut reads: each class's work list, units per chunk and weight (time
+    // per cost unit), the order mode; `calibrated` once the weights come from a launch
+    std::vector<const ChunkList*> lists;
+    std::vector<int> parts;
+    std::vector<double> wt;
+    std::vector<int4> items;
+    double unit_cost = 0.0;   // fixed time per unit (calibrated), in the weights' units
+    double total = 0.0;       // estimated wave-time of all items, in the weights' units
+    bool grouped = false;
+    int launches = 0;
+    bool calibrated = false;
 };
+
+// Longest run of units in one work item: every unit carries a fixed cost (its bra table,
+// dependent loads and gradient atomics) that the cost model does not see, so runs of the
+// cheapest chunks at the end of a class list must not grow without bound (a cut by cost
+// alone put 40 000 of them into single items that ran for over a second).
+constexpr int kMaxItemUnits = 256;
 
 // The whole fused pass of a bra set as persistent launches: `reg` runs the
 // register-resident bodies (plan option 5 = 1) of every class that has one, `gen` the
@@ -213,6 +231,7 @@ void free_persist_part(PersistPart& q)
     if (q.d_cls) (void)hipFree(q.d_cls);
     if (q.d_items) (void)hipFree(q.d_items);
     if (q.d_counter) (void)hipFree(q.d_counter);
+    if (q.d_ticks) (void)hipFree(q.d_ticks);
     q = PersistPart();
 }
 
@@ -911,6 +930,90 @@ int ensure_side_streams(ccsd_plan* p)
 // an item of its own). Order: class by class (`grouped`: the costliest class first,
 // decreasing cost inside a class), so that the waves resident on a CU mostly run one
 // body and share its instructions in cache, or by decreasing cost only.
+// Cut the classes' units into work items from q.lists / q.parts / q.wt and upload them
+// (replacing any earlier cut). The costs in time units are each class's cost model
+// (primitive products x quartets) times its weight, so that items of different classes
+// take about equally long (unweighted, a run of d-shell units cut to the same cost as an
+// s-shell run took tens of times longer, and a few such items held the launch for most
+// of its time).
+int cut_persist_items(PersistPart& q)
+{
+    const size_t nc = q.lists.size();
+    double total = 0.0;
+    std::vector<double> cls_cost(nc, 0.0);
+    for (size_t ci = 0; ci < nc; ci++) {
+        for (float w : q.lists[ci]->cost) cls_cost[ci] += w;
+        cls_cost[ci] = cls_cost[ci] * q.wt[ci] + q.unit_cost * (double)q.lists[ci]->n * q.parts[ci];
+        total += cls_cost[ci];
+    }
+    q.total = total;
+    std::vector<int4>& h = q.items;
+    h.clear();
+    if (q.grouped) {
+        // classes costliest first, inside a class its units in list order (costliest chunk
+        // first); items cut by guided self-scheduling: an item takes 1/(4 nblocks) of the
+        // work not yet assigned, so items shrink towards the end of the list and the waves
+        // drawing the last ones finish close together even where a class's weight is off
+        std::vector<int> order(nc);
+        for (size_t ci = 0; ci < nc; ci++) order[ci] = (int)ci;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cls_cost[x] > cls_cost[y]; });
+        const double share = 1.0 / (4.0 * q.nblocks);
+        double remaining = total;
+        for (int ci : order) {
+            const ChunkList* cl = q.lists[ci];
+            const int np = q.parts[ci];
+            const int nu = cl->n * np;
+            int u0 = 0;
+            double acc = 0.0;
+            for (int u = 0; u < nu; u++) {
+                acc += cl->cost[u / np] / np * q.wt[ci] + q.unit_cost;
+                if (acc >= remaining * share || u + 1 == nu || u + 1 - u0 >= kMaxItemUnits) {
+                    h.push_back(make_int4(ci, u0, u + 1, 0));
+                    remaining -= acc;
+                    u0 = u + 1;
+                    acc = 0.0;
+                }
+            }
+        }
+    } else {
+        // by cost only: items of 1/(32 nblocks) of the whole, costliest first
+        const double target = total / ((double)q.nblocks * 32.0);
+        std::vector<std::pair<double, int4>> items;
+        for (size_t ci = 0; ci < nc; ci++) {
+            const ChunkList* cl = q.lists[ci];
+            const int np = q.parts[ci];
+            const int nu = cl->n * np;
+            int u0 = 0;
+            double acc = 0.0;
+            for (int u = 0; u < nu; u++) {
+                acc += cl->cost[u / np] / np * q.wt[ci] + q.unit_cost;
+                if (acc >= target || u + 1 == nu || u + 1 - u0 >= kMaxItemUnits) {
+                    items.push_back(std::make_pair(acc, make_int4((int)ci, u0, u + 1, 0)));
+                    u0 = u + 1;
+                    acc = 0.0;
+                }
+            }
+        }
+        std::stable_sort(items.begin(), items.end(),
+                         [](const std::pair<double, int4>& x, const std::pair<double, int4>& y) {
+                             return x.first > y.first;
+                         });
+        h.resize(items.size());
+        for (size_t t = 0; t < items.size(); t++) h[t] = items[t].second;
+    }
+    if (h.size() > 2147483647ULL) return fail(CCSD_ERR_UNSUPPORTED, "too many work items");
+    if (q.d_items) (void)hipFree(q.d_items);
+    if (q.d_ticks) (void)hipFree(q.d_ticks);
+    q.d_items = nullptr;
+    q.d_ticks = nullptr;
+    q.nitems = (int)h.size();
+    HIP_TRY(hipMalloc(&q.d_items, sizeof(int4) * std::max<size_t>(1, h.size())));
+    HIP_TRY(hipMemcpy(q.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
+    return CCSD_OK;
+}
+
+// One persistent launch's class table, grid and work items. wt: each class's time per
+// cost unit (the autotuner's measurement, or a model where a class was not timed).
 int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const std::vector<const ChunkList*>& lists,
                        const std::vector<int>& parts, const std::vector<double>& wt, bool reg, bool grouped,
                        PersistPart* out)
@@ -926,55 +1029,72 @@ int build_persist_part(ccsd_plan* p, const std::vector<EriClassArgs>& cls, const
     const int bpc = reg ? ccsd_reg_persist_blocks_per_cu(q.lds) : ccsd_persistent_blocks_per_cu(q.lds);
     if (bpc <= 0) return fail(CCSD_ERR_UNSUPPORTED, "persistent kernel cannot be resident at this LDS size");
     q.nblocks = bpc * p->n_cu;
-    // the work lists' costs in time units: each class's cost model (primitive products x
-    // quartets) times its weight, so that items of different classes take about equally
-    // long (unweighted, a run of d-shell units cut to the same cost as an s-shell run took
-    // tens of times longer, and a few such items held the launch for most of its time)
-    double total = 0.0;
-    std::vector<double> cls_cost(cls.size(), 0.0);
-    for (size_t ci = 0; ci < cls.size(); ci++) {
-        for (float w : lists[ci]->cost) cls_cost[ci] += w;
-        cls_cost[ci] *= wt[ci];
-    }
-    for (double w : cls_cost) total += w;
-    const double target = total / ((double)q.nblocks * 8.0);
-    // (sort key, item); the key orders classes first when items stay class by class
-    std::vector<std::pair<std::pair<double, double>, int4>> items;
-    for (size_t ci = 0; ci < cls.size(); ci++) {
-        const ChunkList* cl = lists[ci];
-        const int np = parts[ci];
-        if ((long long)cl->n * np > 2147483647LL) return fail(CCSD_ERR_UNSUPPORTED, "work list too long");
-        const int nu = cl->n * np;
-        int u0 = 0;
-        double acc = 0.0;
-        for (int u = 0; u < nu; u++) {
-            acc += cl->cost[u / np] / np * wt[ci];
-            if (acc >= target || u + 1 == nu) {
-                items.push_back(std::make_pair(std::make_pair(grouped ? cls_cost[ci] : 0.0, acc),
-                                               make_int4((int)ci, u0, u + 1, 0)));
-                u0 = u + 1;
-                acc = 0.0;
-            }
-        }
-    }
-    std::stable_sort(items.begin(), items.end(),
-                     [](const std::pair<std::pair<double, double>, int4>& x,
-                        const std::pair<std::pair<double, double>, int4>& y) {
-                         if (x.first.first != y.first.first) return x.first.first > y.first.first;
-                         if (x.first.first != 0.0 && x.second.x != y.second.x)
-                             return x.second.x < y.second.x;   // equal-cost classes stay apart
-                         return x.first.second > y.first.second;
-                     });
-    std::vector<int4> h(items.size());
-    for (size_t t = 0; t < items.size(); t++) h[t] = items[t].second;
+    for (size_t ci = 0; ci < cls.size(); ci++)
+        if ((long long)lists[ci]->n * parts[ci] > 2147483647LL) return fail(CCSD_ERR_UNSUPPORTED, "work list too long");
+    q.lists = lists;
+    q.parts = parts;
+    q.wt = wt;
+    q.grouped = grouped;
+    int rc = cut_persist_items(q);
+    if (rc) return rc;
     q.ncls = (int)cls.size();
-    q.nitems = (int)h.size();
-    q.nblocks = std::min(q.nblocks, q.nitems);
     HIP_TRY(hipMalloc(&q.d_cls, sizeof(EriClassArgs) * cls.size()));
     HIP_TRY(hipMemcpy(q.d_cls, cls.data(), sizeof(EriClassArgs) * cls.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMalloc(&q.d_items, sizeof(int4) * h.size()));
-    HIP_TRY(hipMemcpy(q.d_items, h.data(), sizeof(int4) * h.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&q.d_counter, sizeof(double)));
+    return CCSD_OK;
+}
+
+// Re-weight the register launch's classes from the item durations its last launch
+// recorded, as this kernel runs them (its residency and part counts): item time =
+// a_c x (cost of its units) + b x (number of units), a_c per class and b, the fixed
+// time per unit, shared, fitted by alternating least squares; then cut the items
+// again. Classes without a recorded item keep their weight scaled by the median ratio
+// of the fitted ones.
+int calibrate_persist_part(PersistPart& q, hipStream_t stream)
+{
+    std::vector<unsigned int> ticks(q.nitems);
+    HIP_TRY(hipStreamSynchronize(stream));
+    HIP_TRY(hipMemcpy(ticks.data(), q.d_ticks, sizeof(unsigned int) * ticks.size(), hipMemcpyDeviceToHost));
+    const size_t nc = q.lists.size();
+    const int n = q.nitems;
+    std::vector<double> C(n), N(n), T(n);
+    for (int t = 0; t < n; t++) {
+        const int4 it = q.items[t];
+        const ChunkList* cl = q.lists[it.x];
+        const int np = q.parts[it.x];
+        double c = 0.0;
+        for (int u = it.y; u < it.z; u++) c += cl->cost[u / np] / np;
+        C[t] = c;
+        N[t] = it.z - it.y;
+        T[t] = ticks[t];
+    }
+    std::vector<double> a(nc, 0.0);
+    double b = 0.0;
+    for (int iter = 0; iter < 20; iter++) {
+        std::vector<double> num(nc, 0.0), den(nc, 0.0);
+        for (int t = 0; t < n; t++) {
+            num[q.items[t].x] += (T[t] - b * N[t]) * C[t];
+            den[q.items[t].x] += C[t] * C[t];
+        }
+        for (size_t ci = 0; ci < nc; ci++) a[ci] = den[ci] > 0.0 ? std::max(0.0, num[ci] / den[ci]) : 0.0;
+        double nb = 0.0, db = 0.0;
+        for (int t = 0; t < n; t++) {
+            nb += (T[t] - a[q.items[t].x] * C[t]) * N[t];
+            db += N[t] * N[t];
+        }
+        b = db > 0.0 ? std::max(0.0, nb / db) : 0.0;
+    }
+    std::vector<double> ratio;
+    for (size_t ci = 0; ci < nc; ci++)
+        if (a[ci] > 0.0 && q.wt[ci] > 0.0) ratio.push_back(a[ci] / q.wt[ci]);
+    if (ratio.empty()) return CCSD_OK;
+    std::nth_element(ratio.begin(), ratio.begin() + ratio.size() / 2, ratio.end());
+    const double med = ratio[ratio.size() / 2];
+    for (size_t ci = 0; ci < nc; ci++) q.wt[ci] = a[ci] > 0.0 ? a[ci] : q.wt[ci] * med;
+    q.unit_cost = b;
+    int rc = cut_persist_items(q);
+    if (rc) return rc;
+    q.calibrated = true;
     return CCSD_OK;
 }
 
@@ -1096,7 +1216,25 @@ int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* d
         }
         it = bs.persist.emplace(screened ? 1 : 0, pp).first;
     }
-    const PersistPlan& pp = it->second;
+    PersistPlan& pp = it->second;
+    // the second call of a launch re-cuts its items from the first one's recorded item
+    // durations (not while a graph is captured: that needs a host read)
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(stream, &cap));
+    const bool capturing = cap != hipStreamCaptureStatusNone;
+    bool record[2] = {false, false};
+    PersistPart* parts_[2] = {&pp.reg, &pp.gen};
+    for (int k = 0; k < 2; k++) {
+        PersistPart& q = *parts_[k];
+        if (q.nitems <= 0 || capturing) continue;
+        if (q.launches > 0 && !q.calibrated && q.d_ticks) {
+            HIP_TRY(hipDeviceSynchronize());   // the generic launch ran on a side stream
+            int rc = calibrate_persist_part(q, stream);
+            if (rc) return rc;
+        }
+        record[k] = !q.calibrated;
+        if (record[k] && !q.d_ticks) HIP_TRY(hipMalloc(&q.d_ticks, sizeof(unsigned int) * (size_t)q.nitems));
+    }
     p->last_quartets = pp.quartets;
     p->last_launches = pp.launches;
     int rc = defx_begin(p, stream);
@@ -1113,13 +1251,28 @@ int run_fused_persistent(ccsd_plan* p, BraSet& bs, int ish0, int ish1, double* d
         HIP_TRY(hipEventRecord(p->fork_ev, stream));
         HIP_TRY(hipStreamWaitEvent(gst, p->fork_ev, 0));
     }
-    if (pp.gen.nitems > 0)
-        HIP_TRY(ccsd_launch_eri_persistent(pp.gen.d_cls, pp.gen.d_items, pp.gen.nitems, pp.gen.d_counter, gamma, dm,
-                                           pm, p->d_defx, scale, gamma_mode, pp.gen.nblocks, pp.gen.lds, gst));
-    if (pp.reg.nitems > 0)
+    // The generic launch goes first and its waves stay resident until its list is
+    // exhausted: given every slot, it would hold the register launch back until then.
+    // It gets the share of the wave slots its estimated work is of the whole (both
+    // estimates in the same units: the autotuner's, later the recorded durations), so
+    // that the two launches end together.
+    int gen_blocks = pp.gen.nblocks;
+    if (both && pp.gen.total > 0.0 && pp.reg.total > 0.0) {
+        const double f = pp.gen.total / (pp.gen.total + pp.reg.total);
+        gen_blocks = (int)std::min<double>(pp.gen.nblocks, std::max(1.0, std::ceil(f * pp.reg.nblocks)));
+    }
+    if (pp.gen.nitems > 0) {
+        HIP_TRY(ccsd_launch_eri_persistent(pp.gen.d_cls, pp.gen.d_items, pp.gen.nitems, pp.gen.d_counter,
+                                           record[1] ? pp.gen.d_ticks : nullptr, gamma, dm, pm, p->d_defx, scale,
+                                           gamma_mode, gen_blocks, pp.gen.lds, gst));
+        if (record[1]) pp.gen.launches++;
+    }
+    if (pp.reg.nitems > 0) {
         HIP_TRY(ccsd_launch_reg_persistent(pp.reg.d_cls, pp.reg.ncls, pp.reg.d_items, pp.reg.nitems,
-                                           pp.reg.d_counter, gamma, dm, pm, p->d_defx, scale, gamma_mode,
-                                           pp.reg.nblocks, pp.reg.lds, stream));
+                                           pp.reg.d_counter, record[0] ? pp.reg.d_ticks : nullptr, gamma, dm, pm,
+                                           p->d_defx, scale, gamma_mode, pp.reg.nblocks, pp.reg.lds, stream));
+        if (record[0]) pp.reg.launches++;
+    }
     if (both) {
         HIP_TRY(hipEventRecord(p->join_ev[0], gst));
         HIP_TRY(hipStreamWaitEvent(stream, p->join_ev[0], 0));

@@ -376,13 +376,16 @@ struct PersistDyn {
 // chunks [items[t].y, items[t].z) of class items[t].x (class table `classes`: the
 // launch description of each class, work list included). Items are ordered by
 // decreasing estimated cost, so the last waves to finish run short items. `counter`
-// must read 0 at launch; every wave leaves the loop once it draws t >= nitems.
+// must read 0 at launch; every wave leaves the loop once it draws t >= nitems. ticks
+// (optional, nitems words): each item's duration in wall_clock64 ticks.
 __global__ void __launch_bounds__(64) eri_persistent_kernel(const EriClassArgs* __restrict__ classes,
                                                             const int4* __restrict__ items, int nitems,
-                                                            unsigned int* __restrict__ counter, const PersistDyn dyn)
+                                                            unsigned int* __restrict__ counter,
+                                                            unsigned int* __restrict__ ticks, const PersistDyn dyn)
 {
     extern __shared__ double lds[];
     for (;;) {
+        const unsigned long long t_start = wall_clock64();
         int t = 0;
         if (threadIdx.x == 0) t = (int)atomicAdd(counter, 1u);
         t = __builtin_amdgcn_readfirstlane(__shfl(t, 0, 64));
@@ -406,6 +409,10 @@ __global__ void __launch_bounds__(64) eri_persistent_kernel(const EriClassArgs* 
                 quartet(a, L, lds, ch.x, kq, gdx, gdy, gdz);
             }
             add_chunk_gradient(a, ch.x, gdx, gdy, gdz);
+        }
+        if (ticks && threadIdx.x == 0) {
+            const unsigned long long dt = wall_clock64() - t_start;
+            ticks[t] = dt > 0xffffffffULL ? 0xffffffffu : (unsigned int)dt;
         }
         __syncthreads();
     }
@@ -508,9 +515,9 @@ hipError_t ccsd_launch_defx_to_de(const unsigned long long* defx, int natm, doub
 }
 
 hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* items, int nitems,
-                                      unsigned int* counter, const double* gamma, const double* dm,
-                                      const double* pm, unsigned long long* defx, double scale, int gamma_mode,
-                                      int nblocks, size_t lds, hipStream_t stream)
+                                      unsigned int* counter, unsigned int* ticks, const double* gamma,
+                                      const double* dm, const double* pm, unsigned long long* defx, double scale,
+                                      int gamma_mode, int nblocks, size_t lds, hipStream_t stream)
 {
     if (nitems <= 0 || nblocks <= 0) return hipSuccess;
     ccsd::PersistDyn dyn;
@@ -521,7 +528,7 @@ hipError_t ccsd_launch_eri_persistent(const EriClassArgs* classes, const int4* i
     dyn.scale = scale;
     dyn.gamma_mode = gamma_mode;
     hipLaunchKernelGGL(ccsd::eri_persistent_kernel, dim3((unsigned)nblocks), dim3(64), lds, stream, classes, items,
-                       nitems, counter, dyn);
+                       nitems, counter, ticks, dyn);
     return hipGetLastError();
 }
 

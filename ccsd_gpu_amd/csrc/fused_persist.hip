@@ -86,7 +86,8 @@ __device__ __forceinline__ const T* uniform_ptr(const T* p)
 template <int CODE>
 __device__ __attribute__((noinline)) int persist_class(const EriClassArgs* __restrict__ classes,
                                                        const int4* __restrict__ items, int nitems,
-                                                       unsigned int* __restrict__ counter, int t, int cls)
+                                                       unsigned int* __restrict__ counter,
+                                                       unsigned int* __restrict__ ticks, int t, int cls)
 {
     using R = RegCode<CODE>;
     if constexpr (!R::valid) {
@@ -106,6 +107,7 @@ __device__ __attribute__((noinline)) int persist_class(const EriClassArgs* __res
         const double norm = 34.98683665524972497 * shell_norm_dev(LI) * shell_norm_dev(LJ) * shell_norm_dev(LK) *
                             shell_norm_dev(LL);
         for (;;) {
+            const unsigned long long t_start = wall_clock64();
             const int4 it = gitems[t];
             const int u0 = __builtin_amdgcn_readfirstlane(it.y), u1 = __builtin_amdgcn_readfirstlane(it.z);
             for (int u = u0; u < u1; u++) {
@@ -130,6 +132,11 @@ __device__ __attribute__((noinline)) int persist_class(const EriClassArgs* __res
                 }
                 if (lane == 0) defx_add(a.defx, si.atom, a.gamma_scale * sx, a.gamma_scale * sy, a.gamma_scale * sz);
             }
+            if (ticks && lane == 0) {
+                // the item's duration in constant-rate clock ticks (host calibration)
+                const unsigned long long dt = wall_clock64() - t_start;
+                gst(ticks)[t] = dt > 0xffffffffULL ? 0xffffffffu : (unsigned int)dt;
+            }
             t = draw(counter);
             if (t >= nitems) return t;
             if (__builtin_amdgcn_readfirstlane(gitems[t].x) != cls) return t;
@@ -139,10 +146,11 @@ __device__ __attribute__((noinline)) int persist_class(const EriClassArgs* __res
 
 template <int... C>
 __device__ __forceinline__ int dispatch(int code, const EriClassArgs* classes, const int4* items, int nitems,
-                                        unsigned int* counter, int t, int cls, std::integer_sequence<int, C...>)
+                                        unsigned int* counter, unsigned int* ticks, int t, int cls,
+                                        std::integer_sequence<int, C...>)
 {
     int next = nitems;
-    ((code == C ? (void)(next = persist_class<C>(classes, items, nitems, counter, t, cls)) : void()), ...);
+    ((code == C ? (void)(next = persist_class<C>(classes, items, nitems, counter, ticks, t, cls)) : void()), ...);
     return next;
 }
 
@@ -161,12 +169,13 @@ __global__ void persist_patch_kernel(EriClassArgs* __restrict__ classes, int n, 
 
 // items[t] = (class, first unit, end unit, unused); `counter` must read 0 at launch and
 // every wave leaves the loop once it draws t >= nitems; the class table carries this
-// call's Gamma and gradient fields (persist_patch_kernel). Dynamic LDS: the bra table
-// of the largest class (max_nij x kBraFields doubles). Two waves per SIMD: every body
-// fits that register budget on its own.
+// call's Gamma and gradient fields (persist_patch_kernel). ticks (optional, nitems
+// words): each item's duration in wall_clock64 ticks. Dynamic LDS: the bra table of the
+// largest class (max_nij x kBraFields doubles). Two waves per SIMD: every body fits that
+// register budget on its own.
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
     fused_reg_persistent_kernel(const EriClassArgs* __restrict__ classes, const int4* __restrict__ items, int nitems,
-                                unsigned int* __restrict__ counter)
+                                unsigned int* __restrict__ counter, unsigned int* __restrict__ ticks)
 {
     int t = draw(counter);
     while (t < nitems) {
@@ -174,7 +183,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2)))
         const EriClassArgs* ap = classes + cls;
         const int gc = (ap->max_ncij > 1 || ap->max_nckl > 1) ? 1 : 0;
         const int code = __builtin_amdgcn_readfirstlane(reg_code(ap->li, ap->lj, ap->lk, ap->ll, gc));
-        t = dispatch(code, classes, items, nitems, counter, t, cls, std::make_integer_sequence<int, kRegCodes>{});
+        t = dispatch(code, classes, items, nitems, counter, ticks, t, cls,
+                     std::make_integer_sequence<int, kRegCodes>{});
     }
 }
 
@@ -203,9 +213,9 @@ int ccsd_reg_persist_blocks_per_cu(size_t lds)
 }
 
 hipError_t ccsd_launch_reg_persistent(EriClassArgs* classes, int nclasses, const int4* items, int nitems,
-                                      unsigned int* counter, const double* gamma, const double* dm, const double* pm,
-                                      unsigned long long* defx, double scale, int gamma_mode, int nblocks, size_t lds,
-                                      hipStream_t stream)
+                                      unsigned int* counter, unsigned int* ticks, const double* gamma, const double* dm,
+                                      const double* pm, unsigned long long* defx, double scale, int gamma_mode,
+                                      int nblocks, size_t lds, hipStream_t stream)
 {
     if (nitems <= 0) return hipSuccess;
     if (nblocks <= 0 || nclasses <= 0 || lds > 65536) return hipErrorInvalidValue;
@@ -220,6 +230,6 @@ hipError_t ccsd_launch_reg_persistent(EriClassArgs* classes, int nclasses, const
     hipLaunchKernelGGL(ccsd_reg::persist_patch_kernel, dim3((unsigned)((nclasses + 63) / 64)), dim3(64), 0, stream,
                        classes, nclasses, dyn);
     hipLaunchKernelGGL(ccsd_reg::fused_reg_persistent_kernel, dim3((unsigned)nblocks), dim3(64), lds, stream, classes,
-                       items, nitems, counter);
+                       items, nitems, counter, ticks);
     return hipGetLastError();
 }
