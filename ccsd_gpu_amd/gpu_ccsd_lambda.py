@@ -212,7 +212,8 @@ def _update_lambda(mycc, t1, t2, l1, l2, eris, imds, use_cpu_vvvv, precision, ho
     mij = C("ikcd,jkcd->ij", l2, theta)
     mba1 = C("jc,jb->bc", l1, t1) + mba
     mij1 = C("kb,jb->kj", l1, t1) + mij
-    mia1 = t1 * inh + C("kc,jkbc->jb", l1, t2) * 2 - C("kc,jkcb->jb", l1, t2)
+    # 2 t2[j,k,b,c] - t2[j,k,c,b] is theta: one pass over it instead of two over t2
+    mia1 = t1 * inh + C("kc,jkbc->jb", l1, theta)
     mia1 = mia1 - t1 @ l1.T @ t1 - C("bd,jd->jb", mba, t1) - C("lj,lb->jb", mij, t1)
 
     # l2new starts as half the l2 (ac|bd) term; its singles part is 2 X_ijab - X_jiab
