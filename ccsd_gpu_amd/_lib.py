@@ -68,6 +68,12 @@ _SIGNATURES = {
     "ccsd_vvvv_pm_unpack": (c_int, [c_void_p, c_int, c_int, ctypes.c_longlong, ctypes.c_double, c_void_p, c_void_p]),
     "ccsd_l2_finish": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_void_p,
                                c_void_p, c_void_p]),
+    "ccsd_diis_workspace": (ctypes.c_longlong, [ctypes.c_longlong, c_int]),
+    "ccsd_diis_push": (c_int, [c_void_p, c_void_p, ctypes.c_longlong, ctypes.c_longlong, c_void_p, c_void_p, c_void_p,
+                               c_void_p, c_void_p, ctypes.c_longlong, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                               ctypes.c_longlong, c_void_p]),
+    "ccsd_diis_combine": (c_int, [c_void_p, ctypes.c_longlong, c_int, c_void_p, c_void_p, c_void_p, ctypes.c_longlong,
+                                  c_void_p]),
     "ccsd_gemm_bf16x3_nt_scatter": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
                                             ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
                                             c_void_p, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),

@@ -8,6 +8,8 @@ ovvv, vvvv, fock, mo_energy); the amplitude equations are the standard
 closed-shell spin-adapted CCSD equations with Fov/Foo/Fvv, Loo/Lvv and
 W-intermediates. Tensors are float64 torch tensors on the HIP device.
 """
+import ctypes
+
 import numpy as np
 import torch
 
@@ -242,16 +244,24 @@ class DIIS:
 
     Vectors and errors live in two (space, N) ring buffers on the device; each
     update adds one row of the error overlap matrix (one GEMV, one host copy of
-    `space` numbers) and extrapolates with one GEMV over the stored vectors."""
+    `space` numbers) and extrapolates with one GEMV over the stored vectors.
 
-    def __init__(self, space=6):
+    store="fp32" (float64 vectors on the device, space <= 15): the ring buffers hold
+    FP32 copies, half the bytes that every update reads twice; one library pass stores
+    the new vector and its error and forms the error's overlaps (summed in FP64), one
+    forms the FP64 extrapolation. For low-precision solves, whose sweeps carry far
+    larger rounding than the FP32 copies add; the vectors returned stay FP64."""
+
+    def __init__(self, space=6, store=None):
         self.space = space
+        self.store = store
         self.X = self.E = None
         self.B = np.zeros((space, space))
         self.count = 0
         self.prev = None
         self.res_norm = None        # |sum_s c_s E[s]| of the last extrapolation
         self.c_abs = None           # sum_s |c_s| of the last extrapolation
+        self._fp32 = None
 
     def _alloc(self, n, like):
         if self.X is None:
@@ -272,6 +282,11 @@ class DIIS:
         history buffer (no separate concatenated copy). Returns (extrapolated vector,
         ||x - previous vector||): the previous vector is the last one returned, or
         `start` on the first call (None: no norm)."""
+        if self._fp32 is None:
+            self._fp32 = (self.store == "fp32" and parts[0].is_cuda and parts[0].dtype == torch.float64
+                          and len(parts) <= 2 and self.space <= 15)
+        if self._fp32:
+            return self._push32(parts, rhs, start)
         n = sum(p.numel() for p in parts)
         self._alloc(n, parts[0])
         first = self.prev is None
@@ -289,12 +304,51 @@ class DIIS:
             return dst, (None if start is None else float(torch.linalg.norm(dst - start)))
         return self._extrapolate()
 
-    def _extrapolate(self):
+    def _push32(self, parts, rhs, start):
+        from . import _lib
+        lib = _lib.load()
+        a = parts[0].reshape(-1).contiguous()
+        b = parts[1].reshape(-1).contiguous() if len(parts) > 1 else None
+        n0 = a.numel()
+        n = n0 + (b.numel() if b is not None else 0)
+        dev = a.device
+        if self.X is None:
+            self.X = torch.empty((self.space + 1, n), dtype=torch.float32, device=dev)
+            self.E = torch.empty((self.space, n), dtype=torch.float32, device=dev)
+            self._ws = torch.empty(max(1, lib.ccsd_diis_workspace(n, self.space)), dtype=torch.float64, device=dev)
+            self._dots = torch.empty(self.space + 1, dtype=torch.float64, device=dev)
+        if rhs is not None:
+            rhs = rhs.contiguous()
+        ptr = lambda t: _lib.ptr(t) if t is not None else None  # noqa: E731
+        stream = _lib.current_stream_ptr()
+        if self.prev is None:
+            # the first vector: its FP64 copy is returned (and is the next error's base)
+            out = torch.empty(n, dtype=torch.float64, device=dev)
+            base = start.contiguous() if start is not None else None
+            _lib.check(lib.ccsd_diis_push(ptr(a), ptr(b), n0, n, ptr(rhs), ptr(base), ptr(self.X[self.space]), None,
+                                          None, 0, 0, 0, ptr(out), ptr(self._dots), ptr(self._ws), self._ws.numel(),
+                                          stream), "ccsd_diis_push")
+            self.prev = out
+            nrm = None if start is None else float(np.sqrt(max(float(self._dots[0]), 0.0)))
+            return out, nrm
         slot = self.count % self.space
         self.count += 1
-        n = min(self.count, self.space)
-        torch.sub(self.X[slot], self.prev, out=self.E[slot])
-        row = (self.E[:n] @ self.E[slot]).cpu().numpy()
+        nr = min(self.count, self.space)
+        _lib.check(lib.ccsd_diis_push(ptr(a), ptr(b), n0, n, ptr(rhs), ptr(self.prev), ptr(self.X[slot]),
+                                      ptr(self.E[slot]), ptr(self.E), n, nr, slot, None, ptr(self._dots),
+                                      ptr(self._ws), self._ws.numel(), stream), "ccsd_diis_push")
+        row = self._dots[:nr].cpu().numpy()
+        cn = self._coefficients(slot, nr, row)
+        out = torch.empty(n, dtype=torch.float64, device=dev)
+        coef = (ctypes.c_double * nr)(*cn)
+        _lib.check(lib.ccsd_diis_combine(ptr(self.X), n, nr, coef, None, ptr(out), n, stream), "ccsd_diis_combine")
+        self.prev = out
+        return out, float(np.sqrt(max(row[slot], 0.0)))
+
+    def _coefficients(self, slot, n, row):
+        """Enter `row` (overlaps of error `slot` with errors 0..n-1) into the overlap
+        matrix and solve for the extrapolation coefficients (numpy); sets res_norm and
+        c_abs."""
         self.B[slot, :n] = row
         self.B[:n, slot] = row
         A = -np.ones((n + 1, n + 1))
@@ -306,6 +360,16 @@ class DIIS:
         # |sum_s c_s E[s]|, from the overlap matrix: see input_combination
         self.res_norm = float(np.sqrt(max(float(cn @ self.B[:n, :n] @ cn), 0.0)))
         self.c_abs = float(np.abs(cn).sum())
+        self._cn = cn
+        return cn
+
+    def _extrapolate(self):
+        slot = self.count % self.space
+        self.count += 1
+        n = min(self.count, self.space)
+        torch.sub(self.X[slot], self.prev, out=self.E[slot])
+        row = (self.E[:n] @ self.E[slot]).cpu().numpy()
+        cn = self._coefficients(slot, n, row)
         c = torch.as_tensor(cn, dtype=self.X.dtype, device=self.X.device)
         self._c = c
         if self.X.is_cuda:
@@ -326,6 +390,14 @@ class DIIS:
         F(x) - x = sum_s c_s E[s] exactly, whose norm is res_norm: a converged iterate
         known without another application of F. Call it before the next push."""
         n = min(self.count, self.space)
+        if self._fp32:
+            from . import _lib
+            out = torch.empty_like(self.prev)
+            coef = (ctypes.c_double * n)(*(-self._cn))
+            _lib.check(_lib.load().ccsd_diis_combine(_lib.ptr(self.E), self.E.shape[1], n, coef, _lib.ptr(self.prev),
+                                                     _lib.ptr(out), out.numel(), _lib.current_stream_ptr()),
+                       "ccsd_diis_combine")
+            return out
         if self.X.is_cuda:
             from . import mfma
             ebar = mfma.gemv(self.E[:n].T, self._c, torch.empty(self.E.shape[1], dtype=self.E.dtype,

@@ -81,6 +81,13 @@ hipError_t ccsd_launch_l1_terms(const double* X, int o, int v, const double* y, 
                                 hipStream_t stream);
 hipError_t ccsd_launch_l2_finish(const double* X, int o, int v, const double* u, const double* w, const double* S,
                                  const double* sp, double sh, const double* D, double* out, hipStream_t stream);
+int ccsd_diis_max_rows();
+long long ccsd_diis_push_blocks(long long n);
+hipError_t ccsd_launch_diis_push(const double* s0, const double* s1, long long n0, long long n, const double* rhs,
+                                 const double* prev, float* xs, float* es, const float* E, long long lde, int nrows,
+                                 int slot, double* prev_out, double* dots, double* ws, hipStream_t stream);
+hipError_t ccsd_launch_diis_combine(const float* X, long long ldx, int nrows, const double* coef, const double* base,
+                                    double* out, long long n, hipStream_t stream);
 hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o, int v, double c1, double c2,
                                     double c3, double c4, const double* D, double* out, const long long* os, int beta,
                                     hipStream_t stream);
@@ -1824,6 +1831,40 @@ int ccsd_l2_finish(const double* x, int o, int v, const double* u, const double*
     if ((long long)o * o * nt * nt > 0x7fffffffLL) return fail(CCSD_ERR_SHAPE, "too many tiles for one launch");
     hipError_t e = ccsd_launch_l2_finish(x, o, v, u, w, s, sdev, shost, d, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "l2_finish_kernel");
+    return CCSD_OK;
+}
+
+long long ccsd_diis_workspace(long long n, int nrows)
+{
+    if (n < 0 || nrows < 0 || nrows > ccsd_diis_max_rows()) return -1;
+    return ccsd_diis_push_blocks(n) * (nrows + 1);
+}
+
+int ccsd_diis_push(const double* s0, const double* s1, long long n0, long long n, const double* rhs, const double* prev,
+                   float* xs, float* es, const float* e_rows, long long lde, int nrows, int slot, double* prev_out,
+                   double* dots, double* ws, long long ws_len, void* stream)
+{
+    if (!s0 || (n > n0 && !s1)) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n0 < 0 || n < n0 || nrows < 0 || nrows > ccsd_diis_max_rows() || (nrows > 0 && (slot < 0 || slot >= nrows)) ||
+        (nrows > 1 && (!e_rows || lde < n)))
+        return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    if (prev && (!dots || !ws || ws_len < ccsd_diis_workspace(n, nrows)))
+        return fail(CCSD_ERR_SHAPE, "dots / workspace missing or too small");
+    if (es && (nrows < 1 || !prev)) return fail(CCSD_ERR_SHAPE, "an error vector needs prev and its row");
+    hipError_t e = ccsd_launch_diis_push(s0, s1, n0, n, rhs, prev, xs, es, e_rows, lde, nrows, slot, prev_out, dots,
+                                         ws, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "diis_push_kernel");
+    return CCSD_OK;
+}
+
+int ccsd_diis_combine(const float* rows, long long ldx, int nrows, const double* coef, const double* base, double* out,
+                      long long n, void* stream)
+{
+    if (!out || (nrows > 0 && (!rows || !coef))) return fail(CCSD_ERR_NULL, "null pointer");
+    if (n < 0 || nrows < 0 || nrows > ccsd_diis_max_rows() + 1 || (nrows > 1 && ldx < n))
+        return fail(CCSD_ERR_SHAPE, "bad dimensions");
+    hipError_t e = ccsd_launch_diis_combine(rows, ldx, nrows, coef, base, out, n, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e, "diis_combine_kernel");
     return CCSD_OK;
 }
 

@@ -422,6 +422,96 @@ __global__ void __launch_bounds__(256) l2_finish_kernel(const double* __restrict
     }
 }
 
+// DIIS with the history in FP32 (the mixed-precision lambda solve): one pass that
+// stores a new vector and its error vector and forms the error's overlaps with the
+// stored ones, and one pass that forms a combination of stored rows, both summing in
+// FP64. The new vector x is the concatenation of two FP64 segments (s0: n0 elements,
+// s1: the rest) plus an optional FP64 right-hand side.
+constexpr int kDiisMaxRows = 15;
+
+// x[k]; xs = f32(x) when given; prev_out = x when given; with prev: e = x - prev, es =
+// f32(e) when given, and the per-block partials of sum_k E[r][k] ef[k] for r < nrows (the
+// stored row r, or ef itself for r = slot) and of sum_k ef[k]^2 (ef = the stored e, or e
+// in full when es is NULL) into part[blockIdx.x][0..nrows].
+__global__ void __launch_bounds__(256) diis_push_kernel(const double* __restrict__ s0, const double* __restrict__ s1,
+                                                         long long n0, long long n, const double* __restrict__ rhs,
+                                                         const double* __restrict__ prev, float* __restrict__ xs,
+                                                         float* __restrict__ es, const float* __restrict__ E,
+                                                         long long lde, int nrows, int slot,
+                                                         double* __restrict__ prev_out, double* __restrict__ part)
+{
+    double acc[kDiisMaxRows + 1];
+#pragma unroll
+    for (int r = 0; r <= kDiisMaxRows; r++) acc[r] = 0.0;
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        double x = k < n0 ? s0[k] : s1[k - n0];
+        if (rhs) x += rhs[k];
+        if (xs) xs[k] = (float)x;
+        if (prev_out) prev_out[k] = x;
+        if (prev) {
+            const double e = x - prev[k];
+            double ef = e;
+            if (es) {
+                const float f = (float)e;
+                es[k] = f;
+                ef = (double)f;
+            }
+#pragma unroll
+            for (int r = 0; r < kDiisMaxRows; r++)
+                if (r < nrows) acc[r] += (r == slot ? ef : (double)E[(long long)r * lde + k]) * ef;
+            acc[kDiisMaxRows] += ef * ef;
+        }
+    }
+    if (!prev) return;
+    __shared__ double red[4][kDiisMaxRows + 1];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r <= kDiisMaxRows; r++) {
+        if (r < nrows || r == kDiisMaxRows) {
+            const double v = wave_sum(acc[r]);
+            if (lane == 0) red[wave][r] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x <= nrows) {
+        const int r = threadIdx.x < nrows ? threadIdx.x : kDiisMaxRows;
+        part[(long long)blockIdx.x * (nrows + 1) + threadIdx.x] = (red[0][r] + red[1][r]) + (red[2][r] + red[3][r]);
+    }
+}
+
+// out[j] = sum_b part[b][j] for j = blockIdx.x <= nrows: thread t adds blocks t, t + 256,
+// ..., then a fixed-order tree over the threads (deterministic).
+__global__ void __launch_bounds__(256) diis_dots_kernel(const double* __restrict__ part, int nblocks, int nrows,
+                                                         double* __restrict__ out)
+{
+    const int j = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) s += part[(long long)b * (nrows + 1) + j];
+    __shared__ double red[4];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[j] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+struct DiisCoef {
+    double c[kDiisMaxRows + 1];
+};
+
+// out[k] = base[k] (0 when NULL) + sum_r c[r] X[r][k], X FP32 rows of stride ldx.
+__global__ void __launch_bounds__(256) diis_combine_kernel(const float* __restrict__ X, long long ldx, int nrows,
+                                                            const DiisCoef c, const double* __restrict__ base,
+                                                            double* __restrict__ out, long long n)
+{
+    for (long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        double s = base ? base[k] : 0.0;
+#pragma unroll
+        for (int r = 0; r <= kDiisMaxRows; r++)
+            if (r < nrows) s += c.c[r] * (double)X[(long long)r * ldx + k];
+        out[k] = s;
+    }
+}
+
 }  // namespace
 
 hipError_t ccsd_launch_vvvv_pm_pack(const double* l2, int o, int v, const long long* iu, const long long* ju,
@@ -516,5 +606,40 @@ hipError_t ccsd_launch_pair_combine(const double* X, const long long* xs, int o,
     const long long nb = (long long)o * o * nt * nt;
     hipLaunchKernelGGL(pair_combine_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, xs[0], xs[1], xs[2], xs[3], o,
                        v, c1, c2, c3, c4, D, out, os[0], os[1], os[2], os[3], beta);
+    return hipGetLastError();
+}
+
+int ccsd_diis_max_rows() { return kDiisMaxRows; }
+
+// Blocks of the push pass: its partials need ws of blocks x (nrows + 1) doubles.
+long long ccsd_diis_push_blocks(long long n)
+{
+    long long nb = (n + 255) / 256;
+    return nb < 4096 ? (nb < 1 ? 1 : nb) : 4096;
+}
+
+hipError_t ccsd_launch_diis_push(const double* s0, const double* s1, long long n0, long long n, const double* rhs,
+                                 const double* prev, float* xs, float* es, const float* E, long long lde, int nrows,
+                                 int slot, double* prev_out, double* dots, double* ws, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const long long nb = ccsd_diis_push_blocks(n);
+    hipLaunchKernelGGL(diis_push_kernel, dim3((unsigned)nb), dim3(256), 0, stream, s0, s1, n0, n, rhs, prev, xs, es, E,
+                       lde, nrows, slot, prev_out, ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || !prev) return e;
+    hipLaunchKernelGGL(diis_dots_kernel, dim3((unsigned)(nrows + 1)), dim3(256), 0, stream, ws, (int)nb, nrows, dots);
+    return hipGetLastError();
+}
+
+hipError_t ccsd_launch_diis_combine(const float* X, long long ldx, int nrows, const double* coef, const double* base,
+                                    double* out, long long n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    DiisCoef c;
+    for (int r = 0; r <= kDiisMaxRows; r++) c.c[r] = r < nrows ? coef[r] : 0.0;
+    long long nb = (n + 255) / 256;
+    if (nb > 65536LL * 8) nb = 65536LL * 8;
+    hipLaunchKernelGGL(diis_combine_kernel, dim3((unsigned)nb), dim3(256), 0, stream, X, ldx, nrows, c, base, out, n);
     return hipGetLastError();
 }

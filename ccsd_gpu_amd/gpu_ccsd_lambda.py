@@ -473,7 +473,8 @@ def kernel_gpu(mycc, eris=None, t1=None, t2=None, l1=None, l2=None, max_cycle=No
         accept = (tol, margin, state): after each DIIS step, stop with DIIS's input
         combination (DIIS.input_combination) when its residual norm plus `margin` times
         sum |c| |d| is below tol; state["residual"] is then that norm."""
-        adiis = DIIS(diis_space) if use_diis else None
+        # low-precision sweeps keep the DIIS history in FP32 (half the bytes read per sweep)
+        adiis = DIIS(diis_space, store=None if prec == "fp64" else "fp32") if use_diis else None
         rhs_vec = amplitudes_to_vector(*rhs) if rhs is not None else None
         normt = None
         best, best_it = float("inf"), 0

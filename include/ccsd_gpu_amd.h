@@ -273,6 +273,27 @@ int ccsd_pair_combine(const double* x, const long long* xstrides, int o, int v, 
 int ccsd_l2_finish(const double* x, int o, int v, const double* u, const double* w, const double* s,
                    const double* sdev, double shost, const double* d, double* out, void* stream);
 
+/* DIIS with its history in FP32 (the mixed-precision lambda solve; the reference keeps
+ * PySCF's FP64 lib.diis on the host, ccsd_gpu/gpu_ccsd_lambda.py:376-399). The new
+ * vector x = concat(s0[0:n0], s1[0:n-n0])
+ * (+ rhs when given), FP64 device arrays. One pass: xs = f32(x) (when given),
+ * prev_out = x (when given); with prev: e = x - prev, es = f32(e) (when given) and
+ * dots[r] = sum_k E[r][k] ef[k] for r < nrows (row r of e_rows, stride lde, or ef itself
+ * for r = slot), dots[nrows] = sum_k ef[k]^2, ef = the stored e (or e when es is NULL),
+ * summed in FP64 in a fixed order; ws: ccsd_diis_workspace(n, nrows) doubles.
+ * nrows <= 15. */
+long long ccsd_diis_workspace(long long n, int nrows);
+int ccsd_diis_push(const double* s0, const double* s1, long long n0, long long n, const double* rhs, const double* prev,
+                   float* xs, float* es, const float* e_rows, long long lde, int nrows, int slot, double* prev_out,
+                   double* dots, double* ws, long long ws_len, void* stream);
+
+/* out[k] = base[k] (0 when NULL) + sum_{r < nrows} coef[r] rows[r][k]: FP32 rows of
+ * stride ldx, host coefficients, FP64 base and out; nrows <= 16. The DIIS
+ * extrapolation (base NULL) and input combination (base = the last extrapolation,
+ * coefficients negated, rows = the error vectors) over an FP32 history. */
+int ccsd_diis_combine(const float* rows, long long ldx, int nrows, const double* coef, const double* base, double* out,
+                      long long n, void* stream);
+
 /* Copy a strided view into a contiguous destination: ndim <= 6 dimensions in the
  * destination's order (sizes), with the source's element stride of each
  * (strides; any permutation). Writes float64 to dst, or, when dst is NULL,

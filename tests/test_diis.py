@@ -1,6 +1,7 @@
 """DIIS: the piecewise push (vector written straight into the history buffer) against
 the whole-vector update, and the change norm it returns."""
 import numpy as np
+import pytest
 import torch
 
 from ccsd_gpu_amd.cc import DIIS
@@ -60,3 +61,42 @@ def test_input_combination_residual_is_res_norm_for_an_affine_map():
                 checked += 1
             x = y
         assert checked >= 10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("space", [3, 6])
+def test_fp32_history_matches_fp64_on_device(space):
+    """The FP32-history DIIS (library push / combine passes) against the FP64 one on the
+    same device vectors (an affine map with a right-hand side, as the correction solves
+    run it): extrapolations, change norms, DIIS residual norms and input combinations
+    agree to FP32 rounding of the stored vectors; a second run repeats it bit for bit."""
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    g = torch.Generator(device="cuda").manual_seed(11)
+    n1, n2 = 37, 5005
+    n = n1 + n2
+    A = torch.randn(n, dtype=torch.float64, device="cuda", generator=g) * 0.4    # diagonal contraction
+    b = torch.randn(n, dtype=torch.float64, device="cuda", generator=g)
+    rhs = torch.randn(n, dtype=torch.float64, device="cuda", generator=g) * 0.1
+
+    def run(store):
+        d = DIIS(space, store=store)
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        outs = []
+        for it in range(10):
+            y = A * x + b
+            v, nrm = d.push((y[:n1], y[n1:].reshape(7, -1)), rhs, start=x if it == 0 else None)
+            comb = d.input_combination() if d.res_norm is not None else None
+            outs.append((v.clone(), nrm, d.res_norm, None if comb is None else comb.clone()))
+            x = v
+        return outs
+
+    ref, got, again = run(None), run("fp32"), run("fp32")
+    for (v0, n0, r0, c0), (v1, n1_, r1, c1), (v2, n2_, r2, c2) in zip(ref, got, again):
+        scale = float(v0.abs().max())
+        assert float((v1 - v0).abs().max()) <= 1e-5 * scale
+        assert abs(n1_ - n0) <= 1e-5 * n0
+        if r0 is not None:
+            assert abs(r1 - r0) <= 1e-4 * max(r0, 1e-6)
+            assert float((c1 - c0).abs().max()) <= 1e-5 * scale
+        assert torch.equal(v1, v2) and n1_ == n2_
