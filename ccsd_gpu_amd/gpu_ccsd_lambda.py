@@ -50,6 +50,20 @@ def _require_gpu():
         raise RuntimeError("No usable HIP device for the lambda solve (this build has no CPU fallback).")
 
 
+def _pair_sum(X, c1=0.0, c2=0.0, c3=0.0, c4=0.0):
+    """c1 X[i,j,a,b] + c2 X[i,j,b,a] + c3 X[j,i,a,b] + c4 X[j,i,b,a] as a new contiguous
+    tensor: the library's tiled pair kernel on the device (one pass; torch's strided
+    elementwise path ran 3-6x slower on these permuted sums), torch on the host."""
+    if X.is_cuda and X.dtype == torch.float64:
+        out = torch.empty(X.shape, dtype=X.dtype, device=X.device)     # contiguous whatever X's strides
+        return pair_combine(X, c1=c1, c2=c2, c3=c3, c4=c4, out=out)
+    out = torch.zeros_like(X, memory_format=torch.contiguous_format)
+    for c, perm in ((c1, (0, 1, 2, 3)), (c2, (0, 1, 3, 2)), (c3, (1, 0, 2, 3)), (c4, (1, 0, 3, 2))):
+        if c:
+            out += c * X.permute(perm)
+    return out
+
+
 def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=None):
     """The lambda intermediates (the reference's make_intermediates, gpu_ccsd_lambda.py:68-180).
 
@@ -78,7 +92,7 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
 
     w1 = w1 + C("jcba,jc->ba", ovvv, t1 * 2) - C("jabc,jc->ba", ovvv, t1)
-    theta = t2 * 2 - t2.permute(1, 0, 2, 3)
+    theta = _pair_sum(t2, c1=2.0, c3=-1.0)                 # 2 t2[i,j,a,b] - t2[j,i,a,b]
     w3 = w3 + C("jkcd,kdcb->bj", theta, ovvv)
     wVOov = C("jbcd,kd->bjkc", ovvv, t1)
     wvOOv = C("cbjd,kd->cjkb", vvov, -t1)
@@ -89,14 +103,14 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     wexp = CW("jabd,jkcd->abkc", ovvv, t2) * -1.5
     wcheap = mfma.combine([(2, vvov.permute(0, 3, 2, 1)), (-1, vvov)])
     g2vvov = mfma.combine([(2, vvov), (-1, ovvv.permute(1, 2, 0, 3))])
-    theta_b = t2 * 2 - t2.permute(0, 1, 3, 2)
+    theta_b = _pair_sum(t2, c1=2.0, c2=-1.0)               # 2 t2[i,j,a,b] - t2[i,j,b,a]
     vackb = CW("acjd,kjbd->ackb", g2vvov, theta_b)
     mfma.combine([(1, vackb.permute(0, 3, 2, 1)), (-0.5, vackb)], out=wexp, beta=1)
     del vackb
 
     ovoo = eris.ovoo
     w2 = w2 + C("kbij,kb->ij", ovoo, t1) * 2 - C("ibkj,kb->ij", ovoo, t1)
-    theta = t2.permute(1, 0, 2, 3) * 2 - t2
+    theta = _pair_sum(t2, c1=-1.0, c3=2.0)
     w3 = w3 - C("lckj,klcb->bj", ovoo, theta)
     tmp = C("lc,jcik->ijkl", t1, ovoo)
     woooo = tmp + tmp.permute(1, 0, 3, 2)
@@ -105,7 +119,7 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     wvooo = wvooo + ovoo.permute(1, 3, 2, 0) * 2 - ovoo.permute(1, 0, 2, 3)
     wvooo = wvooo - C("klbc,iblj->ckij", t2, ovoo * 1.5)
     g2ovoo = ovoo * 2 - ovoo.permute(2, 1, 0, 3)
-    theta = t2 * 2 - t2.permute(1, 0, 2, 3)
+    theta = _pair_sum(t2, c1=2.0, c3=-1.0)
     vcjik = C("jlcb,lbki->cjki", theta, g2ovoo)
     wvooo = wvooo + vcjik.permute(0, 3, 2, 1) - vcjik * 0.5
 
@@ -147,22 +161,20 @@ def make_intermediates_gpu(mycc, t1, t2, eris, precision="fp64", lp_precision=No
     if lp_precision is not None:
         imds.wvvov_cheap = wcheap
         imds.wvvov_fac = (g2vvov, theta_b, g2ovoo)
-    _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0))
+    _static_pieces(imds, t1, t2, eris, getattr(mycc, "level_shift", 0.0), theta=theta_b)
     return imds
 
 
-def _static_pieces(imds, t1, t2, eris, level_shift):
+def _static_pieces(imds, t1, t2, eris, level_shift, theta=None):
     """Amplitude-only pieces of the lambda update, fixed during a solve (the reference
     recomputes them in every update, gpu_ccsd_lambda.py:227-334)."""
     nocc = t1.shape[0]
-    imds.theta = (t2 * 2 - t2.permute(0, 1, 3, 2)).contiguous()
+    imds.theta = theta if theta is not None else _pair_sum(t2, c1=2.0, c2=-1.0)   # 2 t2 - t2[i,j,b,a]
     imds.tau = t2 + torch.einsum("ia,jb->ijab", t1, t1)
     imds.tmp_voov = (imds.wVOov * 2 + imds.wvOOv).contiguous()
     # the bare integral term of l2new, voov.permute(1, 2, 0, 3) / 2, as the closing
     # symmetrisation sees it: c[i,j,a,b] + c[j,i,b,a]
-    l2c = eris.ovvo.permute(0, 3, 1, 2) * 0.5
-    imds.l2_sym = (l2c + l2c.permute(1, 0, 3, 2)).contiguous()
-    del l2c
+    imds.l2_sym = _pair_sum(eris.ovvo.permute(0, 3, 1, 2), c1=0.5, c4=0.5)
     # integral combinations that two matrix-vector terms of every update read in two index
     # orders: one combined pass instead of two
     ovvv = eris.ovvv
